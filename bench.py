@@ -1,0 +1,116 @@
+"""Headline benchmark: plain HMC of the Lennard-Jones liquid at N = 32768, rho* = 0.8442, T* = 0.722,
+rc = 2.5, 2nd-order Omelyan (lambda = 0.1932) with 10 steps of dt = 0.01, FP32 pair forces with FP64
+positions / momenta / dH, on one MI355X.
+
+A "step" is one HMC trajectory (heat bath, 10 OMF2 steps = 21 force evaluations, FP64 V_f, K_f,
+Metropolis, restore-on-reject), replayed from one captured hipGraph.  The timed region starts with
+the state resident in HBM and ends after the device has finished the K-th trajectory.
+
+  value      atom-steps/s = N * 10 MD steps * K trajectories / seconds
+  ess_per_s  ESS/s of U/N over the timed trajectories (Sokal-windowed tau_int)
+  roofline   the dominant kernel, the FP32 LJ sweep: algorithmic flops per launch (8 N_cand + 13 N_int
+             with N_cand, N_int counted on the device for the current configuration) / its average
+             duration measured with HIP events on the context's stream
+  cpu_baseline  the FP64 C oracle (a port of the reference algorithm, cpu-opt binning) timed on this
+             host's cores for a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, RHO, T, RC = 32768, 0.8442, 0.722, 2.5
+DT, NSTEPS, LAM = 0.01, 10, 0.1932
+PEAK_FP32_TFLOPS = 157.3        # MI355X vector FP32 (vendor figure; packed FP32)
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_baseline(seconds_budget=20.0):
+    """FP64 oracle (OpenMP) on the host: OMF2 trajectories of the same configuration until ~budget."""
+    import oracle
+    L = (N / RHO) ** (1 / 3)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    s = oracle.Sim(N, [L] * 3, cutoff=RC, beta=1 / T)
+    s.x[:] = oracle.lattice_init(N, [L] * 3, 1, 0, 0.05)
+    ntraj = 0
+    t0 = time.perf_counter()
+    while True:
+        s.hmc(1, 0, oracle.OMF2, DT, NSTEPS, lam=LAM, seed=1 + ntraj)
+        ntraj += 1
+        el = time.perf_counter() - t0
+        if el > seconds_budget or ntraj >= 50:
+            break
+    return {"value": N * NSTEPS * ntraj / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{ntraj} OMF2 trajectories of the N=32768 workload (FP64 oracle, O(N) binning), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000, help="timed HMC trajectories")
+    ap.add_argument("--warmup", type=int, default=100, help="untimed trajectories (also thermalisation)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import chemhmc_amd as chmc
+    from chemhmc_amd import stats
+
+    L = (N / RHO) ** (1 / 3)
+    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
+    e.init_lattice(1, 0.05)
+    e.set_force_tiling(0)                       # autotune the sweep tiling on this configuration
+    islots = e.force_tiling()
+    if args.warmup > 0:
+        e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=True)
+
+    t0 = time.perf_counter()
+    rec, nacc = e.hmc(args.steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=True)
+    el = time.perf_counter() - t0
+    ms = el / args.steps * 1e3
+    value = N * NSTEPS * args.steps / el
+
+    u = rec["u"][:, 0] / N
+    tau, _ = stats.tau_int(u)
+    ess = args.steps / (2 * tau)
+
+    # dominant kernel: FP32 LJ sweep; algorithmic flops of this configuration
+    ncand, nint = e.pair_counts(0)
+    flops = 8.0 * ncand + 13.0 * nint
+    ms_bin, ms_force = e.time_force(reps=50)
+    achieved = flops / (ms_force * 1e-3) / 1e12
+
+    out = {
+        "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N",
+        "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair forces, f64 x/p/dH", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
+        "config": {"workload": "plain HMC N=32768 rho=0.8442 T=0.722 rc=2.5 OMF2(lambda=0.1932) 10x0.01",
+                   "N": N, "trajectories": args.steps, "md_steps_per_traj": NSTEPS, "force_evals_per_traj": 2 * NSTEPS + 1,
+                   "force_tiling_islots": islots},
+        "acceptance": float(nacc[0]) / args.steps,
+        "u_per_n": float(u.mean()), "tau_int_u": tau, "ess_per_s": ess / el,
+        "roofline": {"bound": "valu", "kernel": "k_lj<float> force+kick sweep", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": None, "flops_per_launch": flops, "ms_per_launch": ms_force,
+                     "n_cand_per_atom": ncand / N, "n_int_per_atom": nint / N, "ms_binning": ms_bin},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,311 @@
+"""chemhmc_amd -- MI355X-native Lennard-Jones HMC engine (Python front-end of libchmc.so).
+
+Everything below calls the C ABI of ``include/chmc.h`` through ctypes; there is no Python or CPU
+compute path.  Loading fails loudly when ``libchmc.so`` has not been built or when no GPU is
+visible.  Arrays cross the boundary as numpy float64 (N, 3) C-contiguous (AoS per particle, the
+layout of the reference's ``config<coordinate>``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libchmc.so")
+
+LEAPFROG, OMF4, EULER, RUTH, OMF2 = 0, 2, 4, 5, 6
+FORCE_FP32, FORCE_FP64, ORDER_ATOM_ID = 0, 1, 2
+OMF2_LAMBDA_REFERENCE = 0.1938       # code/modules/integrator.cpp:72
+
+_D, _I, _U64 = C.c_double, C.c_int, C.c_uint64
+_PD = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_PI = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_PU64 = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+
+class ChainDesc(C.Structure):
+    _fields_ = [("N", _I), ("L", _D * 3), ("cutoff", _D), ("sigma", _D), ("eps", _D), ("mass", _D), ("beta", _D)]
+
+
+class HmcParams(C.Structure):
+    _fields_ = [("ntraj", _I), ("therm", _I), ("scheme", _I), ("steps", _I), ("dt", _D), ("lam", _D),
+                ("seed", _U64), ("use_graph", _I), ("randomize", _I)]
+
+
+HMC_RECORD_DTYPE = np.dtype([("dh", "f8"), ("u", "f8"), ("k", "f8"), ("w", "f8"), ("accepted", "i4"), ("pad", "i4")])
+
+
+class ChmcError(RuntimeError):
+    pass
+
+
+def build(verbose: bool = False) -> None:
+    """Compile libchmc.so for gfx950 with hipcc (in-tree)."""
+    out = None if verbose else subprocess.DEVNULL
+    subprocess.run(["make", "-s", "-C", _HERE, "-j8", "libchmc.so"], check=True, stdout=out)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ChmcError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    P = C.c_void_p
+    sig = {
+        "chmc_create": ([C.POINTER(P), _I, C.POINTER(ChainDesc), _I, _I], _I),
+        "chmc_destroy": ([P], _I),
+        "chmc_last_error": ([P], C.c_char_p),
+        "chmc_sync": ([P], _I),
+        "chmc_get_geometry": ([P, _I, C.POINTER(_I), C.POINTER(_D), C.POINTER(_I)], _I),
+        "chmc_num_atoms": ([P, C.POINTER(_I), C.POINTER(_I)], _I),
+        "chmc_set_positions": ([P, _I, _PD], _I),
+        "chmc_get_positions": ([P, _I, _PD], _I),
+        "chmc_set_momenta": ([P, _I, _PD], _I),
+        "chmc_get_momenta": ([P, _I, _PD], _I),
+        "chmc_init_lattice": ([P, _U64, _D], _I),
+        "chmc_bin": ([P], _I),
+        "chmc_get_bins": ([P, _I, _PI, _PI, _PI], _I),
+        "chmc_compute_force": ([P, _I], _I),
+        "chmc_get_force": ([P, _I, _PD], _I),
+        "chmc_compute_potential": ([P, _PD], _I),
+        "chmc_evaluate_potential": ([P, _PD], _I),
+        "chmc_virial": ([P, _PD], _I),
+        "chmc_kinetic": ([P, _PD], _I),
+        "chmc_pair_counts": ([P, _I, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], _I),
+        "chmc_update_momenta": ([P, _D], _I),
+        "chmc_update_positions": ([P, _D], _I),
+        "chmc_kick": ([P, _D], _I),
+        "chmc_integrate": ([P, _I, _D, _I, _D], _I),
+        "chmc_heatbath": ([P, _U64, _U64], _I),
+        "chmc_save_positions": ([P], _I),
+        "chmc_restore_positions": ([P], _I),
+        "chmc_hmc_run": ([P, C.POINTER(HmcParams), C.c_void_p, C.c_void_p], _I),
+        "chmc_rdf": ([P, _I, _I, _D, C.c_void_p, C.c_void_p], _I),
+        "chmc_potential_energy_open": ([_PD, _I, _I, _D, _D, C.POINTER(_D)], _I),
+        "chmc_set_force_tiling": ([P, _I], _I),
+        "chmc_get_force_tiling": ([P, C.POINTER(_I)], _I),
+        "chmc_time_force": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+EXPORTED = (
+    "chmc_create", "chmc_destroy", "chmc_last_error", "chmc_sync", "chmc_get_geometry", "chmc_num_atoms",
+    "chmc_set_positions", "chmc_get_positions", "chmc_set_momenta", "chmc_get_momenta", "chmc_init_lattice",
+    "chmc_bin", "chmc_get_bins", "chmc_compute_force", "chmc_get_force", "chmc_compute_potential",
+    "chmc_evaluate_potential", "chmc_virial", "chmc_pair_counts", "chmc_kinetic", "chmc_update_momenta", "chmc_update_positions",
+    "chmc_kick", "chmc_integrate", "chmc_heatbath", "chmc_save_positions", "chmc_restore_positions",
+    "chmc_hmc_run", "chmc_rdf", "chmc_potential_energy_open", "chmc_set_force_tiling",
+    "chmc_get_force_tiling", "chmc_time_force",
+)
+
+
+@dataclass
+class Chain:
+    """One LJ system (particles.{N,mass,beta,cutoff,eps,sigma}, geometry.{Lx,Ly,Lz})."""
+    N: int
+    L: float | Sequence[float]
+    cutoff: float = 2.5
+    sigma: float = 1.0
+    eps: float = 1.0
+    mass: float = 1.0
+    beta: float = 1.0
+
+    def box(self):
+        return tuple(float(v) for v in np.broadcast_to(np.asarray(self.L, dtype=np.float64), (3,)))
+
+    def to_c(self):
+        return ChainDesc(int(self.N), (_D * 3)(*self.box()), self.cutoff, self.sigma, self.eps, self.mass, self.beta)
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Engine:
+    """A context of one or more chains on one GPU (all calls act on every chain at once)."""
+
+    def __init__(self, chains: Sequence[Chain] | Chain, device: int = -1, fp64_forces: bool = False,
+                 atom_id_order: bool = False):
+        if isinstance(chains, Chain):
+            chains = [chains]
+        self.chains = list(chains)
+        arr = (ChainDesc * len(self.chains))(*[c.to_c() for c in self.chains])
+        flags = (FORCE_FP64 if fp64_forces else 0) | (ORDER_ATOM_ID if atom_id_order else 0)
+        self._h = C.c_void_p()
+        rc = lib().chmc_create(C.byref(self._h), len(self.chains), arr, int(device), flags)
+        if rc != 0:
+            raise ChmcError(f"chmc_create failed ({rc}): {lib().chmc_last_error(None).decode()}")
+
+    # -- plumbing ----------------------------------------------------------------------------
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise ChmcError(f"{what} failed ({rc}): {lib().chmc_last_error(self._h).decode()}")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().chmc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def nchains(self):
+        return len(self.chains)
+
+    def geometry(self, chain=0):
+        nb, sb, bt = (_I * 3)(), (_D * 3)(), _I()
+        self._ck(lib().chmc_get_geometry(self._h, chain, nb, sb, C.byref(bt)), "get_geometry")
+        return [nb[i] for i in range(3)], [sb[i] for i in range(3)], bt.value
+
+    # -- state ---------------------------------------------------------------------------------
+    def set_positions(self, x, chain=0):
+        self._ck(lib().chmc_set_positions(self._h, chain, _c(x)), "set_positions")
+
+    def positions(self, chain=0):
+        out = np.zeros((self.chains[chain].N, 3))
+        self._ck(lib().chmc_get_positions(self._h, chain, out), "get_positions")
+        return out
+
+    def set_momenta(self, p, chain=0):
+        self._ck(lib().chmc_set_momenta(self._h, chain, _c(p)), "set_momenta")
+
+    def momenta(self, chain=0):
+        out = np.zeros((self.chains[chain].N, 3))
+        self._ck(lib().chmc_get_momenta(self._h, chain, out), "get_momenta")
+        return out
+
+    def init_lattice(self, seed, jitter=0.05):
+        self._ck(lib().chmc_init_lattice(self._h, int(seed), float(jitter)), "init_lattice")
+
+    # -- hot path ------------------------------------------------------------------------------
+    def bin(self):
+        self._ck(lib().chmc_bin(self._h), "bin")
+
+    def bins(self, chain=0):
+        _, _, bt = self.geometry(chain)
+        cnt = np.zeros(bt, np.int32)
+        off = np.zeros(bt, np.int32)
+        perm = np.zeros(self.chains[chain].N, np.int32)
+        self._ck(lib().chmc_get_bins(self._h, chain, cnt, off, perm), "get_bins")
+        return perm, cnt, off
+
+    def compute_force(self, fp64=False):
+        self._ck(lib().chmc_compute_force(self._h, int(bool(fp64))), "compute_force")
+
+    def force(self, chain=0):
+        out = np.zeros((self.chains[chain].N, 3))
+        self._ck(lib().chmc_get_force(self._h, chain, out), "get_force")
+        return out
+
+    def compute_potential(self):
+        out = np.zeros(self.nchains)
+        self._ck(lib().chmc_compute_potential(self._h, out), "compute_potential")
+        return out
+
+    def evaluate_potential(self):
+        out = np.zeros(self.nchains)
+        self._ck(lib().chmc_evaluate_potential(self._h, out), "evaluate_potential")
+        return out
+
+    def virial(self):
+        out = np.zeros(self.nchains)
+        self._ck(lib().chmc_virial(self._h, out), "virial")
+        return out
+
+    def pair_counts(self, chain=0):
+        nc, ni = C.c_longlong(), C.c_longlong()
+        self._ck(lib().chmc_pair_counts(self._h, chain, C.byref(nc), C.byref(ni)), "pair_counts")
+        return nc.value, ni.value
+
+    def kinetic(self):
+        out = np.zeros(self.nchains)
+        self._ck(lib().chmc_kinetic(self._h, out), "kinetic")
+        return out
+
+    def update_momenta(self, dt):
+        self._ck(lib().chmc_update_momenta(self._h, float(dt)), "update_momenta")
+
+    def update_positions(self, dt):
+        self._ck(lib().chmc_update_positions(self._h, float(dt)), "update_positions")
+
+    def kick(self, dt):
+        self._ck(lib().chmc_kick(self._h, float(dt)), "kick")
+
+    def integrate(self, scheme, dt, steps, lam=OMF2_LAMBDA_REFERENCE):
+        self._ck(lib().chmc_integrate(self._h, int(scheme), float(dt), int(steps), float(lam)), "integrate")
+
+    def heatbath(self, seed, traj):
+        self._ck(lib().chmc_heatbath(self._h, int(seed), int(traj)), "heatbath")
+
+    def save_positions(self):
+        self._ck(lib().chmc_save_positions(self._h), "save_positions")
+
+    def restore_positions(self):
+        self._ck(lib().chmc_restore_positions(self._h), "restore_positions")
+
+    def hmc(self, ntraj, therm, scheme, dt, steps, lam=OMF2_LAMBDA_REFERENCE, seed=1, use_graph=True,
+            randomize=False):
+        hp = HmcParams(int(ntraj), int(therm), int(scheme), int(steps), float(dt), float(lam), int(seed),
+                       int(bool(use_graph)), int(bool(randomize)))
+        rec = np.zeros(ntraj * self.nchains, HMC_RECORD_DTYPE)
+        nacc = np.zeros(self.nchains, np.int32)
+        self._ck(lib().chmc_hmc_run(self._h, C.byref(hp), rec.ctypes.data, nacc.ctypes.data), "hmc_run")
+        return rec.reshape(ntraj, self.nchains), nacc
+
+    def rdf(self, nbins, size_bin, chain=0):
+        counts = np.zeros(nbins, np.uint64)
+        g = np.zeros(nbins)
+        self._ck(lib().chmc_rdf(self._h, chain, int(nbins), float(size_bin), counts.ctypes.data, g.ctypes.data), "rdf")
+        return counts, g
+
+    # -- tuning --------------------------------------------------------------------------------
+    def set_force_tiling(self, islots):
+        self._ck(lib().chmc_set_force_tiling(self._h, int(islots)), "set_force_tiling")
+
+    def force_tiling(self):
+        v = _I()
+        self._ck(lib().chmc_get_force_tiling(self._h, C.byref(v)), "get_force_tiling")
+        return v.value
+
+    def time_force(self, reps=20):
+        b, f = _D(), _D()
+        self._ck(lib().chmc_time_force(self._h, int(reps), C.byref(b), C.byref(f)), "time_force")
+        return b.value, f.value
+
+
+def potential_energy_open(x, n=12, a=1e-12, b=1e-6):
+    """code-carsten potential_energy() on the GPU (open boundaries, all pairs)."""
+    out = _D()
+    rc = lib().chmc_potential_energy_open(_c(x), int(np.asarray(x).shape[0]), int(n), float(a), float(b), C.byref(out))
+    if rc != 0:
+        raise ChmcError(f"potential_energy_open failed ({rc}): {lib().chmc_last_error(None).decode()}")
+    return out.value

@@ -1,0 +1,967 @@
+// Host side of libchmc: context, state transfer, sequencing of the hot path, the HMC loop (direct
+// or hipGraph-captured), the autotuner of the LJ sweep tiling, and the C ABI of include/chmc.h.
+#include "chmc.h"
+#include "chmc_internal.hpp"
+#include "philox.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#pragma clang fp contract(off)   // the host lattice generator must round like the FP64 oracle
+
+using namespace chmc;
+
+static_assert(sizeof(chmc_hmc_record) == sizeof(HmcRecord), "record layout");
+
+struct chmc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int flags = 0;
+    int nchains = 0, Ntot = 0, Ctot = 0, nchunks = 0;
+    std::vector<chmc_chain_desc> desc;
+    std::vector<ChainDev> hchains;
+    std::vector<Chunk> hchunks;
+    DevView v{};
+    ChainDev* d_chains = nullptr;
+    std::vector<void*> allocs;
+    bool bins_valid = false;
+    int force_kind = 0;          // 0 none, 1 fp32 stored, 2 fp64 stored
+    int islots = 8;
+    double pending_drift = 0.0;  // drift deferred into the next binning (integrator sequencing)
+    int traj_done = 0;           // trajectories completed on the device counter
+    std::string err;
+    double* d_uniforms = nullptr;
+    size_t uniforms_cap = 0;
+    HmcRecord* d_rec = nullptr;
+    size_t rec_cap = 0;
+    unsigned long long* d_rdf = nullptr;
+    int rdf_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+};
+
+static thread_local std::string g_create_error;
+
+static int fail(chmc_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                            \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess) return fail((ctx), CHMC_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK_CTX(ctx) do { if (!(ctx)) return CHMC_E_ARG; } while (0)
+
+template <typename T>
+static hipError_t dalloc(chmc_ctx* c, T** p, size_t n)
+{
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e == hipSuccess) {
+        c->allocs.push_back(q);
+        *p = static_cast<T*>(q);
+    }
+    return e;
+}
+
+// identical_particles_binning.cpp:4-29
+static int cutoff_binning(const double L[3], double cutoff, int nbin[3], double sizebin[3])
+{
+    for (int i = 0; i < 3; i++) {
+        if (!(L[i] > cutoff)) return -1;
+        nbin[i] = (int)floor(L[i] / cutoff);
+        sizebin[i] = L[i] / ((double)nbin[i]);
+        if (sizebin[i] < cutoff) return -2;
+    }
+    return 0;
+}
+
+static void destroy_ctx(chmc_ctx* c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->d_uniforms) hipFree(c->d_uniforms);
+    if (c->d_rec) hipFree(c->d_rec);
+    if (c->d_rdf) hipFree(c->d_rdf);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->ev2) hipEventDestroy(c->ev2);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int chmc_create(chmc_ctx** out, int nchains, const chmc_chain_desc* chains, int device, int flags)
+{
+    if (!out || nchains < 1 || !chains) return fail(nullptr, CHMC_E_ARG, "chmc_create: bad arguments");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, CHMC_E_NODEV, "no HIP device");
+    chmc_ctx* c = new chmc_ctx();
+    if (device < 0) hipGetDevice(&device);
+    if (device >= ndev) { delete c; return fail(nullptr, CHMC_E_ARG, "device index out of range"); }
+    c->device = device;
+    c->flags = flags;
+    c->nchains = nchains;
+    c->desc.assign(chains, chains + nchains);
+    // ---- per-chain geometry and offsets -------------------------------------------------------
+    long long natoms = 0, ncells = 0;
+    for (int k = 0; k < nchains; k++) {
+        const chmc_chain_desc& d = chains[k];
+        if (d.N < 1 || !(d.cutoff > 0) || !(d.mass > 0) || !(d.beta > 0) || !(d.sigma > 0)) {
+            delete c;
+            return fail(nullptr, CHMC_E_ARG, "chain " + std::to_string(k) + ": N, cutoff, sigma, mass and beta must be positive");
+        }
+        ChainDev cd{};
+        if (cutoff_binning(d.L, d.cutoff, cd.nbin, cd.sizebin) != 0) {
+            delete c;
+            return fail(nullptr, CHMC_E_GEOMETRY, "chain " + std::to_string(k) + ": every box dimension must exceed the cutoff");
+        }
+        cd.atom_off = (int)natoms;
+        cd.natoms = d.N;
+        cd.cell_off = (int)ncells;
+        cd.ncells = cd.nbin[0] * cd.nbin[1] * cd.nbin[2];
+        for (int i = 0; i < 3; i++) {
+            cd.L[i] = d.L[i];
+            cd.sizebin_f[i] = (float)cd.sizebin[i];
+        }
+        cd.rc = d.cutoff; cd.rc2 = d.cutoff * d.cutoff;
+        cd.sigma = d.sigma; cd.sigma2 = d.sigma * d.sigma;
+        cd.eps = d.eps; cd.mass = d.mass; cd.beta = d.beta;
+        cd.rc2_f = (float)cd.rc2; cd.sigma2_f = (float)cd.sigma2; cd.eps48_f = (float)(48 * d.eps);
+        cd.chunk_first = (int)c->hchunks.size();
+        for (int b = 0; b < d.N; b += kRedChunk) c->hchunks.push_back(Chunk{ k, cd.atom_off + b, cd.atom_off + std::min(d.N, b + kRedChunk), 0 });
+        cd.nchunks = (int)c->hchunks.size() - cd.chunk_first;
+        natoms += d.N;
+        ncells += cd.ncells;
+        c->hchains.push_back(cd);
+    }
+    if (natoms > (1LL << 30) || ncells > (1LL << 28)) { delete c; return fail(nullptr, CHMC_E_ARG, "batch too large"); }
+    c->Ntot = (int)natoms;
+    c->Ctot = (int)ncells;
+    c->nchunks = (int)c->hchunks.size();
+    const int N = c->Ntot, Cn = c->Ctot;
+
+    auto bail = [&](const char* what, hipError_t e) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        destroy_ctx(c);
+        return fail(nullptr, CHMC_E_HIP, m);
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return bail("hipSetDevice", e);
+    if ((e = hipStreamCreate(&c->stream)) != hipSuccess) return bail("hipStreamCreate", e);
+    hipEventCreate(&c->ev0); hipEventCreate(&c->ev1); hipEventCreate(&c->ev2);
+    DevView& v = c->v;
+    v.Ntot = N; v.Ctot = Cn; v.nchains = nchains; v.nchunks = c->nchunks;
+    int *atom_chain, *cell_chain;
+    Chunk* chunks;
+    if ((e = dalloc(c, &c->d_chains, nchains)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &atom_chain, N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &cell_chain, Cn)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &chunks, c->nchunks)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.x, 3 * (size_t)N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.p, 3 * (size_t)N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.x_saved, 3 * (size_t)N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.f64, 3 * (size_t)N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.f32, 3 * (size_t)N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.cell_of, N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.slot, N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.count, Cn)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.offs, Cn + 1)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.skey, N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.satom, N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.perm, N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.invperm, N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.xs, N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.red, (size_t)nchains * RED_NSLOT)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.chunk_part, (size_t)c->nchunks)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.cell_part, 3 * (size_t)Cn)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.ticket, nchains)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.state, nchains)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.traj, 1)) != hipSuccess) return bail("hipMalloc", e);
+    v.chains = c->d_chains; v.atom_chain = atom_chain; v.cell_chain = cell_chain; v.chunks = chunks;
+
+    std::vector<int> ha(N), hc(Cn), iota(N);
+    for (int k = 0; k < nchains; k++) {
+        const ChainDev& cd = c->hchains[k];
+        for (int i = 0; i < cd.natoms; i++) ha[cd.atom_off + i] = k;
+        for (int i = 0; i < cd.ncells; i++) hc[cd.cell_off + i] = k;
+    }
+    for (int i = 0; i < N; i++) iota[i] = i;
+    if ((e = hipMemcpy(c->d_chains, c->hchains.data(), sizeof(ChainDev) * nchains, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
+    if ((e = hipMemcpy(atom_chain, ha.data(), sizeof(int) * N, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
+    if ((e = hipMemcpy(cell_chain, hc.data(), sizeof(int) * Cn, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
+    if ((e = hipMemcpy(chunks, c->hchunks.data(), sizeof(Chunk) * c->nchunks, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
+    // permute_vector = identity (parallel_binning_init, identical_particles_binning.cpp:37-44)
+    if ((e = hipMemcpy(v.perm, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
+    if ((e = hipMemcpy(v.invperm, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
+    if ((e = hipMemset(v.count, 0, sizeof(int) * Cn)) != hipSuccess) return bail("hipMemset", e);
+    if ((e = hipMemset(v.ticket, 0, sizeof(int) * nchains)) != hipSuccess) return bail("hipMemset", e);
+    if ((e = hipMemset(v.traj, 0, sizeof(int))) != hipSuccess) return bail("hipMemset", e);
+    if ((e = hipMemset(v.state, 0, sizeof(ChainState) * nchains)) != hipSuccess) return bail("hipMemset", e);
+    if ((e = hipMemset(v.red, 0, sizeof(double) * nchains * RED_NSLOT)) != hipSuccess) return bail("hipMemset", e);
+    if ((e = hipMemset(v.x, 0, sizeof(double) * 3 * (size_t)N)) != hipSuccess) return bail("hipMemset", e);
+    if ((e = hipMemset(v.p, 0, sizeof(double) * 3 * (size_t)N)) != hipSuccess) return bail("hipMemset", e);
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return bail("hipDeviceSynchronize", e);
+    *out = c;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_destroy(chmc_ctx* c)
+{
+    CHECK_CTX(c);
+    destroy_ctx(c);
+    return CHMC_OK;
+}
+
+extern "C" const char* chmc_last_error(const chmc_ctx* c)
+{
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" int chmc_sync(chmc_ctx* c)
+{
+    CHECK_CTX(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+extern "C" int chmc_get_geometry(const chmc_ctx* c, int chain, int nbin[3], double sizebin[3], int* bintot)
+{
+    if (!c || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    const ChainDev& cd = c->hchains[chain];
+    for (int i = 0; i < 3; i++) {
+        if (nbin) nbin[i] = cd.nbin[i];
+        if (sizebin) sizebin[i] = cd.sizebin[i];
+    }
+    if (bintot) *bintot = cd.ncells;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_num_atoms(const chmc_ctx* c, int* total, int* nchains)
+{
+    CHECK_CTX(c);
+    if (total) *total = c->Ntot;
+    if (nchains) *nchains = c->nchains;
+    return CHMC_OK;
+}
+
+// ---- state transfer ------------------------------------------------------------------------------
+static int put_soa(chmc_ctx* c, double* dbase, int chain, const double* aos)
+{
+    const ChainDev& cd = c->hchains[chain];
+    const int n = cd.natoms;
+    std::vector<double> h(3 * (size_t)n);
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) h[(size_t)d * n + i] = aos[3 * i + d];
+    for (int d = 0; d < 3; d++)
+        HIPCHK(c, hipMemcpyAsync(dbase + (size_t)d * c->Ntot + cd.atom_off, h.data() + (size_t)d * n,
+                                 sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+template <typename T>
+static int get_soa(chmc_ctx* c, const T* dbase, int chain, double* aos)
+{
+    const ChainDev& cd = c->hchains[chain];
+    const int n = cd.natoms;
+    std::vector<T> h(3 * (size_t)n);
+    for (int d = 0; d < 3; d++)
+        HIPCHK(c, hipMemcpyAsync(h.data() + (size_t)d * n, dbase + (size_t)d * c->Ntot + cd.atom_off,
+                                 sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) aos[3 * i + d] = (double)h[(size_t)d * n + i];
+    return CHMC_OK;
+}
+
+static int flush_drift(chmc_ctx* c);
+
+extern "C" int chmc_set_positions(chmc_ctx* c, int chain, const double* xyz)
+{
+    if (!c || !xyz || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    const ChainDev& cd = c->hchains[chain];
+    // check_in_volume (identical_particles.cpp:228-236): every coordinate in [0, L)
+    for (int i = 0; i < cd.natoms; i++)
+        for (int d = 0; d < 3; d++) {
+            const double q = xyz[3 * i + d];
+            if (!(q >= 0.0 && q < cd.L[d]))
+                return fail(c, CHMC_E_ARG, "particle " + std::to_string(i) + " outside the box in dimension " + std::to_string(d));
+        }
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    c->bins_valid = false;
+    c->force_kind = 0;
+    return put_soa(c, c->v.x, chain, xyz);
+}
+
+extern "C" int chmc_get_positions(chmc_ctx* c, int chain, double* xyz)
+{
+    if (!c || !xyz || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    return get_soa(c, c->v.x, chain, xyz);
+}
+
+extern "C" int chmc_set_momenta(chmc_ctx* c, int chain, const double* pxyz)
+{
+    if (!c || !pxyz || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    return put_soa(c, c->v.p, chain, pxyz);
+}
+
+extern "C" int chmc_get_momenta(chmc_ctx* c, int chain, double* pxyz)
+{
+    if (!c || !pxyz || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    return get_soa(c, c->v.p, chain, pxyz);
+}
+
+static int lattice_side(int N)
+{
+    int n = (int)llround(cbrt((double)N));
+    if (n < 1) n = 1;
+    while ((long long)n * n * n < N) n++;
+    while (n > 1 && (long long)(n - 1) * (n - 1) * (n - 1) >= N) n--;
+    return n;
+}
+
+extern "C" int chmc_init_lattice(chmc_ctx* c, uint64_t seed, double jitter)
+{
+    CHECK_CTX(c);
+    if (!(jitter >= 0.0)) return CHMC_E_ARG;
+    for (int k = 0; k < c->nchains; k++) {
+        const ChainDev& cd = c->hchains[k];
+        const int n = lattice_side(cd.natoms);
+        std::vector<double> x(3 * (size_t)cd.natoms);
+        for (int i = 0; i < cd.natoms; i++) {
+            const int idx[3] = { i % n, (i / n) % n, i / (n * n) };
+            const u64x4 r = philox4x64_10((uint64_t)i, (uint64_t)k, 0, 0, seed, kKeyLattice);
+            for (int d = 0; d < 3; d++) {
+                const double a = cd.L[d] / (double)n;
+                const double site = ((double)idx[d] + 0.5) * a;
+                const double u = u01_co(r.v[d]);
+                double q = site + (2.0 * u - 1.0) * jitter;
+                q = q - cd.L[d] * floor(q / cd.L[d]);
+                x[3 * (size_t)i + d] = q;
+            }
+        }
+        int rc = chmc_set_positions(c, k, x.data());
+        if (rc) return rc;
+    }
+    return CHMC_OK;
+}
+
+// ---- hot-path sequencing ------------------------------------------------------------------------
+static bool fp64_md(const chmc_ctx* c) { return (c->flags & CHMC_FORCE_FP64) != 0; }
+static bool id_order(const chmc_ctx* c) { return (c->flags & CHMC_ORDER_ATOM_ID) != 0; }
+
+static int flush_drift(chmc_ctx* c)
+{
+    if (c->pending_drift != 0.0) {
+        HIPCHK(c, launch_drift(c->v, c->pending_drift, c->stream));
+        c->pending_drift = 0.0;
+        c->bins_valid = false;
+    }
+    return CHMC_OK;
+}
+
+// binning of the current positions, with a deferred drift applied first inside the same kernel
+static int do_bin(chmc_ctx* c)
+{
+    HIPCHK(c, launch_bin(c->v, c->pending_drift, id_order(c), c->stream));
+    c->pending_drift = 0.0;
+    c->bins_valid = true;
+    return CHMC_OK;
+}
+
+static int do_drift(chmc_ctx* c, double dt)
+{
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    c->pending_drift = dt;
+    c->bins_valid = false;
+    c->force_kind = 0;
+    return CHMC_OK;
+}
+
+// update_momenta: force with fresh bins, then the kick, in one sweep
+static int do_force_kick(chmc_ctx* c, double dt, bool storef)
+{
+    int rc = do_bin(c);
+    if (rc) return rc;
+    const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0);
+    HIPCHK(c, launch_lj(c->v, mode, fp64_md(c), c->islots, dt, c->stream));
+    if (storef) c->force_kind = fp64_md(c) ? 2 : 1;
+    return CHMC_OK;
+}
+
+static int do_energy(chmc_ctx* c, bool rebin)
+{
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    if (rebin || !c->bins_valid) {
+        rc = do_bin(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, launch_lj(c->v, LJ_ENERGY, true, c->islots, 0.0, c->stream));
+    return CHMC_OK;
+}
+
+static int read_red(chmc_ctx* c, int slot, double* out)
+{
+    std::vector<double> h((size_t)c->nchains * RED_NSLOT);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->v.red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->nchains; k++) out[k] = h[(size_t)k * RED_NSLOT + slot];
+    return CHMC_OK;
+}
+
+extern "C" int chmc_bin(chmc_ctx* c)
+{
+    CHECK_CTX(c);
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    rc = do_bin(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+extern "C" int chmc_get_bins(chmc_ctx* c, int chain, int* bincount, int* binoffsets, int* permute_vector)
+{
+    if (!c || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    if (!c->bins_valid) return fail(c, CHMC_E_STATE, "no binning of the current positions");
+    const ChainDev& cd = c->hchains[chain];
+    std::vector<int> offs(cd.ncells + 1), perm(cd.natoms);
+    HIPCHK(c, hipMemcpyAsync(offs.data(), c->v.offs + cd.cell_off, sizeof(int) * (cd.ncells + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(perm.data(), c->v.perm + cd.atom_off, sizeof(int) * cd.natoms, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < cd.ncells; k++) {
+        if (bincount) bincount[k] = offs[k + 1] - offs[k];
+        if (binoffsets) binoffsets[k] = offs[k] - cd.atom_off;
+    }
+    if (permute_vector)
+        for (int k = 0; k < cd.natoms; k++) permute_vector[k] = perm[k] - cd.atom_off;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_compute_force(chmc_ctx* c, int fp64)
+{
+    CHECK_CTX(c);
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    rc = do_bin(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_lj(c->v, LJ_FORCE | LJ_STOREF, fp64 != 0, c->islots, 0.0, c->stream));
+    c->force_kind = fp64 ? 2 : 1;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+extern "C" int chmc_get_force(chmc_ctx* c, int chain, double* grad)
+{
+    if (!c || !grad || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    if (c->force_kind == 0) return fail(c, CHMC_E_STATE, "no force computed for the current positions");
+    if (c->force_kind == 2) return get_soa(c, c->v.f64, chain, grad);
+    return get_soa(c, c->v.f32, chain, grad);
+}
+
+extern "C" int chmc_compute_potential(chmc_ctx* c, double* V)
+{
+    if (!c || !V) return CHMC_E_ARG;
+    int rc = do_energy(c, true);
+    if (rc) return rc;
+    return read_red(c, RED_V, V);
+}
+
+extern "C" int chmc_evaluate_potential(chmc_ctx* c, double* V)
+{
+    if (!c || !V) return CHMC_E_ARG;
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    if (!c->bins_valid) {
+        // evaluate_potential uses the bins as they are; positions moved since the last binning keep
+        // their old cells (identical_particles.cpp:387-393), which needs a binning to exist at all
+        rc = do_bin(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, launch_lj(c->v, LJ_ENERGY, true, c->islots, 0.0, c->stream));
+    return read_red(c, RED_V, V);
+}
+
+extern "C" int chmc_virial(chmc_ctx* c, double* W)
+{
+    if (!c || !W) return CHMC_E_ARG;
+    int rc = do_energy(c, !c->bins_valid);
+    if (rc) return rc;
+    return read_red(c, RED_W, W);
+}
+
+extern "C" int chmc_pair_counts(chmc_ctx* c, int chain, long long* ncand, long long* nint)
+{
+    if (!c || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    int rc = do_energy(c, !c->bins_valid);
+    if (rc) return rc;
+    std::vector<double> ni(c->nchains);
+    rc = read_red(c, RED_NINT, ni.data());
+    if (rc) return rc;
+    // candidates of the 27-cell stencil: sum over cells of count(c) * sum_{27 neighbours} count(n), minus self
+    const ChainDev& cd = c->hchains[chain];
+    std::vector<int> offs(cd.ncells + 1);
+    HIPCHK(c, hipMemcpy(offs.data(), c->v.offs + cd.cell_off, sizeof(int) * (cd.ncells + 1), hipMemcpyDeviceToHost));
+    const int nx = cd.nbin[0], ny = cd.nbin[1], nz = cd.nbin[2];
+    long long cand = 0;
+    for (int bz = 0; bz < nz; bz++)
+        for (int by = 0; by < ny; by++)
+            for (int bx = 0; bx < nx; bx++) {
+                const int ci = bx + nx * (by + ny * bz);
+                const long long ni_c = offs[ci + 1] - offs[ci];
+                long long nb = 0;
+                for (int dz = -1; dz <= 1; dz++)
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++) {
+                            const int j = ((bx + dx + nx) % nx) + nx * (((by + dy + ny) % ny) + ny * ((bz + dz + nz) % nz));
+                            nb += offs[j + 1] - offs[j];
+                        }
+                cand += ni_c * nb - ni_c;
+            }
+    if (ncand) *ncand = cand;
+    if (nint) *nint = (long long)llround(ni[chain]);
+    return CHMC_OK;
+}
+
+extern "C" int chmc_kinetic(chmc_ctx* c, double* K)
+{
+    if (!c || !K) return CHMC_E_ARG;
+    HIPCHK(c, launch_kinetic(c->v, RED_K, c->stream));
+    return read_red(c, RED_K, K);
+}
+
+extern "C" int chmc_update_momenta(chmc_ctx* c, double dt)
+{
+    CHECK_CTX(c);
+    int rc = do_force_kick(c, dt, true);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+extern "C" int chmc_update_positions(chmc_ctx* c, double dt)
+{
+    CHECK_CTX(c);
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_drift(c->v, dt, c->stream));
+    c->bins_valid = false;
+    c->force_kind = 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+extern "C" int chmc_kick(chmc_ctx* c, double dt)
+{
+    CHECK_CTX(c);
+    if (c->force_kind == 0) return fail(c, CHMC_E_STATE, "no force computed for the current positions");
+    HIPCHK(c, launch_kick(c->v, dt, c->force_kind == 2, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+// integrator_type::integrate (integrator.cpp:52-131; code-carsten integrator.hh:200-240).  Drifts are
+// deferred and fused into the binning that starts the next force evaluation; storef keeps the force
+// of the last evaluation readable.
+static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double lambda, bool storef)
+{
+    auto M = [&](double h) { return do_force_kick(c, h, storef); };
+    auto X = [&](double h) { return do_drift(c, h); };
+    int rc = 0;
+#define RUN(expr) do { rc = (expr); if (rc) return rc; } while (0)
+    if (scheme == CHMC_LEAPFROG) {
+        RUN(M(dt / 2.));
+        RUN(X(dt));
+        for (int i = 0; i < steps - 1; i++) { RUN(M(dt)); RUN(X(dt)); }
+        RUN(M(dt / 2.));
+    } else if (scheme == CHMC_OMF2) {
+        const double om2l = 1. - 2. * lambda;
+        RUN(M(lambda * dt));
+        for (int i = 0; i < steps - 1; i++) {
+            RUN(X(dt / 2.)); RUN(M(om2l * dt)); RUN(X(dt / 2.)); RUN(M(2. * lambda * dt));
+        }
+        RUN(X(dt / 2.)); RUN(M(om2l * dt)); RUN(X(dt / 2.)); RUN(M(lambda * dt));
+    } else if (scheme == CHMC_OMF4) {
+        const double rho = 0.2539785108410595, theta = -0.03230286765269967;
+        const double vartheta = 0.08398315262876693, lam = 0.6822365335719091;
+        const double e[10] = { rho * dt, lam * dt, theta * dt, 0.5 * (1 - 2. * (lam + vartheta)) * dt,
+                               (1 - 2. * (theta + rho)) * dt, 0.5 * (1 - 2. * (lam + vartheta)) * dt,
+                               theta * dt, lam * dt, rho * dt, 2 * vartheta * dt };
+        RUN(M(0.5 * e[9]));
+        for (int i = 1; i < steps - 1; i++)      // the reference starts at i = 1 (integrator.cpp:118)
+            for (int j = 0; j < 5; j++) { RUN(X(e[2 * j])); RUN(M(e[2 * j + 1])); }
+        for (int j = 0; j < 4; j++) { RUN(X(e[2 * j])); RUN(M(e[2 * j + 1])); }
+        RUN(X(e[8]));
+        RUN(M(0.5 * e[9]));
+    } else if (scheme == CHMC_EULER) {
+        for (int i = 0; i < steps; i++) { RUN(M(dt)); RUN(X(dt)); }
+    } else if (scheme == CHMC_RUTH) {
+        for (int i = 0; i < steps; i++) {
+            RUN(M(dt)); RUN(X(-1. / 24. * dt)); RUN(M(-2. / 3. * dt)); RUN(X(3. / 4. * dt));
+            RUN(M(2. / 3. * dt)); RUN(X(7. / 24. * dt));
+        }
+    } else {
+        return fail(c, CHMC_E_ARG, "unknown integrator scheme");
+    }
+#undef RUN
+    return CHMC_OK;
+}
+
+static bool scheme_ok(int s)
+{
+    return s == CHMC_LEAPFROG || s == CHMC_OMF2 || s == CHMC_OMF4 || s == CHMC_EULER || s == CHMC_RUTH;
+}
+
+extern "C" int chmc_integrate(chmc_ctx* c, int scheme, double dt, int steps, double lambda)
+{
+    CHECK_CTX(c);
+    if (steps < 1 || !scheme_ok(scheme)) return fail(c, CHMC_E_ARG, "integrate: steps must be >= 1 and the scheme known");
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    rc = integrate_seq(c, scheme, dt, steps, lambda, true);
+    if (rc) return rc;
+    rc = flush_drift(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+extern "C" int chmc_heatbath(chmc_ctx* c, uint64_t seed, uint64_t traj)
+{
+    CHECK_CTX(c);
+    HIPCHK(c, launch_heatbath(c->v, seed, (long long)traj, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+extern "C" int chmc_save_positions(chmc_ctx* c)
+{
+    CHECK_CTX(c);
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_save_restore(c->v, 0, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+extern "C" int chmc_restore_positions(chmc_ctx* c)
+{
+    CHECK_CTX(c);
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_save_restore(c->v, 1, c->stream));
+    c->bins_valid = false;
+    c->force_kind = 0;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+// ---- HMC ----------------------------------------------------------------------------------------
+// mt19937_64 of HMC.cpp:38 (gen64), host side, used to draw the Metropolis uniforms up front
+struct Mt64 {
+    uint64_t mt[312];
+    int mti;
+    explicit Mt64(uint64_t seed)
+    {
+        mt[0] = seed;
+        for (int i = 1; i < 312; i++) mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (uint64_t)i;
+        mti = 312;
+    }
+    uint64_t next()
+    {
+        const uint64_t UM = 0xFFFFFFFF80000000ULL, LM = 0x7FFFFFFFULL, MA = 0xB5026F5AA96619E9ULL;
+        if (mti >= 312) {
+            int i;
+            for (i = 0; i < 156; i++) {
+                uint64_t x = (mt[i] & UM) | (mt[i + 1] & LM);
+                mt[i] = mt[i + 156] ^ (x >> 1) ^ ((x & 1ULL) ? MA : 0ULL);
+            }
+            for (; i < 311; i++) {
+                uint64_t x = (mt[i] & UM) | (mt[i + 1] & LM);
+                mt[i] = mt[i - 156] ^ (x >> 1) ^ ((x & 1ULL) ? MA : 0ULL);
+            }
+            uint64_t x = (mt[311] & UM) | (mt[0] & LM);
+            mt[311] = mt[155] ^ (x >> 1) ^ ((x & 1ULL) ? MA : 0ULL);
+            mti = 0;
+        }
+        uint64_t x = mt[mti++];
+        x ^= (x >> 29) & 0x5555555555555555ULL;
+        x ^= (x << 17) & 0x71D67FFFEDA60000ULL;
+        x ^= (x << 37) & 0xFFF7EEE000000000ULL;
+        x ^= (x >> 43);
+        return x;
+    }
+    // gen_random, HMC.cpp:48-50
+    double uniform() { return ((double)next() - 0.0) / (double)(0xFFFFFFFFFFFFFFFFULL - 0ULL); }
+    // set_binomial_steps, integrator.cpp:37-49
+    int binomial(int average_steps)
+    {
+        int s = 0;
+        for (int i = 0; i < 2 * average_steps; i++)
+            if (uniform() < 0.5) s++;
+        return s;
+    }
+};
+
+// one trajectory after the heat bath's counter is known: everything between hb() and the copy of
+// HMC.cpp:103/111 (HMC.cpp:77-117), enqueued on the context stream
+static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps, int ufirst, int rec_base)
+{
+    int rc;
+    HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, c->stream));
+    HIPCHK(c, launch_kinetic(c->v, RED_KI, c->stream));
+    // a zero-length binomial draw leaves x and p unchanged (the reference's size_t loop would not end, Q18)
+    if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false))) return rc;
+    // V_f: after LEAP/OMF2/OMF4 the last binning is of the final positions; after a closing drift
+    // (euler/ruth) it is redone -- the reference rebins in either case (identical_particles.cpp:379)
+    if ((rc = do_energy(c, false))) return rc;
+    HIPCHK(c, launch_kinetic(c->v, RED_K, c->stream));
+    HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, c->stream));
+    HIPCHK(c, launch_save_restore(c->v, 2, c->stream));
+    c->force_kind = 0;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_record* records, int* nacc)
+{
+    if (!c || !hp) return CHMC_E_ARG;
+    if (hp->ntraj < 1 || hp->steps < 1 || !scheme_ok(hp->scheme) || hp->therm < 0)
+        return fail(c, CHMC_E_ARG, "hmc_run: ntraj >= 1, steps >= 1, therm >= 0 and a known scheme are required");
+    const int nch = c->nchains, ntraj = hp->ntraj;
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    // ---- Metropolis uniforms: chain k draws from mt19937_64(seed + k) in the reference's order ----
+    const int rec_base = c->traj_done;
+    const int ufirst = std::max(hp->therm, rec_base + 1);
+    std::vector<Mt64> gens;
+    for (int k = 0; k < nch; k++) gens.emplace_back(hp->seed + (uint64_t)k);
+    std::vector<int> rsteps;
+    std::vector<double> u((size_t)nch * ntraj, 2.0);
+    if (!hp->randomize) {
+        for (int k = 0; k < nch; k++)
+            for (int t = 0; t < ntraj; t++) u[(size_t)k * ntraj + t] = gens[k].uniform();
+    } else {
+        if (nch != 1) return fail(c, CHMC_E_ARG, "randomize_trajectory needs a single-chain context");
+        int q = 0;
+        for (int t = 0; t < ntraj; t++) {
+            rsteps.push_back(gens[0].binomial(hp->steps));
+            if (rec_base + t + 1 >= hp->therm) u[q++] = gens[0].uniform();
+        }
+    }
+    if (c->uniforms_cap < u.size()) {
+        if (c->d_uniforms) hipFree(c->d_uniforms);
+        c->d_uniforms = nullptr;
+        HIPCHK(c, hipMalloc(&c->d_uniforms, sizeof(double) * u.size()));
+        c->uniforms_cap = u.size();
+    }
+    const size_t nrec = (size_t)nch * ntraj;
+    if (c->rec_cap < nrec) {
+        if (c->d_rec) hipFree(c->d_rec);
+        c->d_rec = nullptr;
+        HIPCHK(c, hipMalloc(&c->d_rec, sizeof(HmcRecord) * nrec));
+        c->rec_cap = nrec;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_uniforms, u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice, c->stream));
+
+    // ---- V_i, W_i of the start configuration (HMC.cpp:64), h_x = x (HMC.cpp:72) ----
+    if ((rc = do_energy(c, true))) return rc;
+    std::vector<double> red((size_t)nch * RED_NSLOT);
+    HIPCHK(c, hipMemcpyAsync(red.data(), c->v.red, sizeof(double) * red.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<ChainState> st(nch);
+    for (int k = 0; k < nch; k++) {
+        st[k] = ChainState{};
+        st[k].Vi = red[(size_t)k * RED_NSLOT + RED_V];
+        st[k].Wi = red[(size_t)k * RED_NSLOT + RED_W];
+    }
+    HIPCHK(c, hipMemcpyAsync(c->v.state, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_save_restore(c->v, 0, c->stream));
+
+    if (hp->use_graph && !hp->randomize) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_trajectory(c, *hp, hp->steps, ufirst, rec_base);
+        hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+        if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
+        HIPCHK(c, ec);
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) { hipGraphDestroy(graph); HIPCHK(c, ei); }
+        hipError_t el = hipSuccess;
+        for (int t = 0; t < ntraj && el == hipSuccess; t++) el = hipGraphLaunch(exec, c->stream);
+        hipError_t es = hipStreamSynchronize(c->stream);
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        HIPCHK(c, el);
+        HIPCHK(c, es);
+    } else {
+        for (int t = 0; t < ntraj; t++) {
+            const int steps = hp->randomize ? rsteps[t] : hp->steps;
+            rc = enqueue_trajectory(c, *hp, steps, ufirst, rec_base);
+            if (rc) return rc;
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->traj_done += ntraj;
+    c->bins_valid = false;
+    if (records) HIPCHK(c, hipMemcpy(records, c->d_rec, sizeof(HmcRecord) * nrec, hipMemcpyDeviceToHost));
+    if (nacc) {
+        HIPCHK(c, hipMemcpy(st.data(), c->v.state, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
+        for (int k = 0; k < nch; k++) nacc[k] = st[k].nacc;
+    }
+    return CHMC_OK;
+}
+
+// ---- measurements --------------------------------------------------------------------------------
+extern "C" int chmc_rdf(chmc_ctx* c, int chain, int nbins, double size_bin, unsigned long long* counts, double* g)
+{
+    if (!c || chain < 0 || chain >= c->nchains || nbins < 1 || nbins > 4000 || !(size_bin > 0)) return CHMC_E_ARG;
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    if (c->rdf_cap < nbins) {
+        if (c->d_rdf) hipFree(c->d_rdf);
+        c->d_rdf = nullptr;
+        HIPCHK(c, hipMalloc(&c->d_rdf, sizeof(unsigned long long) * nbins));
+        c->rdf_cap = nbins;
+    }
+    const ChainDev& cd = c->hchains[chain];
+    HIPCHK(c, hipMemsetAsync(c->d_rdf, 0, sizeof(unsigned long long) * nbins, c->stream));
+    HIPCHK(c, launch_rdf(c->v, chain, cd.natoms, nbins, size_bin, c->d_rdf, c->stream));
+    std::vector<unsigned long long> h(nbins);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_rdf, sizeof(unsigned long long) * nbins, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double V = cd.L[0] * cd.L[1] * cd.L[2];
+    const int N = cd.natoms;
+    for (int ib = 0; ib < nbins; ib++) {
+        if (counts) counts[ib] = h[ib];
+        if (g) {
+            double ri = ib * size_bin;
+            double rf = ri + size_bin;
+            ri = ri * ri * ri;
+            rf = rf * rf * rf;
+            // identical_particles.cpp:768-774, including its 1.33333333333 constant
+            g[ib] = (double)h[ib] * V / (1.33333333333 * M_PI * (rf - ri) * (N - 1) * N);
+        }
+    }
+    return CHMC_OK;
+}
+
+extern "C" int chmc_potential_energy_open(const double* xyz, int n_particles, int n, double a, double b, double* result)
+{
+    if (!xyz || !result || n_particles < 0) return CHMC_E_ARG;
+    *result = 0.0;
+    if (n_particles < 2) return CHMC_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(nullptr, CHMC_E_NODEV, "no HIP device");
+    const int np = n_particles;
+    const int T = (np + 255) / 256;
+    std::vector<double> soa(3 * (size_t)np);
+    for (int i = 0; i < np; i++)
+        for (int d = 0; d < 3; d++) soa[(size_t)d * np + i] = xyz[3 * i + d];
+    double *dx = nullptr, *dp = nullptr;
+    hipStream_t s = nullptr;
+    auto cleanup = [&]() {
+        if (dx) hipFree(dx);
+        if (dp) hipFree(dp);
+        if (s) hipStreamDestroy(s);
+    };
+    hipError_t e = hipStreamCreate(&s);
+    if (e == hipSuccess) e = hipMalloc(&dx, sizeof(double) * soa.size());
+    if (e == hipSuccess) e = hipMalloc(&dp, sizeof(double) * (size_t)T * T);
+    if (e == hipSuccess) e = hipMemcpyAsync(dx, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice, s);
+    int nparts = 0;
+    if (e == hipSuccess) e = launch_open_potential(dx, np, n, a, b, dp, &nparts, s);
+    std::vector<double> parts((size_t)T * T);
+    if (e == hipSuccess) e = hipMemcpyAsync(parts.data(), dp, sizeof(double) * parts.size(), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess) return fail(nullptr, CHMC_E_HIP, std::string("potential_energy_open: ") + hipGetErrorString(e));
+    double r = 0.0;
+    for (double q : parts) r += q;
+    *result = r;
+    return CHMC_OK;
+}
+
+// ---- tuning ---------------------------------------------------------------------------------------
+static int time_force_is(chmc_ctx* c, int islots, int reps, double* ms_bin, double* ms_force)
+{
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    // warm-up
+    HIPCHK(c, launch_bin(c->v, 0.0, id_order(c), c->stream));
+    HIPCHK(c, launch_lj(c->v, LJ_FORCE | LJ_STOREF, false, islots, 0.0, c->stream));
+    float tb = 0.f, tf = 0.f;
+    for (int r = 0; r < reps; r++) {
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        HIPCHK(c, launch_bin(c->v, 0.0, id_order(c), c->stream));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, launch_lj(c->v, LJ_FORCE | LJ_STOREF, false, islots, 0.0, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev2, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->ev2));
+        float a = 0.f, b = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a, c->ev0, c->ev1));
+        HIPCHK(c, hipEventElapsedTime(&b, c->ev1, c->ev2));
+        tb += a;
+        tf += b;
+    }
+    c->bins_valid = true;
+    c->force_kind = 1;
+    if (ms_bin) *ms_bin = tb / reps;
+    if (ms_force) *ms_force = tf / reps;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_set_force_tiling(chmc_ctx* c, int islots)
+{
+    CHECK_CTX(c);
+    if (islots == 4 || islots == 8 || islots == 16) {
+        c->islots = islots;
+        return CHMC_OK;
+    }
+    if (islots != 0) return fail(c, CHMC_E_ARG, "islots must be 0, 4, 8 or 16");
+    // autotuner: time each tiling of the FP32 sweep on the current configuration, keep the fastest
+    const int cand[3] = { 4, 8, 16 };
+    double best = 1e30;
+    int best_is = c->islots;
+    for (int is : cand) {
+        double tf = 0.0;
+        int rc = time_force_is(c, is, 5, nullptr, &tf);
+        if (rc) return rc;
+        if (tf < best) { best = tf; best_is = is; }
+    }
+    c->islots = best_is;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_get_force_tiling(const chmc_ctx* c, int* islots)
+{
+    CHECK_CTX(c);
+    if (islots) *islots = c->islots;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_time_force(chmc_ctx* c, int reps, double* ms_bin, double* ms_force)
+{
+    CHECK_CTX(c);
+    if (reps < 1) return CHMC_E_ARG;
+    return time_force_is(c, c->islots, reps, ms_bin, ms_force);
+}

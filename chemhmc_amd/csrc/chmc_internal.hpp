@@ -1,0 +1,89 @@
+// Internal declarations shared by the HIP translation units of libchmc.
+//
+// Data layout in HBM (one context = a batch of independent chains, concatenated):
+//   x, p, x_saved : double SoA [3][Ntot]      (component d of atom a at d*Ntot + a)
+//   f32           : float  SoA [3][Ntot]      (+grad U, FP32 pair sums)
+//   f64           : double SoA [3][Ntot]      (+grad U, FP64 path)
+//   atom_chain    : int [Ntot]
+//   cell_of, slot : int [Ntot]                (binning scratch: cell id, arrival slot)
+//   count         : int [Ctot]                (zero between binnings; the scan clears it)
+//   offs          : int [Ctot + 1]            (global exclusive scan of the counts)
+//   skey, satom   : int [Ntot]                (unsorted within-cell scatter, sort key and atom)
+//   perm, invperm : int [Ntot]                (sorted slot -> atom, atom -> sorted slot)
+//   xs            : float4 [Ntot]             (sorted cell-relative FP32 coordinates, atom id in w)
+//   cell_chain    : int [Ctot]
+// Chains are contiguous in both the atom and the cell index spaces, so the global scan of the
+// per-cell counts equals chain atom offset + the chain's own exclusive scan.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace chmc {
+
+struct ChainDev {
+    int atom_off, natoms, cell_off, ncells;
+    int nbin[3];
+    int chunk_first, nchunks;      // deterministic reduction chunks of this chain
+    int pad_;
+    double L[3], sizebin[3];
+    double rc, rc2, sigma, sigma2, eps, mass, beta;
+    float  sizebin_f[3], rc2_f, sigma2_f, eps48_f;
+};
+
+// per-chain reduction slots
+enum RedSlot { RED_K = 0, RED_V = 1, RED_W = 2, RED_KI = 3, RED_NINT = 4, RED_NSLOT = 5 };
+
+// persistent per-chain HMC state on the device
+struct ChainState {
+    double Vi, Wi, Ki;
+    int nacc, accepted;
+    int pad_[2];
+};
+
+struct HmcRecord {
+    double dh, u, k, w;
+    int acc, pad_;
+};
+
+struct Chunk { int chain, begin, end, pad_; };
+
+constexpr int kMaxJ = 768;         // j-list entries staged in LDS per pass
+constexpr int kRedChunk = 4096;    // atoms per deterministic reduction chunk
+
+// ---- launch wrappers (each defined in the TU that owns the kernel) ----------------------
+struct DevView {
+    int Ntot, Ctot, nchains, nchunks;
+    const ChainDev* chains;
+    const int* atom_chain;
+    const int* cell_chain;
+    const Chunk* chunks;
+    double *x, *p, *x_saved, *f64;
+    float* f32;
+    int *cell_of, *slot, *count, *offs, *skey, *satom, *perm, *invperm;
+    float4* xs;
+    double* red;          // [nchains][RED_NSLOT]
+    double* chunk_part;   // [nchunks * 2]
+    double* cell_part;    // [Ctot * 3]: energy, virial, interacting pairs
+    int* ticket;          // [nchains]
+    ChainState* state;    // [nchains]
+    int* traj;            // [1] device trajectory counter (1-based index of the last finished one)
+};
+
+enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8 };
+
+hipError_t launch_bin(const DevView& v, double drift_dt, bool atom_id_order, hipStream_t s);
+hipError_t launch_drift(const DevView& v, double dt, hipStream_t s);
+hipError_t launch_kick(const DevView& v, double dt, bool fp64, hipStream_t s);
+hipError_t launch_lj(const DevView& v, int mode, bool fp64, int islots, double kick_dt, hipStream_t s);
+hipError_t launch_kinetic(const DevView& v, int slot, hipStream_t s);
+hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, hipStream_t s);
+hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustride, int therm, int ufirst,
+                             HmcRecord* rec, int rec_base, hipStream_t s);
+hipError_t launch_save_restore(const DevView& v, int mode, hipStream_t s);
+hipError_t launch_rdf(const DevView& v, int chain, int natoms, int nbins, double size_bin,
+                      unsigned long long* counts, hipStream_t s);
+hipError_t launch_open_potential(const double* xyz_soa, int n, int nexp, double a, double b,
+                                 double* partials, int* nparts, hipStream_t s);
+hipError_t launch_fill_int(int* p, int value, int n, hipStream_t s);
+
+}  // namespace chmc

@@ -1,0 +1,164 @@
+/*
+ * chmc.h -- C ABI of the MI355X Lennard-Jones HMC engine (libchmc.so).
+ *
+ * This is the only door to device code.  All entry points are extern "C", take plain pointers and
+ * sizes, return 0 on success and a negative CHMC_E_* code on failure (message via
+ * chmc_last_error), never throw, and never hand device pointers to the caller.  Host buffers are
+ * caller-owned; device memory is owned by the context; each context owns one HIP stream and is not
+ * thread-safe (one context per host thread).
+ *
+ * A context holds a batch of independent chains ("systems"), each with its own N, box, cutoff,
+ * LJ parameters, mass and beta, its own cell grid and its own Metropolis decision; every entry
+ * point below acts on all chains at once (one launch per stage for the whole batch).  With one
+ * chain it is the reference's single `particles_type`.
+ *
+ * Units and conventions follow the reference (code/modules/identical_particles.cpp):
+ *   forces hold +grad U (Q1, :499-509, :613-633); kicks subtract p -= dt*beta*f (:686-688);
+ *   drifts add x += dt*(beta/m)*p and wrap x -= L*floor(x/L) (:665-667); time is beta-scaled.
+ * Positions/momenta/forces cross the ABI as AoS double[3] per particle (the layout of
+ * code-carsten/config.hh config<coordinate>); inside the context they are SoA in HBM.
+ */
+#ifndef CHMC_H
+#define CHMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chmc_ctx chmc_ctx;
+
+enum {
+    CHMC_OK = 0,
+    CHMC_E_ARG = -1,      /* invalid argument (bad chain index, size, scheme, ...) */
+    CHMC_E_GEOMETRY = -2, /* box dimension <= cutoff (identical_particles_binning.cpp:7-10) */
+    CHMC_E_HIP = -3,      /* a HIP runtime call failed */
+    CHMC_E_NODEV = -4,    /* no usable GPU */
+    CHMC_E_STATE = -5     /* call out of order (e.g. bins requested before any binning) */
+};
+
+/* One chain; replaces the YAML keys particles.{N,mass,beta,cutoff,eps,sigma} and geometry.{Lx,Ly,Lz}
+ * read by particles_type / identical_particles (particles.cpp:17-36, identical_particles.cpp:18-28). */
+typedef struct chmc_chain_desc {
+    int N;
+    double L[3];
+    double cutoff, sigma, eps, mass, beta;
+} chmc_chain_desc;
+
+/* context flags */
+enum {
+    CHMC_FORCE_FP32 = 0,       /* MD forces: FP32 pairs, FP32 per-atom sums (default) */
+    CHMC_FORCE_FP64 = 1 << 0,  /* MD forces in FP64 (the reference's arithmetic) */
+    CHMC_ORDER_ATOM_ID = 1 << 1/* in-cell order by atom id instead of the reference's stable order */
+};
+
+/* integrator schemes: the numbering of code-carsten/integrator.hh:14 (set_integrator) */
+enum { CHMC_LEAPFROG = 0, CHMC_OMF4 = 2, CHMC_EULER = 4, CHMC_RUTH = 5, CHMC_OMF2 = 6 };
+
+/* ---- lifetime ------------------------------------------------------------------------------ */
+/* Replaces the identical_particles constructor + cutoff_binning + parallel_binning_init
+ * (identical_particles.cpp:18-87, identical_particles_binning.cpp:4-44): allocates every chain's
+ * state and cell grid, permute_vector = identity.  device < 0 picks the current device. */
+int chmc_create(chmc_ctx** ctx, int nchains, const chmc_chain_desc* chains, int device, int flags);
+int chmc_destroy(chmc_ctx* ctx);
+const char* chmc_last_error(const chmc_ctx* ctx);   /* NULL ctx: last error of a failed chmc_create */
+int chmc_sync(chmc_ctx* ctx);
+
+/* geometry of one chain: nbin[3] and sizebin[3] as cutoff_binning computes them */
+int chmc_get_geometry(const chmc_ctx* ctx, int chain, int nbin[3], double sizebin[3], int* bintot);
+int chmc_num_atoms(const chmc_ctx* ctx, int* total, int* nchains);
+
+/* ---- state transfer (AoS double[3]; deep_copy of x / h_x in HMC.cpp:72,102,111,115) --------- */
+int chmc_set_positions(chmc_ctx* ctx, int chain, const double* xyz);
+int chmc_get_positions(chmc_ctx* ctx, int chain, double* xyz);
+int chmc_set_momenta(chmc_ctx* ctx, int chain, const double* pxyz);
+int chmc_get_momenta(chmc_ctx* ctx, int chain, double* pxyz);
+/* project input generator: n^3 simple-cubic sites (n^3 >= N) at (i+1/2)L/n, first N in
+ * lexicographic order, + uniform jitter in [-jitter, jitter) (replaces InitX cold/hot starts,
+ * identical_particles.cpp:198-257, whose Kokkos RNG stream cannot be reproduced) */
+int chmc_init_lattice(chmc_ctx* ctx, uint64_t seed, double jitter);
+
+/* ---- binning (create_binning, particles.hpp:92; parallel_binning, identical_particles_binning.cpp:97-110) */
+int chmc_bin(chmc_ctx* ctx);
+/* bincount / binoffsets / permute_vector of one chain, chain-local indices
+ * (particles.hpp:45-47); any pointer may be NULL */
+int chmc_get_bins(chmc_ctx* ctx, int chain, int* bincount, int* binoffsets, int* permute_vector);
+
+/* ---- LJ sweeps -------------------------------------------------------------------------------- */
+/* compute_force (identical_particles.cpp:559-563): binning + force; fp64 = 0 FP32 pairs, 1 FP64 */
+int chmc_compute_force(chmc_ctx* ctx, int fp64);
+int chmc_get_force(chmc_ctx* ctx, int chain, double* grad_xyz);   /* +grad U of the last chmc_compute_force */
+/* compute_potential (identical_particles.cpp:377-385): binning + FP64 V per chain (V[nchains]) */
+int chmc_compute_potential(chmc_ctx* ctx, double* V);
+/* evaluate_potential (identical_particles.cpp:387-393): FP64 V with the current bins */
+int chmc_evaluate_potential(chmc_ctx* ctx, double* V);
+/* pair virial W = sum_{i<j} r_ij . F_ij of the current configuration (no reference counterpart) */
+int chmc_virial(chmc_ctx* ctx, double* W);
+/* candidate pairs of the 27-cell stencil and interacting pairs (r < cutoff), ordered, of one chain in
+ * its current configuration: the N_cand and N_int of the LJ sweep's algorithmic operation count */
+int chmc_pair_counts(chmc_ctx* ctx, int chain, long long* ncand, long long* nint);
+/* compute_kinetic_E (identical_particles.cpp:459-463): K per chain */
+int chmc_kinetic(chmc_ctx* ctx, double* K);
+
+/* ---- integrator building blocks --------------------------------------------------------------- */
+/* update_momenta (identical_particles.cpp:691-694): force (with binning) + kick p -= dt*beta*f */
+int chmc_update_momenta(chmc_ctx* ctx, double dt);
+/* update_positions (identical_particles.cpp:671-673): x += dt*(beta/m)*p, wrap */
+int chmc_update_positions(chmc_ctx* ctx, double dt);
+/* kick with the force of the last chmc_compute_force, without recomputing it */
+int chmc_kick(chmc_ctx* ctx, double dt);
+/* integrator_type::integrate for LEAP / OMF2 / OMF4 (integrator.cpp:52-131) and code-carsten's euler /
+ * ruth (integrator.hh:200-240); lambda is OMF2's (reference: 0.1938, integrator.cpp:72).  Each
+ * drift is fused into the binning of the following force evaluation. */
+int chmc_integrate(chmc_ctx* ctx, int scheme, double dt, int steps, double lambda);
+
+/* ---- HMC pieces ------------------------------------------------------------------------------- */
+/* hb (identical_particles.cpp:283-296): p ~ N(0, m/beta), Philox4x64-10 key {seed, "HEAT"},
+ * counter {atom, traj, chain, 0} */
+int chmc_heatbath(chmc_ctx* ctx, uint64_t seed, uint64_t traj);
+int chmc_save_positions(chmc_ctx* ctx);      /* h_x = x   (HMC.cpp:72,103,107) */
+int chmc_restore_positions(chmc_ctx* ctx);   /* x = h_x   (HMC.cpp:111) */
+
+/* HMC_class::run (HMC.cpp:61-136) for every chain of the context, all on the device: heat bath,
+ * K_i, MD, V_f (FP64), K_f, dh = beta (K_f + V_f - K_i - V_i), thermalisation, Metropolis with
+ * u from a per-chain mt19937_64 seeded seed + chain (gen_random, HMC.cpp:48-50), restore on
+ * reject.  use_graph = 1 captures one trajectory in a hipGraph and replays it. */
+typedef struct chmc_hmc_params {
+    int ntraj, therm, scheme, steps;
+    double dt, lambda;
+    uint64_t seed;
+    int use_graph;
+    int randomize;     /* randomize_trajectory (HMC.cpp:82-85): binomial steps, no graph */
+} chmc_hmc_params;
+
+/* per trajectory and chain, record index = traj * nchains + chain */
+typedef struct chmc_hmc_record {
+    double dh, u, k, w;   /* dH, U after the decision, K at the start, virial W after the decision */
+    int accepted, pad_;
+} chmc_hmc_record;
+
+/* records may be NULL; nacc[nchains] (may be NULL) receives the accepted post-thermalisation counts */
+int chmc_hmc_run(chmc_ctx* ctx, const chmc_hmc_params* hp, chmc_hmc_record* records, int* nacc);
+
+/* ---- measurements ----------------------------------------------------------------------------- */
+/* compute_RDF (identical_particles.cpp:698-776) of one chain: counts[nbins] of ordered pairs and
+ * g[nbins] normalised as the reference does; either pointer may be NULL */
+int chmc_rdf(chmc_ctx* ctx, int chain, int nbins, double size_bin, unsigned long long* counts, double* g);
+
+/* code-carsten potential_energy (potential_energy.hh:6-15) on the GPU: sum_{i<j}
+ * LennardJones(|r_i - r_j|, n, a, b) (lennardjones.hh:5-9) of an AoS double[3] set, open boundaries */
+int chmc_potential_energy_open(const double* xyz, int n_particles, int n, double a, double b, double* result);
+
+/* ---- tuning ----------------------------------------------------------------------------------- */
+/* i-slots per wavefront of the LJ sweep (4, 8 or 16; 0 = run the autotuner on the current state) */
+int chmc_set_force_tiling(chmc_ctx* ctx, int islots);
+int chmc_get_force_tiling(const chmc_ctx* ctx, int* islots);
+/* average device time (ms) of one force evaluation (binning + FP32 sweep + kick) over reps */
+int chmc_time_force(chmc_ctx* ctx, int reps, double* ms_bin, double* ms_force);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,251 @@
+"""Parity of the HIP hot path (through the C ABI) with the FP64 oracle on identical seeded inputs.
+
+Bars (written next to each assertion):
+  bit-exact  : lattice generator, cell counts / offsets / permute_vector (incl. the reference's
+               stable in-cell order across re-binnings), drift + wrap, kick given the force, RDF counts
+  FP64 sweep : forces, potential and virial within 1e-12 relative to the oracle
+  FP32 sweep : per-atom force error against the FP64 oracle, max <= 2e-5 * f_rms, rms <= 2e-6 * f_rms
+  heat bath  : Box-Muller through device log/cos/sin, within 1e-13 relative to the host libm
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RC = 2.5
+
+
+def box(N, rho):
+    return (N / rho) ** (1.0 / 3.0)
+
+
+def make(chmc, N, rho, T=1.0, seed=7, L=None, **kw):
+    L = box(N, rho) if L is None else L
+    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1.0 / T), **kw)
+    e.init_lattice(seed, 0.05)
+    return e, L
+
+
+def min_image(d, L):
+    L = np.broadcast_to(np.asarray(L, float), (3,))
+    return d - L * np.round(d / L)
+
+
+CASES = [  # N, rho, box (None = cubic from rho)
+    (512, 0.8, None),              # config 1: nbin = 3
+    (32768, 0.8442, None),         # config 2: nbin = 13
+    (777, 0.6, None),              # ragged N (lattice truncated)
+    (125, 0.58, None),             # nbin = 2: neighbour cells visited twice with different images
+    (51, 0.8, None),               # nbin = 1
+    (300, None, (8.0, 6.0, 4.5)),  # anisotropic box: nbin = (3, 2, 1)
+]
+
+
+def _eng(chmc, N, rho, Lb, **kw):
+    if Lb is None:
+        return make(chmc, N, rho, **kw)
+    return make(chmc, N, None, L=Lb, **kw)
+
+
+@pytest.mark.parametrize("N,rho,Lb", CASES)
+def test_lattice_and_binning_bitexact(chmc, orc, N, rho, Lb):
+    e, L = _eng(chmc, N, rho, Lb)
+    x = e.positions()
+    assert np.array_equal(x, orc.lattice_init(N, L, 7, 0, 0.05))          # bit-exact
+    e.bin()
+    perm, cnt, off = e.bins()
+    po, co, oo = orc.binning(x, L, RC)
+    assert np.array_equal(cnt, co) and np.array_equal(off, oo) and np.array_equal(perm, po)
+    # drift with random momenta, then re-bin: stable order w.r.t. the previous permutation
+    rng = np.random.default_rng(N)
+    p = rng.normal(size=(N, 3))
+    e.set_momenta(p)
+    e.update_positions(0.3)
+    x2 = e.positions()
+    assert np.array_equal(x2, orc.drift(x, p, 0.3, 1.0, L))                 # bit-exact drift + wrap
+    e.bin()
+    perm2, cnt2, off2 = e.bins()
+    po2, co2, oo2 = orc.binning(x2, L, RC, perm=po)
+    assert np.array_equal(cnt2, co2) and np.array_equal(off2, oo2) and np.array_equal(perm2, po2)
+    assert cnt2.sum() == N and np.array_equal(np.sort(perm2), np.arange(N))
+
+
+def test_binning_atom_id_order(chmc, orc):
+    N, rho = 4096, 0.8442
+    e, L = make(chmc, N, rho, atom_id_order=True)
+    rng = np.random.default_rng(3)
+    e.set_momenta(rng.normal(size=(N, 3)))
+    e.update_positions(0.5)
+    x = e.positions()
+    e.bin()
+    perm, cnt, off = e.bins()
+    po, co, oo = orc.binning(x, L, RC)        # stable sort of the identity = order by atom id
+    assert np.array_equal(perm, po) and np.array_equal(cnt, co) and np.array_equal(off, oo)
+
+
+@pytest.mark.parametrize("N,rho,Lb", CASES)
+def test_force_fp64_and_fp32(chmc, orc, N, rho, Lb):
+    e, L = _eng(chmc, N, rho, Lb)
+    x = e.positions()
+    fo = orc.force_binning(x, L, RC)
+    frms = np.sqrt(np.mean(fo ** 2))
+    e.compute_force(fp64=True)
+    f64 = e.force()
+    assert np.max(np.abs(f64 - fo)) <= 1e-12 * frms * 10 + 1e-12 * np.max(np.abs(fo))
+    e.compute_force(fp64=False)
+    f32 = e.force()
+    err = np.abs(f32 - fo)
+    assert np.max(err) <= 2e-5 * frms, np.max(err) / frms
+    assert np.sqrt(np.mean(err ** 2)) <= 2e-6 * frms
+
+
+@pytest.mark.parametrize("N,rho,Lb", CASES)
+def test_potential_virial_kinetic(chmc, orc, N, rho, Lb):
+    e, L = _eng(chmc, N, rho, Lb)
+    x = e.positions()
+    V = e.compute_potential()[0]
+    Vo = orc.potential_binning(x, L, RC)
+    assert abs(V - Vo) <= 1e-12 * abs(Vo)
+    W = e.virial()[0]
+    Wo = orc.virial_binning(x, L, RC)
+    assert abs(W - Wo) <= 1e-12 * abs(Wo)
+    e.heatbath(11, 5)
+    p = e.momenta()
+    po = orc.heatbath(N, 1.0, 1.0, 11, 0, 5)
+    assert np.max(np.abs(p - po)) <= 1e-13 * np.max(np.abs(po))
+    K = e.kinetic()[0]
+    assert abs(K - orc.kinetic(p, 1.0)) <= 1e-13 * K
+
+
+def test_potential_small_all_pairs(chmc, orc):
+    # test_binning.cpp:239-253: binned V equals the all-neighbour V (rel 1e-6; here 1e-12)
+    e, L = make(chmc, 216, 0.5)
+    x = e.positions()
+    assert abs(e.compute_potential()[0] - orc.potential_all(x, L, RC)) <= 1e-12 * abs(orc.potential_all(x, L, RC))
+
+
+def test_force_five_point_derivative(chmc):
+    # test_binning.cpp:108-168: f (= +grad U) equals the 5-point derivative of V, rel 1e-3
+    e, L = make(chmc, 512, 0.8)
+    x0 = e.positions()
+    e.compute_force(fp64=True)
+    f = e.force()
+    h = 1e-6
+    for i, d in [(0, 0), (17, 1), (300, 2), (511, 0)]:
+        vals = []
+        for s in (2, 1, -1, -2):
+            x = x0.copy()
+            x[i, d] += s * h
+            e.set_positions(x)
+            vals.append(e.evaluate_potential()[0])
+        num = (-vals[0] + 8 * vals[1] - 8 * vals[2] + vals[3]) / (12 * h)
+        assert abs(num - f[i, d]) <= 1e-3 * max(abs(num), 1e-6)
+
+
+def test_kick_bitexact_and_fused(chmc, orc):
+    N, rho = 32768, 0.8442
+    e, L = make(chmc, N, rho, T=0.722)
+    beta = 1 / 0.722
+    e.heatbath(3, 1)
+    p0 = e.momenta()
+    e.compute_force(fp64=False)
+    f = e.force()
+    e.kick(0.01)
+    p1 = e.momenta()
+    assert np.array_equal(p1, orc.kick(p0, f, 0.01, beta))                  # bit-exact given f
+    e.set_momenta(p0)
+    e.update_momenta(0.01)                                                  # fused force + kick sweep
+    assert np.array_equal(e.momenta(), p1)
+
+
+@pytest.mark.parametrize("scheme,steps", [(0, 20), (6, 10), (2, 5), (4, 6), (5, 3)])
+def test_integrators_fp64_vs_oracle(chmc, orc, scheme, steps):
+    N, rho, T = 512, 0.8, 1.5
+    e, L = make(chmc, N, rho, T=T, fp64_forces=True)
+    x = e.positions()
+    e.heatbath(5, 1)
+    p = e.momenta()
+    s = orc.Sim(N, [L] * 3, cutoff=RC, beta=1 / T)
+    s.x[:] = x
+    s.p[:] = p
+    e.integrate(scheme, 0.005, steps, 0.1932)
+    s.integrate(scheme, 0.005, steps, 0.1932)
+    assert np.max(np.abs(min_image(e.positions() - s.x, L))) < 1e-10
+    assert np.max(np.abs(e.momenta() - s.p)) < 1e-9
+
+
+def test_integrator_fp32_config2(chmc, orc):
+    N, rho, T = 32768, 0.8442, 0.722
+    e, L = make(chmc, N, rho, T=T)
+    x = e.positions()
+    e.heatbath(9, 1)
+    p = e.momenta()
+    s = orc.Sim(N, [L] * 3, cutoff=RC, beta=1 / T)
+    s.x[:] = x
+    s.p[:] = p
+    e.integrate(6, 0.01, 10, 0.1932)
+    s.integrate(6, 0.01, 10, 0.1932)
+    dx = np.abs(min_image(e.positions() - s.x, L))
+    assert np.max(dx) < 1e-5, np.max(dx)
+
+
+@pytest.mark.parametrize("N,rho,nb", [(512, 0.8, 200), (4000, 0.8442, 500)])
+def test_rdf_counts_bitexact(chmc, orc, N, rho, nb):
+    e, L = make(chmc, N, rho)
+    e.heatbath(1, 1)
+    e.integrate(0, 0.005, 5)
+    x = e.positions()
+    counts, g = e.rdf(nb, L / (2 * nb))
+    co, go = orc.rdf(x, [L] * 3, nb, L / (2 * nb))
+    assert np.array_equal(counts, co)
+    assert np.allclose(g, go, rtol=1e-14, atol=0)
+
+
+def test_potential_energy_open_vs_reference(chmc, orc):
+    import json
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "carsten_potential.json")) as fh:
+        gold = json.load(fh)
+    for case in gold["cases"]:
+        x = np.asarray(case["x"], dtype=np.float64)
+        v = chmc.potential_energy_open(x)
+        assert abs(v - case["potential_energy"]) <= 1e-12 * abs(case["potential_energy"])
+
+
+def test_hmc_graph_equals_direct_and_batch_independence(chmc):
+    T, N, rho = 1.5, 512, 0.8
+    recs = []
+    for g in (True, False):
+        e, L = make(chmc, N, rho, T=T)
+        r, nacc = e.hmc(12, 3, 0, 0.005, 20, seed=4, use_graph=g)
+        recs.append(r)
+        e.close()
+    assert np.array_equal(recs[0], recs[1])
+    # chain 0 of a batch evolves exactly as when it runs alone; chains do not interact
+    chains = [chmc.Chain(N=N, L=box(N, rho), cutoff=RC, beta=1 / T),
+              chmc.Chain(N=343, L=box(343, 0.5), cutoff=RC, beta=1.0),
+              chmc.Chain(N=1000, L=box(1000, 0.7), cutoff=RC, beta=0.8)]
+    eb = chmc.Engine(chains)
+    eb.init_lattice(7, 0.05)
+    rb, _ = eb.hmc(12, 3, 0, 0.005, 20, seed=4, use_graph=True)
+    assert np.array_equal(rb[:, 0], recs[0][:, 0])
+    chains2 = [chains[0], chmc.Chain(N=216, L=box(216, 0.4), cutoff=RC, beta=1.2), chains[2]]
+    eb2 = chmc.Engine(chains2)
+    eb2.init_lattice(7, 0.05)
+    rb2, _ = eb2.hmc(12, 3, 0, 0.005, 20, seed=4, use_graph=True)
+    assert np.array_equal(rb2[:, 0], rb[:, 0]) and np.array_equal(rb2[:, 2], rb[:, 2])
+
+
+def test_error_paths(chmc):
+    with pytest.raises(chmc.ChmcError):
+        chmc.Engine(chmc.Chain(N=10, L=2.0, cutoff=2.5))            # box <= cutoff
+    e, L = make(chmc, 64, 0.5)
+    x = e.positions()
+    x[3, 1] = L + 0.1
+    with pytest.raises(chmc.ChmcError):
+        e.set_positions(x)                                          # check_in_volume
+    with pytest.raises(chmc.ChmcError):
+        e.integrate(3, 0.01, 5)                                     # unknown scheme
+    with pytest.raises(chmc.ChmcError):
+        e.integrate(0, 0.01, 0)                                     # steps < 1 (reference loops forever)
