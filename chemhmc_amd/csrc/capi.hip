@@ -28,9 +28,10 @@ struct chmc_ctx {
     DevView v{};
     ChainDev* d_chains = nullptr;
     std::vector<void*> allocs;
-    bool bins_valid = false;
+    int bin_state = 2;           // 0: bins of the current positions, 1: counted (placement pending), 2: stale
+    bool have_bins = false;      // a placement exists that the local binning path can start from
     int force_kind = 0;          // 0 none, 1 fp32 stored, 2 fp64 stored
-    int islots = 8;
+    int islots = 82;            // LJ sweep tiling: i-slots * 10 + waves per workgroup
     double pending_drift = 0.0;  // drift deferred into the next binning (integrator sequencing)
     int traj_done = 0;           // trajectories completed on the device counter
     std::string err;
@@ -175,17 +176,18 @@ extern "C" int chmc_create(chmc_ctx** out, int nchains, const chmc_chain_desc* c
     if ((e = dalloc(c, &v.f32, 3 * (size_t)N)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.cell_of, N)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.slot, N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.newcell, N)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.count, Cn)) != hipSuccess) return bail("hipMalloc", e);
-    if ((e = dalloc(c, &v.offs, Cn + 1)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.offs_buf, 2 * ((size_t)Cn + 1))) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.bflag, 2)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.skey, N)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.satom, N)) != hipSuccess) return bail("hipMalloc", e);
-    if ((e = dalloc(c, &v.perm, N)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.perm_buf, 2 * (size_t)N)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.invperm, N)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.xs, N)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.red, (size_t)nchains * RED_NSLOT)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.chunk_part, (size_t)c->nchunks)) != hipSuccess) return bail("hipMalloc", e);
-    if ((e = dalloc(c, &v.cell_part, 3 * (size_t)Cn)) != hipSuccess) return bail("hipMalloc", e);
-    if ((e = dalloc(c, &v.ticket, nchains)) != hipSuccess) return bail("hipMalloc", e);
+    if ((e = dalloc(c, &v.cell_part, 4 * (size_t)Cn)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.state, nchains)) != hipSuccess) return bail("hipMalloc", e);
     if ((e = dalloc(c, &v.traj, 1)) != hipSuccess) return bail("hipMalloc", e);
     v.chains = c->d_chains; v.atom_chain = atom_chain; v.cell_chain = cell_chain; v.chunks = chunks;
@@ -202,10 +204,12 @@ extern "C" int chmc_create(chmc_ctx** out, int nchains, const chmc_chain_desc* c
     if ((e = hipMemcpy(cell_chain, hc.data(), sizeof(int) * Cn, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
     if ((e = hipMemcpy(chunks, c->hchunks.data(), sizeof(Chunk) * c->nchunks, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
     // permute_vector = identity (parallel_binning_init, identical_particles_binning.cpp:37-44)
-    if ((e = hipMemcpy(v.perm, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
+    if ((e = hipMemcpy(v.perm_buf, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
+    if ((e = hipMemset(v.bflag, 0, sizeof(int) * 2)) != hipSuccess) return bail("hipMemset", e);
+    v.par = 0;
+    if ((e = hipMemset(v.cell_of, 0, sizeof(int) * N)) != hipSuccess) return bail("hipMemset", e);
     if ((e = hipMemcpy(v.invperm, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice)) != hipSuccess) return bail("hipMemcpy", e);
     if ((e = hipMemset(v.count, 0, sizeof(int) * Cn)) != hipSuccess) return bail("hipMemset", e);
-    if ((e = hipMemset(v.ticket, 0, sizeof(int) * nchains)) != hipSuccess) return bail("hipMemset", e);
     if ((e = hipMemset(v.traj, 0, sizeof(int))) != hipSuccess) return bail("hipMemset", e);
     if ((e = hipMemset(v.state, 0, sizeof(ChainState) * nchains)) != hipSuccess) return bail("hipMemset", e);
     if ((e = hipMemset(v.red, 0, sizeof(double) * nchains * RED_NSLOT)) != hipSuccess) return bail("hipMemset", e);
@@ -300,7 +304,8 @@ extern "C" int chmc_set_positions(chmc_ctx* c, int chain, const double* xyz)
         }
     int rc = flush_drift(c);
     if (rc) return rc;
-    c->bins_valid = false;
+    c->bin_state = 2;
+    c->have_bins = false;     // arbitrary new positions: the next binning takes the general path
     c->force_kind = 0;
     return put_soa(c, c->v.x, chain, xyz);
 }
@@ -366,55 +371,87 @@ extern "C" int chmc_init_lattice(chmc_ctx* c, uint64_t seed, double jitter)
 static bool fp64_md(const chmc_ctx* c) { return (c->flags & CHMC_FORCE_FP64) != 0; }
 static bool id_order(const chmc_ctx* c) { return (c->flags & CHMC_ORDER_ATOM_ID) != 0; }
 
+static int ensure_bins(chmc_ctx* c);
+
+// a count whose placement is still pending is placed before positions change again, so that every
+// count compares against the cells of a completed placement
+static int settle_bins(chmc_ctx* c)
+{
+    return c->bin_state == 1 ? ensure_bins(c) : CHMC_OK;
+}
+
 static int flush_drift(chmc_ctx* c)
 {
     if (c->pending_drift != 0.0) {
         HIPCHK(c, launch_drift(c->v, c->pending_drift, c->stream));
         c->pending_drift = 0.0;
-        c->bins_valid = false;
+        c->bin_state = 2;
     }
     return CHMC_OK;
 }
 
-// binning of the current positions, with a deferred drift applied first inside the same kernel
+// bring the bins up to date with the current positions: a deferred drift is applied inside the count
+// kernel; a counted state only needs the placement
+static int ensure_bins(chmc_ctx* c)
+{
+    if (c->pending_drift != 0.0 || c->bin_state == 2) {
+        HIPCHK(c, launch_count(c->v, c->pending_drift, c->stream));
+        c->pending_drift = 0.0;
+        c->bin_state = 1;
+    }
+    if (c->bin_state == 1) {
+        HIPCHK(c, launch_scan(c->v, c->stream));
+        HIPCHK(c, launch_place(c->v, c->have_bins && !id_order(c), id_order(c), c->stream));
+        c->v.par ^= 1;                 // the placement wrote the other perm/offs buffer
+        c->bin_state = 0;
+        c->have_bins = true;
+    }
+    return CHMC_OK;
+}
+
+// create_binning: always a fresh binning of the current positions
 static int do_bin(chmc_ctx* c)
 {
-    HIPCHK(c, launch_bin(c->v, c->pending_drift, id_order(c), c->stream));
-    c->pending_drift = 0.0;
-    c->bins_valid = true;
-    return CHMC_OK;
+    if (c->bin_state == 0) c->bin_state = 2;
+    return ensure_bins(c);
 }
 
 static int do_drift(chmc_ctx* c, double dt)
 {
     int rc = flush_drift(c);
     if (rc) return rc;
+    if ((rc = settle_bins(c))) return rc;
     c->pending_drift = dt;
-    c->bins_valid = false;
+    c->bin_state = 2;
     c->force_kind = 0;
     return CHMC_OK;
 }
 
-// update_momenta: force with fresh bins, then the kick, in one sweep
-static int do_force_kick(chmc_ctx* c, double dt, bool storef)
+// update_momenta (force with fresh bins, then the kick) in one sweep; with fuse_drift the sweep also
+// applies the following update_positions(drift_dt) to each atom and counts it into its new cell
+static int do_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift, double drift_dt)
 {
-    int rc = do_bin(c);
+    int rc = ensure_bins(c);
     if (rc) return rc;
-    const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0);
-    HIPCHK(c, launch_lj(c->v, mode, fp64_md(c), c->islots, dt, c->stream));
-    if (storef) c->force_kind = fp64_md(c) ? 2 : 1;
+    const bool fuse = fuse_drift && !fp64_md(c);
+    const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0) | (fuse ? LJ_DRIFT : 0);
+    HIPCHK(c, launch_lj(c->v, mode, fp64_md(c), c->islots, dt, fuse ? drift_dt : 0.0, c->stream));
+    c->force_kind = (storef && !fuse) ? (fp64_md(c) ? 2 : 1) : 0;
+    if (fuse) c->bin_state = 1;
+    else if (fuse_drift) return do_drift(c, drift_dt);
     return CHMC_OK;
 }
 
-static int do_energy(chmc_ctx* c, bool rebin)
+// FP64 energy / virial / kinetic sweep; `sums` adds the per-chain reduction into red[] (the HMC
+// trajectory leaves it to the Metropolis kernel)
+static int do_energy(chmc_ctx* c, bool rebin, bool sums = true)
 {
     int rc = flush_drift(c);
     if (rc) return rc;
-    if (rebin || !c->bins_valid) {
-        rc = do_bin(c);
-        if (rc) return rc;
-    }
-    HIPCHK(c, launch_lj(c->v, LJ_ENERGY, true, c->islots, 0.0, c->stream));
+    rc = rebin ? do_bin(c) : ensure_bins(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_lj(c->v, LJ_ENERGY, true, c->islots, 0.0, 0.0, c->stream));
+    if (sums) HIPCHK(c, launch_reduce(c->v, SUM_CELLS, c->stream));
     return CHMC_OK;
 }
 
@@ -430,9 +467,7 @@ static int read_red(chmc_ctx* c, int slot, double* out)
 extern "C" int chmc_bin(chmc_ctx* c)
 {
     CHECK_CTX(c);
-    int rc = flush_drift(c);
-    if (rc) return rc;
-    rc = do_bin(c);
+    int rc = do_bin(c);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
@@ -441,11 +476,14 @@ extern "C" int chmc_bin(chmc_ctx* c)
 extern "C" int chmc_get_bins(chmc_ctx* c, int chain, int* bincount, int* binoffsets, int* permute_vector)
 {
     if (!c || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
-    if (!c->bins_valid) return fail(c, CHMC_E_STATE, "no binning of the current positions");
+    if (c->bin_state != 0 || c->pending_drift != 0.0) return fail(c, CHMC_E_STATE, "no binning of the current positions");
     const ChainDev& cd = c->hchains[chain];
+    const int par = c->v.par;
+    const int* offs_d = c->v.offs_buf + (size_t)par * (c->Ctot + 1);
+    const int* perm_d = c->v.perm_buf + (size_t)par * c->Ntot;
     std::vector<int> offs(cd.ncells + 1), perm(cd.natoms);
-    HIPCHK(c, hipMemcpyAsync(offs.data(), c->v.offs + cd.cell_off, sizeof(int) * (cd.ncells + 1), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(perm.data(), c->v.perm + cd.atom_off, sizeof(int) * cd.natoms, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(offs.data(), offs_d + cd.cell_off, sizeof(int) * (cd.ncells + 1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(perm.data(), perm_d + cd.atom_off, sizeof(int) * cd.natoms, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int k = 0; k < cd.ncells; k++) {
         if (bincount) bincount[k] = offs[k + 1] - offs[k];
@@ -459,11 +497,9 @@ extern "C" int chmc_get_bins(chmc_ctx* c, int chain, int* bincount, int* binoffs
 extern "C" int chmc_compute_force(chmc_ctx* c, int fp64)
 {
     CHECK_CTX(c);
-    int rc = flush_drift(c);
+    int rc = do_bin(c);
     if (rc) return rc;
-    rc = do_bin(c);
-    if (rc) return rc;
-    HIPCHK(c, launch_lj(c->v, LJ_FORCE | LJ_STOREF, fp64 != 0, c->islots, 0.0, c->stream));
+    HIPCHK(c, launch_lj(c->v, LJ_FORCE | LJ_STOREF, fp64 != 0, c->islots, 0.0, 0.0, c->stream));
     c->force_kind = fp64 ? 2 : 1;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
@@ -490,20 +526,21 @@ extern "C" int chmc_evaluate_potential(chmc_ctx* c, double* V)
     if (!c || !V) return CHMC_E_ARG;
     int rc = flush_drift(c);
     if (rc) return rc;
-    if (!c->bins_valid) {
-        // evaluate_potential uses the bins as they are; positions moved since the last binning keep
-        // their old cells (identical_particles.cpp:387-393), which needs a binning to exist at all
-        rc = do_bin(c);
+    if (!c->have_bins || c->bin_state == 1) {
+        // evaluate_potential uses the bins as they are (identical_particles.cpp:387-393): positions moved
+        // since the last binning keep their old cells; a binning must exist at all
+        rc = ensure_bins(c);
         if (rc) return rc;
     }
-    HIPCHK(c, launch_lj(c->v, LJ_ENERGY, true, c->islots, 0.0, c->stream));
+    HIPCHK(c, launch_lj(c->v, LJ_ENERGY, true, c->islots, 0.0, 0.0, c->stream));
+    HIPCHK(c, launch_reduce(c->v, SUM_CELLS, c->stream));
     return read_red(c, RED_V, V);
 }
 
 extern "C" int chmc_virial(chmc_ctx* c, double* W)
 {
     if (!c || !W) return CHMC_E_ARG;
-    int rc = do_energy(c, !c->bins_valid);
+    int rc = do_energy(c, false);
     if (rc) return rc;
     return read_red(c, RED_W, W);
 }
@@ -511,7 +548,7 @@ extern "C" int chmc_virial(chmc_ctx* c, double* W)
 extern "C" int chmc_pair_counts(chmc_ctx* c, int chain, long long* ncand, long long* nint)
 {
     if (!c || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
-    int rc = do_energy(c, !c->bins_valid);
+    int rc = do_energy(c, false);
     if (rc) return rc;
     std::vector<double> ni(c->nchains);
     rc = read_red(c, RED_NINT, ni.data());
@@ -519,7 +556,9 @@ extern "C" int chmc_pair_counts(chmc_ctx* c, int chain, long long* ncand, long l
     // candidates of the 27-cell stencil: sum over cells of count(c) * sum_{27 neighbours} count(n), minus self
     const ChainDev& cd = c->hchains[chain];
     std::vector<int> offs(cd.ncells + 1);
-    HIPCHK(c, hipMemcpy(offs.data(), c->v.offs + cd.cell_off, sizeof(int) * (cd.ncells + 1), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int* offs_d = c->v.offs_buf + (size_t)c->v.par * (c->Ctot + 1);
+    HIPCHK(c, hipMemcpy(offs.data(), offs_d + cd.cell_off, sizeof(int) * (cd.ncells + 1), hipMemcpyDeviceToHost));
     const int nx = cd.nbin[0], ny = cd.nbin[1], nz = cd.nbin[2];
     long long cand = 0;
     for (int bz = 0; bz < nz; bz++)
@@ -544,14 +583,17 @@ extern "C" int chmc_pair_counts(chmc_ctx* c, int chain, long long* ncand, long l
 extern "C" int chmc_kinetic(chmc_ctx* c, double* K)
 {
     if (!c || !K) return CHMC_E_ARG;
-    HIPCHK(c, launch_kinetic(c->v, RED_K, c->stream));
+    HIPCHK(c, launch_kinetic(c->v, c->stream));
+    HIPCHK(c, launch_reduce(c->v, SUM_CHUNKS_K, c->stream));
     return read_red(c, RED_K, K);
 }
 
 extern "C" int chmc_update_momenta(chmc_ctx* c, double dt)
 {
     CHECK_CTX(c);
-    int rc = do_force_kick(c, dt, true);
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    rc = do_force_kick(c, dt, true, false, 0.0);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
@@ -562,8 +604,9 @@ extern "C" int chmc_update_positions(chmc_ctx* c, double dt)
     CHECK_CTX(c);
     int rc = flush_drift(c);
     if (rc) return rc;
+    if ((rc = settle_bins(c))) return rc;
     HIPCHK(c, launch_drift(c->v, dt, c->stream));
-    c->bins_valid = false;
+    c->bin_state = 2;
     c->force_kind = 0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
@@ -578,50 +621,65 @@ extern "C" int chmc_kick(chmc_ctx* c, double dt)
     return CHMC_OK;
 }
 
-// integrator_type::integrate (integrator.cpp:52-131; code-carsten integrator.hh:200-240).  Drifts are
-// deferred and fused into the binning that starts the next force evaluation; storef keeps the force
-// of the last evaluation readable.
-static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double lambda, bool storef)
+// integrator_type::integrate (integrator.cpp:52-131; code-carsten integrator.hh:200-240) as a list of
+// update_momenta (M) / update_positions (X) calls.  Every scheme alternates M and X; an M followed by an
+// X runs as one sweep (kick, then the drift and the count of the next binning); a trailing X is
+// deferred into the next binning.  storef keeps the force of the last evaluation readable.
+struct MdOp { bool kick; double dt; };
+
+static int build_ops(int scheme, double dt, int steps, double lambda, std::vector<MdOp>& ops)
 {
-    auto M = [&](double h) { return do_force_kick(c, h, storef); };
-    auto X = [&](double h) { return do_drift(c, h); };
-    int rc = 0;
-#define RUN(expr) do { rc = (expr); if (rc) return rc; } while (0)
+    auto M = [&](double h) { ops.push_back(MdOp{ true, h }); };
+    auto X = [&](double h) { ops.push_back(MdOp{ false, h }); };
     if (scheme == CHMC_LEAPFROG) {
-        RUN(M(dt / 2.));
-        RUN(X(dt));
-        for (int i = 0; i < steps - 1; i++) { RUN(M(dt)); RUN(X(dt)); }
-        RUN(M(dt / 2.));
+        M(dt / 2.);
+        X(dt);
+        for (int i = 0; i < steps - 1; i++) { M(dt); X(dt); }
+        M(dt / 2.);
     } else if (scheme == CHMC_OMF2) {
         const double om2l = 1. - 2. * lambda;
-        RUN(M(lambda * dt));
-        for (int i = 0; i < steps - 1; i++) {
-            RUN(X(dt / 2.)); RUN(M(om2l * dt)); RUN(X(dt / 2.)); RUN(M(2. * lambda * dt));
-        }
-        RUN(X(dt / 2.)); RUN(M(om2l * dt)); RUN(X(dt / 2.)); RUN(M(lambda * dt));
+        M(lambda * dt);
+        for (int i = 0; i < steps - 1; i++) { X(dt / 2.); M(om2l * dt); X(dt / 2.); M(2. * lambda * dt); }
+        X(dt / 2.); M(om2l * dt); X(dt / 2.); M(lambda * dt);
     } else if (scheme == CHMC_OMF4) {
         const double rho = 0.2539785108410595, theta = -0.03230286765269967;
         const double vartheta = 0.08398315262876693, lam = 0.6822365335719091;
         const double e[10] = { rho * dt, lam * dt, theta * dt, 0.5 * (1 - 2. * (lam + vartheta)) * dt,
                                (1 - 2. * (theta + rho)) * dt, 0.5 * (1 - 2. * (lam + vartheta)) * dt,
                                theta * dt, lam * dt, rho * dt, 2 * vartheta * dt };
-        RUN(M(0.5 * e[9]));
+        M(0.5 * e[9]);
         for (int i = 1; i < steps - 1; i++)      // the reference starts at i = 1 (integrator.cpp:118)
-            for (int j = 0; j < 5; j++) { RUN(X(e[2 * j])); RUN(M(e[2 * j + 1])); }
-        for (int j = 0; j < 4; j++) { RUN(X(e[2 * j])); RUN(M(e[2 * j + 1])); }
-        RUN(X(e[8]));
-        RUN(M(0.5 * e[9]));
+            for (int j = 0; j < 5; j++) { X(e[2 * j]); M(e[2 * j + 1]); }
+        for (int j = 0; j < 4; j++) { X(e[2 * j]); M(e[2 * j + 1]); }
+        X(e[8]);
+        M(0.5 * e[9]);
     } else if (scheme == CHMC_EULER) {
-        for (int i = 0; i < steps; i++) { RUN(M(dt)); RUN(X(dt)); }
+        for (int i = 0; i < steps; i++) { M(dt); X(dt); }
     } else if (scheme == CHMC_RUTH) {
         for (int i = 0; i < steps; i++) {
-            RUN(M(dt)); RUN(X(-1. / 24. * dt)); RUN(M(-2. / 3. * dt)); RUN(X(3. / 4. * dt));
-            RUN(M(2. / 3. * dt)); RUN(X(7. / 24. * dt));
+            M(dt); X(-1. / 24. * dt); M(-2. / 3. * dt); X(3. / 4. * dt); M(2. / 3. * dt); X(7. / 24. * dt);
         }
     } else {
-        return fail(c, CHMC_E_ARG, "unknown integrator scheme");
+        return -1;
     }
-#undef RUN
+    return 0;
+}
+
+static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double lambda, bool storef)
+{
+    std::vector<MdOp> ops;
+    if (build_ops(scheme, dt, steps, lambda, ops) != 0) return fail(c, CHMC_E_ARG, "unknown integrator scheme");
+    for (size_t k = 0; k < ops.size(); k++) {
+        int rc;
+        if (ops[k].kick) {
+            const bool next_x = k + 1 < ops.size() && !ops[k + 1].kick;
+            rc = do_force_kick(c, ops[k].dt, storef, next_x, next_x ? ops[k + 1].dt : 0.0);
+            if (next_x) k++;
+        } else {
+            rc = do_drift(c, ops[k].dt);
+        }
+        if (rc) return rc;
+    }
     return CHMC_OK;
 }
 
@@ -647,7 +705,7 @@ extern "C" int chmc_integrate(chmc_ctx* c, int scheme, double dt, int steps, dou
 extern "C" int chmc_heatbath(chmc_ctx* c, uint64_t seed, uint64_t traj)
 {
     CHECK_CTX(c);
-    HIPCHK(c, launch_heatbath(c->v, seed, (long long)traj, c->stream));
+    HIPCHK(c, launch_heatbath(c->v, seed, (long long)traj, false, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
 }
@@ -667,8 +725,9 @@ extern "C" int chmc_restore_positions(chmc_ctx* c)
     CHECK_CTX(c);
     int rc = flush_drift(c);
     if (rc) return rc;
+    if ((rc = settle_bins(c))) return rc;
     HIPCHK(c, launch_save_restore(c->v, 1, c->stream));
-    c->bins_valid = false;
+    c->bin_state = 2;
     c->force_kind = 0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
@@ -726,17 +785,18 @@ struct Mt64 {
 static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps, int ufirst, int rec_base)
 {
     int rc;
-    HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, c->stream));
-    HIPCHK(c, launch_kinetic(c->v, RED_KI, c->stream));
+    // heat bath + K_i + the count phase of the binning of the (accepted or restored) positions
+    HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, true, c->stream));
+    c->bin_state = 1;
     // a zero-length binomial draw leaves x and p unchanged (the reference's size_t loop would not end, Q18)
     if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false))) return rc;
     // V_f: after LEAP/OMF2/OMF4 the last binning is of the final positions; after a closing drift
     // (euler/ruth) it is redone -- the reference rebins in either case (identical_particles.cpp:379)
-    if ((rc = do_energy(c, false))) return rc;
-    HIPCHK(c, launch_kinetic(c->v, RED_K, c->stream));
+    if ((rc = do_energy(c, false, false))) return rc;               // V_f, W_f and K_f partials
     HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, c->stream));
     HIPCHK(c, launch_save_restore(c->v, 2, c->stream));
     c->force_kind = 0;
+    c->bin_state = 2;
     return CHMC_OK;
 }
 
@@ -796,23 +856,38 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     HIPCHK(c, launch_save_restore(c->v, 0, c->stream));
 
     if (hp->use_graph && !hp->randomize) {
+        // one trajectory per graph when it performs an even number of placements (the binning parity
+        // baked into the captured launches then repeats), two otherwise
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        const int par0 = c->v.par;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         rc = enqueue_trajectory(c, *hp, hp->steps, ufirst, rec_base);
+        int per_graph = 1;
+        if (!rc && c->v.par != par0 && ntraj >= 2) {
+            rc = enqueue_trajectory(c, *hp, hp->steps, ufirst, rec_base);
+            per_graph = 2;
+        }
         hipError_t ec = hipStreamEndCapture(c->stream, &graph);
         if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
         HIPCHK(c, ec);
         hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (ei != hipSuccess) { hipGraphDestroy(graph); HIPCHK(c, ei); }
         hipError_t el = hipSuccess;
-        for (int t = 0; t < ntraj && el == hipSuccess; t++) el = hipGraphLaunch(exec, c->stream);
+        const int nlaunch = ntraj / per_graph;
+        for (int t = 0; t < nlaunch && el == hipSuccess; t++) el = hipGraphLaunch(exec, c->stream);
         hipError_t es = hipStreamSynchronize(c->stream);
         hipGraphExecDestroy(exec);
         hipGraphDestroy(graph);
         HIPCHK(c, el);
         HIPCHK(c, es);
+        // the host state after capture is that of per_graph trajectories; replays match it when
+        // per_graph == 2 or the parity is unchanged; a leftover trajectory runs directly
+        for (int t = nlaunch * per_graph; t < ntraj; t++) {
+            if ((rc = enqueue_trajectory(c, *hp, hp->steps, ufirst, rec_base))) return rc;
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     } else {
         for (int t = 0; t < ntraj; t++) {
             const int steps = hp->randomize ? rsteps[t] : hp->steps;
@@ -822,7 +897,7 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     c->traj_done += ntraj;
-    c->bins_valid = false;
+    c->bin_state = 2;
     if (records) HIPCHK(c, hipMemcpy(records, c->d_rec, sizeof(HmcRecord) * nrec, hipMemcpyDeviceToHost));
     if (nacc) {
         HIPCHK(c, hipMemcpy(st.data(), c->v.state, sizeof(ChainState) * nch, hipMemcpyDeviceToHost));
@@ -904,17 +979,25 @@ extern "C" int chmc_potential_energy_open(const double* xyz, int n_particles, in
 // ---- tuning ---------------------------------------------------------------------------------------
 static int time_force_is(chmc_ctx* c, int islots, int reps, double* ms_bin, double* ms_force)
 {
-    int rc = flush_drift(c);
+    int rc = do_bin(c);
     if (rc) return rc;
+    const bool local = !id_order(c);
+    auto rebin = [&]() -> hipError_t {
+        hipError_t e = launch_count(c->v, 0.0, c->stream);
+        if (e == hipSuccess) e = launch_scan(c->v, c->stream);
+        if (e == hipSuccess) e = launch_place(c->v, local, id_order(c), c->stream);
+        c->v.par ^= 1;
+        return e;
+    };
     // warm-up
-    HIPCHK(c, launch_bin(c->v, 0.0, id_order(c), c->stream));
-    HIPCHK(c, launch_lj(c->v, LJ_FORCE | LJ_STOREF, false, islots, 0.0, c->stream));
+    HIPCHK(c, rebin());
+    HIPCHK(c, launch_lj(c->v, LJ_FORCE | LJ_STOREF, false, islots, 0.0, 0.0, c->stream));
     float tb = 0.f, tf = 0.f;
     for (int r = 0; r < reps; r++) {
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        HIPCHK(c, launch_bin(c->v, 0.0, id_order(c), c->stream));
+        HIPCHK(c, rebin());
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-        HIPCHK(c, launch_lj(c->v, LJ_FORCE | LJ_STOREF, false, islots, 0.0, c->stream));
+        HIPCHK(c, launch_lj(c->v, LJ_FORCE | LJ_STOREF, false, islots, 0.0, 0.0, c->stream));
         HIPCHK(c, hipEventRecord(c->ev2, c->stream));
         HIPCHK(c, hipEventSynchronize(c->ev2));
         float a = 0.f, b = 0.f;
@@ -923,23 +1006,26 @@ static int time_force_is(chmc_ctx* c, int islots, int reps, double* ms_bin, doub
         tb += a;
         tf += b;
     }
-    c->bins_valid = true;
+    c->bin_state = 0;
     c->force_kind = 1;
     if (ms_bin) *ms_bin = tb / reps;
     if (ms_force) *ms_force = tf / reps;
     return CHMC_OK;
 }
 
+static const int kTilings[8] = { 41, 42, 44, 81, 82, 84, 161, 162 };
+
 extern "C" int chmc_set_force_tiling(chmc_ctx* c, int islots)
 {
     CHECK_CTX(c);
-    if (islots == 4 || islots == 8 || islots == 16) {
-        c->islots = islots;
-        return CHMC_OK;
-    }
-    if (islots != 0) return fail(c, CHMC_E_ARG, "islots must be 0, 4, 8 or 16");
+    for (int t : kTilings)
+        if (islots == t) {
+            c->islots = islots;
+            return CHMC_OK;
+        }
+    if (islots != 0) return fail(c, CHMC_E_ARG, "tiling must be 0 (autotune) or i-slots*10+waves from {41,42,44,81,82,84,161,162}");
     // autotuner: time each tiling of the FP32 sweep on the current configuration, keep the fastest
-    const int cand[3] = { 4, 8, 16 };
+    const int (&cand)[8] = kTilings;
     double best = 1e30;
     int best_is = c->islots;
     for (int is : cand) {

@@ -5,12 +5,16 @@
 //   f32           : float  SoA [3][Ntot]      (+grad U, FP32 pair sums)
 //   f64           : double SoA [3][Ntot]      (+grad U, FP64 path)
 //   atom_chain    : int [Ntot]
-//   cell_of, slot : int [Ntot]                (binning scratch: cell id, arrival slot)
+//   cell_of, slot : int [Ntot]                (cell of each atom at the last binning, arrival slot)
+//   newcell       : int [Ntot]                (new cell id, indexed by the atom's previous sorted slot)
 //   count         : int [Ctot]                (zero between binnings; the scan clears it)
-//   offs          : int [Ctot + 1]            (global exclusive scan of the counts)
+//   offs_buf      : int [2][Ctot + 1]         (global exclusive scans of the counts: current / next)
 //   skey, satom   : int [Ntot]                (unsorted within-cell scatter, sort key and atom)
-//   perm, invperm : int [Ntot]                (sorted slot -> atom, atom -> sorted slot)
+//   perm_buf      : int [2][Ntot]             (sorted slot -> atom: current / next)
+//   invperm       : int [Ntot]                (atom -> current sorted slot)
 //   xs            : float4 [Ntot]             (sorted cell-relative FP32 coordinates, atom id in w)
+//   bflag         : int [2]                   (locality-violation flag, by target buffer)
+//   par (argument): which perm/offs buffer is current; the host flips it after every placement
 //   cell_chain    : int [Ctot]
 // Chains are contiguous in both the atom and the cell index spaces, so the global scan of the
 // per-cell counts equals chain atom offset + the chain's own exclusive scan.
@@ -48,7 +52,7 @@ struct HmcRecord {
 struct Chunk { int chain, begin, end, pad_; };
 
 constexpr int kMaxJ = 768;         // j-list entries staged in LDS per pass
-constexpr int kRedChunk = 4096;    // atoms per deterministic reduction chunk
+constexpr int kRedChunk = 1024;    // atoms per deterministic reduction chunk
 
 // ---- launch wrappers (each defined in the TU that owns the kernel) ----------------------
 struct DevView {
@@ -59,24 +63,40 @@ struct DevView {
     const Chunk* chunks;
     double *x, *p, *x_saved, *f64;
     float* f32;
-    int *cell_of, *slot, *count, *offs, *skey, *satom, *perm, *invperm;
+    int *cell_of, *slot, *newcell, *count, *offs_buf, *skey, *satom, *perm_buf, *invperm, *bflag;
+    int par;              // current binning buffer (0/1)
     float4* xs;
     double* red;          // [nchains][RED_NSLOT]
-    double* chunk_part;   // [nchunks * 2]
-    double* cell_part;    // [Ctot * 3]: energy, virial, interacting pairs
-    int* ticket;          // [nchains]
+    double* chunk_part;   // [nchunks]: per-chunk kinetic energy
+    double* cell_part;    // [Ctot * 4]: energy, virial, interacting pairs, kinetic energy
     ChainState* state;    // [nchains]
     int* traj;            // [1] device trajectory counter (1-based index of the last finished one)
 };
 
-enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8 };
+enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8, LJ_DRIFT = 16 };
 
-hipError_t launch_bin(const DevView& v, double drift_dt, bool atom_id_order, hipStream_t s);
+// current / next binning buffers (the parity of the completed-binning counter selects them)
+__device__ __forceinline__ int bin_parity(const DevView& v) { return v.par; }
+__device__ __forceinline__ const int* offs_cur(const DevView& v) { return v.offs_buf + bin_parity(v) * (v.Ctot + 1); }
+__device__ __forceinline__ int* offs_next(const DevView& v) { return v.offs_buf + (bin_parity(v) ^ 1) * (v.Ctot + 1); }
+__device__ __forceinline__ const int* perm_cur(const DevView& v) { return v.perm_buf + bin_parity(v) * v.Ntot; }
+__device__ __forceinline__ int* perm_next(const DevView& v) { return v.perm_buf + (bin_parity(v) ^ 1) * v.Ntot; }
+
+// binning: count (cell, slot, new cell at the previous slot) -> scan -> placement, either general
+// (scatter + rank by key) or local (collect from the previous neighbour cells)
+hipError_t launch_count(const DevView& v, double drift_dt, hipStream_t s);
+hipError_t launch_scan(const DevView& v, hipStream_t s);
+hipError_t launch_place(const DevView& v, bool local, bool atom_id_order, hipStream_t s);
+
+// per-chain sums of the per-cell (V, W, pairs, K) and per-chunk (K) partials into red[]
+enum { SUM_CELLS = 1, SUM_CHUNKS_KI = 2, SUM_CHUNKS_K = 4 };
+hipError_t launch_reduce(const DevView& v, int what, hipStream_t s);
 hipError_t launch_drift(const DevView& v, double dt, hipStream_t s);
 hipError_t launch_kick(const DevView& v, double dt, bool fp64, hipStream_t s);
-hipError_t launch_lj(const DevView& v, int mode, bool fp64, int islots, double kick_dt, hipStream_t s);
-hipError_t launch_kinetic(const DevView& v, int slot, hipStream_t s);
-hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, hipStream_t s);
+// tiling: i-slots per wave (4, 8, 16) x waves per workgroup (1, 2, 4), encoded islots * 10 + waves
+hipError_t launch_lj(const DevView& v, int mode, bool fp64, int tiling, double kick_dt, double drift_dt, hipStream_t s);
+hipError_t launch_kinetic(const DevView& v, hipStream_t s);
+hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, hipStream_t s);
 hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustride, int therm, int ufirst,
                              HmcRecord* rec, int rec_base, hipStream_t s);
 hipError_t launch_save_restore(const DevView& v, int mode, hipStream_t s);

@@ -152,9 +152,10 @@ int chmc_rdf(chmc_ctx* ctx, int chain, int nbins, double size_bin, unsigned long
 int chmc_potential_energy_open(const double* xyz, int n_particles, int n, double a, double b, double* result);
 
 /* ---- tuning ----------------------------------------------------------------------------------- */
-/* i-slots per wavefront of the LJ sweep (4, 8 or 16; 0 = run the autotuner on the current state) */
-int chmc_set_force_tiling(chmc_ctx* ctx, int islots);
-int chmc_get_force_tiling(const chmc_ctx* ctx, int* islots);
+/* tiling of the FP32 LJ sweep, encoded i-slots per wavefront * 10 + wavefronts per workgroup, one of
+ * 41, 42, 44, 81, 82, 84, 161, 162; 0 runs the autotuner (times every tiling on the current state) */
+int chmc_set_force_tiling(chmc_ctx* ctx, int tiling);
+int chmc_get_force_tiling(const chmc_ctx* ctx, int* tiling);
 /* average device time (ms) of one force evaluation (binning + FP32 sweep + kick) over reps */
 int chmc_time_force(chmc_ctx* ctx, int reps, double* ms_bin, double* ms_force);
 
