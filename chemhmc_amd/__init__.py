@@ -34,10 +34,12 @@ class ChainDesc(C.Structure):
 
 class HmcParams(C.Structure):
     _fields_ = [("ntraj", _I), ("therm", _I), ("scheme", _I), ("steps", _I), ("dt", _D), ("lam", _D),
-                ("seed", _U64), ("use_graph", _I), ("randomize", _I)]
+                ("seed", _U64), ("use_graph", _I), ("randomize", _I), ("surrogate", _I),
+                ("fallback_window", _I), ("fallback_acc", _D)]
 
 
-HMC_RECORD_DTYPE = np.dtype([("dh", "f8"), ("u", "f8"), ("k", "f8"), ("w", "f8"), ("accepted", "i4"), ("pad", "i4")])
+HMC_RECORD_DTYPE = np.dtype([("dh", "f8"), ("u", "f8"), ("k", "f8"), ("w", "f8"), ("accepted", "i4"),
+                             ("surrogate", "i4")])
 
 
 class ChmcError(RuntimeError):
@@ -100,6 +102,15 @@ def _declare(L):
         "chmc_set_force_tiling": ([P, _I], _I),
         "chmc_get_force_tiling": ([P, C.POINTER(_I)], _I),
         "chmc_time_force": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
+        "chmc_surrogate_set_params": ([P, C.c_void_p, _I], _I),
+        "chmc_surrogate_get_params": ([P, C.c_void_p, _I], _I),
+        "chmc_surrogate_eval": ([P, C.c_void_p, _I], _I),
+        "chmc_surrogate_features": ([P, _I, C.c_void_p, C.c_void_p], _I),
+        "chmc_surrogate_reserve": ([P, _I], _I),
+        "chmc_surrogate_collect": ([P, C.POINTER(C.c_longlong)], _I),
+        "chmc_surrogate_training_data": ([P, C.c_longlong, _I, C.c_void_p, C.c_void_p], _I),
+        "chmc_surrogate_loss_grad": ([P, C.c_longlong, _I, C.c_void_p], _I),
+        "chmc_surrogate_train": ([P, _I, _I, C.c_float, _U64, C.c_void_p], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -114,8 +125,32 @@ EXPORTED = (
     "chmc_evaluate_potential", "chmc_virial", "chmc_pair_counts", "chmc_kinetic", "chmc_update_momenta", "chmc_update_positions",
     "chmc_kick", "chmc_integrate", "chmc_heatbath", "chmc_save_positions", "chmc_restore_positions",
     "chmc_hmc_run", "chmc_rdf", "chmc_potential_energy_open", "chmc_set_force_tiling",
-    "chmc_get_force_tiling", "chmc_time_force",
+    "chmc_get_force_tiling", "chmc_time_force", "chmc_surrogate_set_params", "chmc_surrogate_get_params",
+    "chmc_surrogate_eval", "chmc_surrogate_features", "chmc_surrogate_reserve", "chmc_surrogate_collect",
+    "chmc_surrogate_training_data", "chmc_surrogate_loss_grad", "chmc_surrogate_train",
 )
+
+SURR_NPARAMS, SURR_NGRAD, SURR_NFEAT, SURR_NHID = 6401, 6338, 32, 64
+# flat parameter layout: name, shape (W1[64][32], b1, W2[64][64], b2, w3, b3, shift, scale)
+SURR_LAYOUT = (("W1", (64, 32)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("w3", (64,)), ("b3", (1,)),
+               ("shift", (32,)), ("scale", (32,)))
+
+
+def pack_surrogate(p):
+    """Flat float32 vector of a parameter dict in the library's layout."""
+    return np.concatenate([np.asarray(p[k], np.float64).reshape(-1) for k, _ in SURR_LAYOUT]).astype(np.float32)
+
+
+def unpack_surrogate(v, trained_only=False):
+    """Parameter dict of a flat vector (trained_only: a gradient vector of SURR_NGRAD - 1 entries)."""
+    out, o = {}, 0
+    for k, shp in SURR_LAYOUT:
+        if trained_only and k in ("shift", "scale"):
+            break
+        n = int(np.prod(shp))
+        out[k] = np.asarray(v[o:o + n], np.float64).reshape(shp)
+        o += n
+    return out
 
 
 @dataclass
@@ -273,9 +308,10 @@ class Engine:
         self._ck(lib().chmc_restore_positions(self._h), "restore_positions")
 
     def hmc(self, ntraj, therm, scheme, dt, steps, lam=OMF2_LAMBDA_REFERENCE, seed=1, use_graph=True,
-            randomize=False):
+            randomize=False, surrogate=False, fallback_window=0, fallback_acc=0.0):
         hp = HmcParams(int(ntraj), int(therm), int(scheme), int(steps), float(dt), float(lam), int(seed),
-                       int(bool(use_graph)), int(bool(randomize)))
+                       int(bool(use_graph)), int(bool(randomize)), int(bool(surrogate)), int(fallback_window),
+                       float(fallback_acc))
         rec = np.zeros(ntraj * self.nchains, HMC_RECORD_DTYPE)
         nacc = np.zeros(self.nchains, np.int32)
         self._ck(lib().chmc_hmc_run(self._h, C.byref(hp), rec.ctypes.data, nacc.ctypes.data), "hmc_run")
@@ -286,6 +322,54 @@ class Engine:
         g = np.zeros(nbins)
         self._ck(lib().chmc_rdf(self._h, chain, int(nbins), float(size_bin), counts.ctypes.data, g.ctypes.data), "rdf")
         return counts, g
+
+    # -- learned surrogate ----------------------------------------------------------------------
+    def surrogate_set_params(self, params):
+        v = np.ascontiguousarray(params if not isinstance(params, dict) else pack_surrogate(params), np.float32)
+        self._ck(lib().chmc_surrogate_set_params(self._h, v.ctypes.data, v.size), "surrogate_set_params")
+
+    def surrogate_params(self):
+        v = np.zeros(SURR_NPARAMS, np.float32)
+        self._ck(lib().chmc_surrogate_get_params(self._h, v.ctypes.data, v.size), "surrogate_get_params")
+        return v
+
+    def surrogate_eval(self, store_force=False):
+        e = np.zeros(self.nchains)
+        self._ck(lib().chmc_surrogate_eval(self._h, e.ctypes.data, int(bool(store_force))), "surrogate_eval")
+        return e
+
+    def surrogate_features(self, chain=0):
+        n = self.chains[chain].N
+        G = np.zeros((n, SURR_NFEAT), np.float32)
+        g = np.zeros((n, SURR_NFEAT), np.float32)
+        self._ck(lib().chmc_surrogate_features(self._h, int(chain), G.ctypes.data, g.ctypes.data), "surrogate_features")
+        return G, g
+
+    def surrogate_reserve(self, nconfigs):
+        self._ck(lib().chmc_surrogate_reserve(self._h, int(nconfigs)), "surrogate_reserve")
+
+    def surrogate_collect(self):
+        r = C.c_longlong()
+        self._ck(lib().chmc_surrogate_collect(self._h, C.byref(r)), "surrogate_collect")
+        return r.value
+
+    def surrogate_training_data(self, row0, nrows):
+        G = np.zeros((nrows, SURR_NFEAT), np.float32)
+        u = np.zeros(nrows, np.float32)
+        self._ck(lib().chmc_surrogate_training_data(self._h, int(row0), int(nrows), G.ctypes.data, u.ctypes.data),
+                 "surrogate_training_data")
+        return G, u
+
+    def surrogate_loss_grad(self, row0, nrows):
+        g = np.zeros(SURR_NGRAD, np.float32)
+        self._ck(lib().chmc_surrogate_loss_grad(self._h, int(row0), int(nrows), g.ctypes.data), "surrogate_loss_grad")
+        return g
+
+    def surrogate_train(self, nsteps, batch_rows, lr=1e-3, seed=1):
+        h = np.zeros(nsteps, np.float32)
+        self._ck(lib().chmc_surrogate_train(self._h, int(nsteps), int(batch_rows), float(lr), int(seed), h.ctypes.data),
+                 "surrogate_train")
+        return h
 
     # -- tuning --------------------------------------------------------------------------------
     def set_force_tiling(self, islots):

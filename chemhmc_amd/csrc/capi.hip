@@ -3,6 +3,7 @@
 #include "chmc.h"
 #include "chmc_internal.hpp"
 #include "philox.hpp"
+#include "surrogate.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -42,6 +43,17 @@ struct chmc_ctx {
     unsigned long long* d_rdf = nullptr;
     int rdf_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    // learned surrogate (allocated on first use)
+    bool surr_alloc = false, surr_params = false;
+    SurrView sv{};
+    float *sG = nullptr, *sg = nullptr, *se = nullptr;
+    float *Gtrain = nullptr, *utrain = nullptr;
+    long long train_cap = 0, train_rows = 0;
+    float* spartial = nullptr;
+    long long spartial_cap = 0;
+    float* sloss = nullptr;
+    int adam_step = 0;
+    bool md_surr = false;        // MD kicks use the surrogate's forces (set for the trajectories of an HMC run)
 };
 
 static thread_local std::string g_create_error;
@@ -94,6 +106,9 @@ static void destroy_ctx(chmc_ctx* c)
     if (c->d_uniforms) hipFree(c->d_uniforms);
     if (c->d_rec) hipFree(c->d_rec);
     if (c->d_rdf) hipFree(c->d_rdf);
+    if (c->Gtrain) hipFree(c->Gtrain);
+    if (c->utrain) hipFree(c->utrain);
+    if (c->spartial) hipFree(c->spartial);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->ev2) hipEventDestroy(c->ev2);
@@ -665,13 +680,19 @@ static int build_ops(int scheme, double dt, int steps, double lambda, std::vecto
     return 0;
 }
 
+static int do_surr_force_kick(chmc_ctx* c, double dt, bool storef);
+static int surr_geometry(chmc_ctx* c);
+static int surr_check_flag(chmc_ctx* c);
+
 static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double lambda, bool storef)
 {
     std::vector<MdOp> ops;
     if (build_ops(scheme, dt, steps, lambda, ops) != 0) return fail(c, CHMC_E_ARG, "unknown integrator scheme");
     for (size_t k = 0; k < ops.size(); k++) {
         int rc;
-        if (ops[k].kick) {
+        if (ops[k].kick && c->md_surr) {
+            rc = do_surr_force_kick(c, ops[k].dt, storef);
+        } else if (ops[k].kick) {
             const bool next_x = k + 1 < ops.size() && !ops[k + 1].kick;
             rc = do_force_kick(c, ops[k].dt, storef, next_x, next_x ? ops[k + 1].dt : 0.0);
             if (next_x) k++;
@@ -793,20 +814,86 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
     // V_f: after LEAP/OMF2/OMF4 the last binning is of the final positions; after a closing drift
     // (euler/ruth) it is redone -- the reference rebins in either case (identical_particles.cpp:379)
     if ((rc = do_energy(c, false, false))) return rc;               // V_f, W_f and K_f partials
-    HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, c->stream));
+    HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, c->md_surr ? 1 : 0,
+                                c->stream));
     HIPCHK(c, launch_save_restore(c->v, 2, c->stream));
     c->force_kind = 0;
     c->bin_state = 2;
     return CHMC_OK;
 }
 
+// a captured trajectory sequence: valid for replays that start at binning parity `par` in MD mode `surr`
+struct TrajGraph {
+    hipGraphExec_t exec;
+    int par, per_graph;
+    bool surr;
+};
+
+// trajectories t0 .. t0+count-1 of a run in the context's current MD mode, by graph replays where
+// allowed and directly otherwise
+static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count, int ufirst, int rec_base,
+                       const std::vector<int>& rsteps, std::vector<TrajGraph>& graphs)
+{
+    int rc;
+    if (!hp.use_graph || hp.randomize) {
+        for (int t = t0; t < t0 + count; t++) {
+            const int steps = hp.randomize ? rsteps[t] : hp.steps;
+            if ((rc = enqueue_trajectory(c, hp, steps, ufirst, rec_base))) return rc;
+        }
+        return CHMC_OK;
+    }
+    int done = 0;
+    const TrajGraph* g = nullptr;
+    for (const TrajGraph& k : graphs)
+        if (k.par == c->v.par && k.surr == c->md_surr) g = &k;
+    if (!g) {
+        // one trajectory per graph when it performs an even number of placements (the binning parity
+        // baked into the captured launches then repeats), two otherwise.  The host state after the
+        // capture is that of per_graph trajectories, which every replay reproduces.
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const int par0 = c->v.par;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base);
+        int per_graph = 1;
+        if (!rc && c->v.par != par0) {
+            rc = enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base);
+            per_graph = 2;
+        }
+        hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+        if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
+        HIPCHK(c, ec);
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        HIPCHK(c, ei);
+        graphs.push_back(TrajGraph{ exec, par0, per_graph, c->md_surr });
+        g = &graphs.back();
+        if (count < per_graph) {
+            // nothing of the capture ran; the host state is restored by running the leftover directly
+            // from the capture's start parity
+            c->v.par = par0;
+        } else {
+            HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
+            done = per_graph;
+        }
+    }
+    for (; done + g->per_graph <= count; done += g->per_graph) HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
+    for (; done < count; done++)
+        if ((rc = enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base))) return rc;
+    return CHMC_OK;
+}
+
 extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_record* records, int* nacc)
 {
     if (!c || !hp) return CHMC_E_ARG;
+    int rc;
     if (hp->ntraj < 1 || hp->steps < 1 || !scheme_ok(hp->scheme) || hp->therm < 0)
         return fail(c, CHMC_E_ARG, "hmc_run: ntraj >= 1, steps >= 1, therm >= 0 and a known scheme are required");
+    if (hp->surrogate && !c->surr_params) return fail(c, CHMC_E_STATE, "hmc_run: surrogate parameters not set");
+    if (hp->surrogate && (rc = surr_geometry(c))) return rc;
     const int nch = c->nchains, ntraj = hp->ntraj;
-    int rc = flush_drift(c);
+    rc = flush_drift(c);
     if (rc) return rc;
     // ---- Metropolis uniforms: chain k draws from mt19937_64(seed + k) in the reference's order ----
     const int rec_base = c->traj_done;
@@ -855,47 +942,29 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     HIPCHK(c, hipMemcpyAsync(c->v.state, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, launch_save_restore(c->v, 0, c->stream));
 
-    if (hp->use_graph && !hp->randomize) {
-        // one trajectory per graph when it performs an even number of placements (the binning parity
-        // baked into the captured launches then repeats), two otherwise
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        const int par0 = c->v.par;
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_trajectory(c, *hp, hp->steps, ufirst, rec_base);
-        int per_graph = 1;
-        if (!rc && c->v.par != par0 && ntraj >= 2) {
-            rc = enqueue_trajectory(c, *hp, hp->steps, ufirst, rec_base);
-            per_graph = 2;
-        }
-        hipError_t ec = hipStreamEndCapture(c->stream, &graph);
-        if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
-        HIPCHK(c, ec);
-        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ei != hipSuccess) { hipGraphDestroy(graph); HIPCHK(c, ei); }
-        hipError_t el = hipSuccess;
-        const int nlaunch = ntraj / per_graph;
-        for (int t = 0; t < nlaunch && el == hipSuccess; t++) el = hipGraphLaunch(exec, c->stream);
-        hipError_t es = hipStreamSynchronize(c->stream);
-        hipGraphExecDestroy(exec);
-        hipGraphDestroy(graph);
-        HIPCHK(c, el);
-        HIPCHK(c, es);
-        // the host state after capture is that of per_graph trajectories; replays match it when
-        // per_graph == 2 or the parity is unchanged; a leftover trajectory runs directly
-        for (int t = nlaunch * per_graph; t < ntraj; t++) {
-            if ((rc = enqueue_trajectory(c, *hp, hp->steps, ufirst, rec_base))) return rc;
-        }
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    } else {
-        for (int t = 0; t < ntraj; t++) {
-            const int steps = hp->randomize ? rsteps[t] : hp->steps;
-            rc = enqueue_trajectory(c, *hp, steps, ufirst, rec_base);
-            if (rc) return rc;
-        }
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+    // ---- trajectories, in windows when the surrogate drives the MD (acceptance-based fallback) ----
+    std::vector<TrajGraph> graphs;
+    c->md_surr = hp->surrogate != 0;
+    const int window = (c->md_surr && hp->fallback_window > 0) ? hp->fallback_window : ntraj;
+    std::vector<HmcRecord> wrec;
+    for (int t0 = 0; t0 < ntraj && !rc; t0 += window) {
+        const int count = std::min(window, ntraj - t0);
+        rc = run_segment(c, *hp, t0, count, ufirst, rec_base, rsteps, graphs);
+        if (rc || !c->md_surr) continue;
+        if ((rc = surr_check_flag(c))) break;
+        if (t0 + count >= ntraj) break;
+        wrec.resize((size_t)count * nch);
+        hipError_t e = hipMemcpy(wrec.data(), c->d_rec + (size_t)t0 * nch, sizeof(HmcRecord) * wrec.size(),
+                                 hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { rc = fail(c, CHMC_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)); break; }
+        int acc = 0;
+        for (const HmcRecord& r : wrec) acc += r.acc;
+        if ((double)acc < hp->fallback_acc * (double)wrec.size()) c->md_surr = false;   // LJ forces from here on
     }
+    for (TrajGraph& g : graphs) hipGraphExecDestroy(g.exec);
+    c->md_surr = false;
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     c->traj_done += ntraj;
     c->bin_state = 2;
     if (records) HIPCHK(c, hipMemcpy(records, c->d_rec, sizeof(HmcRecord) * nrec, hipMemcpyDeviceToHost));
@@ -973,6 +1042,237 @@ extern "C" int chmc_potential_energy_open(const double* xyz, int n_particles, in
     double r = 0.0;
     for (double q : parts) r += q;
     *result = r;
+    return CHMC_OK;
+}
+
+
+// ---- learned surrogate ---------------------------------------------------------------------------
+static int surr_ensure(chmc_ctx* c)
+{
+    if (c->surr_alloc) return CHMC_OK;
+    const size_t N = (size_t)c->Ntot;
+    HIPCHK(c, dalloc(c, &c->sv.params, kSurrNParams));
+    HIPCHK(c, dalloc(c, &c->sv.w1b, kSurrHid * kSurrFeat));
+    HIPCHK(c, dalloc(c, &c->sv.w2b, kSurrHid * kSurrHid));
+    HIPCHK(c, dalloc(c, &c->sv.adam_m, kSurrNTrain));
+    HIPCHK(c, dalloc(c, &c->sv.adam_v, kSurrNTrain));
+    HIPCHK(c, dalloc(c, &c->sv.nb_n, N));
+    HIPCHK(c, dalloc(c, &c->sv.nb_j, N * kSurrMaxNb));
+    HIPCHK(c, dalloc(c, &c->sv.nb_d, N * kSurrMaxNb));
+    HIPCHK(c, dalloc(c, &c->sv.nb_f, N * kSurrMaxNb));
+    HIPCHK(c, dalloc(c, &c->sv.flag, 1));
+    HIPCHK(c, dalloc(c, &c->sG, N * kSurrFeat));
+    HIPCHK(c, dalloc(c, &c->sg, N * kSurrFeat));
+    HIPCHK(c, dalloc(c, &c->se, N));
+    HIPCHK(c, dalloc(c, &c->sloss, 1 << 16));
+    HIPCHK(c, hipMemset(c->sv.flag, 0, sizeof(int)));
+    HIPCHK(c, hipMemset(c->sv.adam_m, 0, sizeof(float) * kSurrNTrain));
+    HIPCHK(c, hipMemset(c->sv.adam_v, 0, sizeof(float) * kSurrNTrain));
+    c->surr_alloc = true;
+    return CHMC_OK;
+}
+
+// the descriptors count each pair once only if no chain's box holds two images within the cutoff
+static int surr_geometry(chmc_ctx* c)
+{
+    for (const ChainDev& cd : c->hchains)
+        for (int d = 0; d < 3; d++)
+            if (cd.L[d] < 2.0 * cd.rc) return fail(c, CHMC_E_GEOMETRY, "surrogate: every box side must be >= 2 cutoff");
+    return CHMC_OK;
+}
+
+static int surr_check_flag(chmc_ctx* c)
+{
+    int f = 0;
+    HIPCHK(c, hipMemcpyAsync(&f, c->sv.flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (f) return fail(c, CHMC_E_STATE, "surrogate neighbour list overflow (more than 128 pairs per atom or 768 stencil entries)");
+    return CHMC_OK;
+}
+
+// pair list + descriptors of the current positions (bins brought up to date first)
+static int surr_descriptors(chmc_ctx* c, float* G)
+{
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    if ((rc = ensure_bins(c))) return rc;
+    HIPCHK(c, launch_surr_pairs(c->v, c->sv, c->stream));
+    HIPCHK(c, launch_surr_desc(c->v, c->sv, G, c->stream));
+    return CHMC_OK;
+}
+
+// update_momenta with the surrogate's gradient
+static int do_surr_force_kick(chmc_ctx* c, double dt, bool storef)
+{
+    int rc = surr_descriptors(c, c->sG);
+    if (rc) return rc;
+    HIPCHK(c, launch_surr_mlp(c->v, c->sv, c->sG, c->sg, nullptr, c->stream));
+    HIPCHK(c, launch_surr_force(c->v, c->sv, c->sg, dt, storef, c->stream));
+    c->force_kind = storef ? 1 : 0;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_surrogate_set_params(chmc_ctx* c, const float* params, int n)
+{
+    if (!c || !params || n != kSurrNParams) return CHMC_E_ARG;
+    int rc = surr_ensure(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(c->sv.params, params, sizeof(float) * kSurrNParams, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->sv.adam_m, 0, sizeof(float) * kSurrNTrain));
+    HIPCHK(c, hipMemset(c->sv.adam_v, 0, sizeof(float) * kSurrNTrain));
+    HIPCHK(c, launch_surr_refresh(c->sv, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->adam_step = 0;
+    c->surr_params = true;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_surrogate_get_params(chmc_ctx* c, float* params, int n)
+{
+    if (!c || !params || n != kSurrNParams) return CHMC_E_ARG;
+    if (!c->surr_params) return fail(c, CHMC_E_STATE, "surrogate parameters not set");
+    HIPCHK(c, hipMemcpy(params, c->sv.params, sizeof(float) * kSurrNParams, hipMemcpyDeviceToHost));
+    return CHMC_OK;
+}
+
+extern "C" int chmc_surrogate_eval(chmc_ctx* c, double* energy, int store_force)
+{
+    if (!c) return CHMC_E_ARG;
+    if (!c->surr_params) return fail(c, CHMC_E_STATE, "surrogate parameters not set");
+    int rc = surr_geometry(c);
+    if (rc) return rc;
+    rc = surr_descriptors(c, c->sG);
+    if (rc) return rc;
+    HIPCHK(c, launch_surr_mlp(c->v, c->sv, c->sG, c->sg, c->se, c->stream));
+    HIPCHK(c, launch_surr_force(c->v, c->sv, c->sg, 0.0, store_force != 0, c->stream));
+    if ((rc = surr_check_flag(c))) return rc;
+    if (store_force) c->force_kind = 1;
+    if (energy) {
+        std::vector<float> e(c->Ntot);
+        HIPCHK(c, hipMemcpy(e.data(), c->se, sizeof(float) * c->Ntot, hipMemcpyDeviceToHost));
+        for (int k = 0; k < c->nchains; k++) {
+            const ChainDev& cd = c->hchains[k];
+            double sum = 0.0;
+            for (int i = 0; i < cd.natoms; i++) sum += e[cd.atom_off + i];     // sorted slots of chain k
+            energy[k] = sum;
+        }
+    }
+    return CHMC_OK;
+}
+
+// descriptors G and input gradients g = dE/dG of the last chmc_surrogate_eval, in atom order
+extern "C" int chmc_surrogate_features(chmc_ctx* c, int chain, float* G, float* g)
+{
+    if (!c || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    if (!c->surr_alloc) return fail(c, CHMC_E_STATE, "surrogate not evaluated");
+    const ChainDev& cd = c->hchains[chain];
+    std::vector<int> perm(cd.natoms);
+    HIPCHK(c, hipMemcpy(perm.data(), c->v.perm_buf + (size_t)c->v.par * c->Ntot + cd.atom_off, sizeof(int) * cd.natoms,
+                        hipMemcpyDeviceToHost));
+    std::vector<float> hG((size_t)cd.natoms * kSurrFeat), hg((size_t)cd.natoms * kSurrFeat);
+    HIPCHK(c, hipMemcpy(hG.data(), c->sG + (size_t)cd.atom_off * kSurrFeat, sizeof(float) * hG.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hg.data(), c->sg + (size_t)cd.atom_off * kSurrFeat, sizeof(float) * hg.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < cd.natoms; k++) {
+        const int a = perm[k] - cd.atom_off;
+        for (int f = 0; f < kSurrFeat; f++) {
+            if (G) G[(size_t)a * kSurrFeat + f] = hG[(size_t)k * kSurrFeat + f];
+            if (g) g[(size_t)a * kSurrFeat + f] = hg[(size_t)k * kSurrFeat + f];
+        }
+    }
+    return CHMC_OK;
+}
+
+extern "C" int chmc_surrogate_reserve(chmc_ctx* c, int nconfigs)
+{
+    if (!c || nconfigs < 1) return CHMC_E_ARG;
+    int rc = surr_ensure(c);
+    if (rc) return rc;
+    const long long cap = (long long)nconfigs * c->Ntot;
+    if (c->Gtrain) hipFree(c->Gtrain);
+    if (c->utrain) hipFree(c->utrain);
+    c->Gtrain = nullptr;
+    c->utrain = nullptr;
+    HIPCHK(c, hipMalloc(&c->Gtrain, sizeof(float) * (size_t)cap * kSurrFeat));
+    HIPCHK(c, hipMalloc(&c->utrain, sizeof(float) * (size_t)cap));
+    c->train_cap = cap;
+    c->train_rows = 0;
+    return CHMC_OK;
+}
+
+// appends the current configuration of every chain (descriptors and per-atom LJ energies) to the
+// device-resident training set
+extern "C" int chmc_surrogate_collect(chmc_ctx* c, long long* rows)
+{
+    if (!c) return CHMC_E_ARG;
+    if (!c->Gtrain) return fail(c, CHMC_E_STATE, "call chmc_surrogate_reserve first");
+    if (c->train_rows + c->Ntot > c->train_cap) return fail(c, CHMC_E_ARG, "training set full");
+    int rc = surr_geometry(c);
+    if (rc) return rc;
+    rc = surr_descriptors(c, c->Gtrain + (size_t)c->train_rows * kSurrFeat);
+    if (rc) return rc;
+    HIPCHK(c, launch_surr_targets(c->v, c->sv, c->utrain + c->train_rows, c->stream));
+    if ((rc = surr_check_flag(c))) return rc;
+    c->train_rows += c->Ntot;
+    if (rows) *rows = c->train_rows;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_surrogate_training_data(chmc_ctx* c, long long row0, int nrows, float* G, float* u)
+{
+    if (!c || row0 < 0 || nrows < 0 || row0 + nrows > c->train_rows) return CHMC_E_ARG;
+    if (G) HIPCHK(c, hipMemcpy(G, c->Gtrain + (size_t)row0 * kSurrFeat, sizeof(float) * (size_t)nrows * kSurrFeat, hipMemcpyDeviceToHost));
+    if (u) HIPCHK(c, hipMemcpy(u, c->utrain + row0, sizeof(float) * (size_t)nrows, hipMemcpyDeviceToHost));
+    return CHMC_OK;
+}
+
+static int surr_partial_room(chmc_ctx* c, int nrows)
+{
+    const long long need = (long long)((nrows + 31) / 32) * kSurrNGrad;
+    if (need > c->spartial_cap) {
+        if (c->spartial) hipFree(c->spartial);
+        c->spartial = nullptr;
+        HIPCHK(c, hipMalloc(&c->spartial, sizeof(float) * (size_t)need));
+        c->spartial_cap = need;
+    }
+    return CHMC_OK;
+}
+
+// summed parameter gradient (and, in the last slot, the sum of squared residuals) of one batch of
+// training rows, without an update: the gradient check against the oracle
+extern "C" int chmc_surrogate_loss_grad(chmc_ctx* c, long long row0, int nrows, float* grad)
+{
+    if (!c || !grad || nrows < 1 || row0 < 0 || row0 + nrows > c->train_rows) return CHMC_E_ARG;
+    if (!c->surr_params) return fail(c, CHMC_E_STATE, "surrogate parameters not set");
+    int rc = surr_partial_room(c, nrows);
+    if (rc) return rc;
+    HIPCHK(c, launch_surr_grad_only(c->sv, c->Gtrain, c->utrain, row0, nrows, c->spartial, c->sloss, c->stream));
+    HIPCHK(c, hipMemcpy(grad, c->sloss, sizeof(float) * kSurrNGrad, hipMemcpyDeviceToHost));
+    return CHMC_OK;
+}
+
+// nsteps Adam steps on batches of batch_rows consecutive training rows starting at random offsets;
+// loss_history[t] (may be NULL) receives the batch's mean squared residual before step t's update
+extern "C" int chmc_surrogate_train(chmc_ctx* c, int nsteps, int batch_rows, float lr, uint64_t seed, float* loss_history)
+{
+    if (!c || nsteps < 1 || batch_rows < 1 || nsteps > (1 << 16)) return CHMC_E_ARG;
+    if (!c->surr_params) return fail(c, CHMC_E_STATE, "surrogate parameters not set");
+    if (c->train_rows < batch_rows) return fail(c, CHMC_E_STATE, "not enough training rows");
+    int rc = surr_partial_room(c, batch_rows);
+    if (rc) return rc;
+    Mt64 rng(seed);
+    const long long span = c->train_rows - batch_rows + 1;
+    for (int t = 0; t < nsteps; t++) {
+        const long long row0 = (long long)(rng.next() % (uint64_t)span);
+        c->adam_step++;
+        HIPCHK(c, launch_surr_train_step(c->sv, c->Gtrain, c->utrain, row0, batch_rows, c->spartial, lr, c->adam_step,
+                                         c->sloss + t, c->stream));
+    }
+    if (loss_history) {
+        std::vector<float> h(nsteps);
+        HIPCHK(c, hipMemcpy(h.data(), c->sloss, sizeof(float) * nsteps, hipMemcpyDeviceToHost));
+        for (int t = 0; t < nsteps; t++) loss_history[t] = h[t] / (float)batch_rows;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
 }
 

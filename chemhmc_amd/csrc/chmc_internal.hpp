@@ -46,7 +46,7 @@ struct ChainState {
 
 struct HmcRecord {
     double dh, u, k, w;
-    int acc, pad_;
+    int acc, surr;
 };
 
 struct Chunk { int chain, begin, end, pad_; };
@@ -98,7 +98,7 @@ hipError_t launch_lj(const DevView& v, int mode, bool fp64, int tiling, double k
 hipError_t launch_kinetic(const DevView& v, hipStream_t s);
 hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, hipStream_t s);
 hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustride, int therm, int ufirst,
-                             HmcRecord* rec, int rec_base, hipStream_t s);
+                             HmcRecord* rec, int rec_base, int surr, hipStream_t s);
 hipError_t launch_save_restore(const DevView& v, int mode, hipStream_t s);
 hipError_t launch_rdf(const DevView& v, int chain, int natoms, int nbins, double size_bin,
                       unsigned long long* counts, hipStream_t s);

@@ -116,7 +116,8 @@ __global__ __launch_bounds__(256) void k_reduce(DevView v, int what)
 // One workgroup per chain: V_f, W_f, K_f from the energy sweep's partials, K_i from the heat bath's,
 // then the Metropolis decision.
 __global__ __launch_bounds__(256) void k_metropolis(DevView v, const double* __restrict__ uniforms, int ustride,
-                                                    int therm, int ufirst, HmcRecord* __restrict__ rec, int rec_base)
+                                                    int therm, int ufirst, HmcRecord* __restrict__ rec, int rec_base,
+                                                    int surr)
 {
     const int ch = blockIdx.x;
     chain_sums(v, ch, SUM_CELLS | SUM_CHUNKS_KI);
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(256) void k_metropolis(DevView v, const double* __r
     v.state[ch] = st;
     if (rec) {
         HmcRecord h;
-        h.dh = dh; h.u = st.Vi; h.k = Ki; h.w = st.Wi; h.acc = acc; h.pad_ = 0;
+        h.dh = dh; h.u = st.Vi; h.k = Ki; h.w = st.Wi; h.acc = acc; h.surr = surr;
         rec[(size_t)(i - 1 - rec_base) * v.nchains + ch] = h;
     }
 }
@@ -198,9 +199,10 @@ hipError_t launch_reduce(const DevView& v, int what, hipStream_t s)
 }
 
 hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustride, int therm, int ufirst,
-                             HmcRecord* rec, int rec_base, hipStream_t s)
+                             HmcRecord* rec, int rec_base, int surr, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_metropolis, dim3(v.nchains), dim3(256), 0, s, v, uniforms, ustride, therm, ufirst, rec, rec_base);
+    hipLaunchKernelGGL(k_metropolis, dim3(v.nchains), dim3(256), 0, s, v, uniforms, ustride, therm, ufirst, rec, rec_base,
+                       surr);
     return hipGetLastError();
 }
 

@@ -131,12 +131,20 @@ typedef struct chmc_hmc_params {
     uint64_t seed;
     int use_graph;
     int randomize;     /* randomize_trajectory (HMC.cpp:82-85): binomial steps, no graph */
+    /* surrogate = 1 drives the MD with the learned surrogate's forces (chmc_surrogate_* below); the
+     * Metropolis test still uses the exact FP64 LJ energies, so the chain stays exact.  After every
+     * window of fallback_window trajectories whose acceptance rate is below fallback_acc the run
+     * switches to LJ forces for its remaining trajectories.  No reference counterpart. */
+    int surrogate;
+    int fallback_window;
+    double fallback_acc;
 } chmc_hmc_params;
 
 /* per trajectory and chain, record index = traj * nchains + chain */
 typedef struct chmc_hmc_record {
     double dh, u, k, w;   /* dH, U after the decision, K at the start, virial W after the decision */
-    int accepted, pad_;
+    int accepted;
+    int surrogate;        /* 1 if the trajectory's MD used the surrogate's forces */
 } chmc_hmc_record;
 
 /* records may be NULL; nacc[nchains] (may be NULL) receives the accepted post-thermalisation counts */
@@ -150,6 +158,35 @@ int chmc_rdf(chmc_ctx* ctx, int chain, int nbins, double size_bin, unsigned long
 /* code-carsten potential_energy (potential_energy.hh:6-15) on the GPU: sum_{i<j}
  * LennardJones(|r_i - r_j|, n, a, b) (lennardjones.hh:5-9) of an AoS double[3] set, open boundaries */
 int chmc_potential_energy_open(const double* xyz, int n_particles, int n, double a, double b, double* result);
+
+/* ---- learned surrogate (no reference counterpart) ----------------------------------------------
+ * A per-atom MLP energy E = sum_i e(G_i) over 32 radial Gaussian descriptors G_i (centres 0.8..2.5,
+ * width 0.1, cosine cutoff at the chain's r_c), 32 -> 64 -> 64 -> 1 with SiLU, bfloat16 MFMA layers
+ * and FP32 accumulation.  Its forces are -grad E, exactly conservative, so HMC driven by them with the
+ * exact LJ Hamiltonian in the accept step samples the LJ distribution.  One parameter vector of
+ * CHMC_SURR_NPARAMS floats: W1[64][32], b1[64], W2[64][64], b2[64], w3[64], b3, shift[32], scale[32]
+ * (input z = (G - shift) * scale); the last 64 are set by the caller, not trained. */
+#define CHMC_SURR_NPARAMS 6401
+#define CHMC_SURR_NGRAD 6338     /* the trained parameters + one slot for the sum of squared residuals */
+#define CHMC_SURR_NFEAT 32
+int chmc_surrogate_set_params(chmc_ctx* ctx, const float* params, int n);
+int chmc_surrogate_get_params(chmc_ctx* ctx, float* params, int n);
+/* E of every chain (energy[nchains], may be NULL) at the current positions; store_force keeps +grad E
+ * readable through chmc_get_force */
+int chmc_surrogate_eval(chmc_ctx* ctx, double* energy, int store_force);
+/* descriptors G[natoms][32] and dE/dG g[natoms][32] of the last chmc_surrogate_eval, atom order */
+int chmc_surrogate_features(chmc_ctx* ctx, int chain, float* G, float* g);
+/* training set on the device: room for nconfigs snapshots of every chain; collect appends the
+ * descriptors and per-atom LJ energies u_i = 2 eps sum_j sr6 (sr6 - 1) of the current positions */
+int chmc_surrogate_reserve(chmc_ctx* ctx, int nconfigs);
+int chmc_surrogate_collect(chmc_ctx* ctx, long long* rows);
+int chmc_surrogate_training_data(chmc_ctx* ctx, long long row0, int nrows, float* G, float* u);
+/* sum over rows of the gradient of (e_i - u_i)^2 / nrows w.r.t. the trained parameters, last slot
+ * sum of (e_i - u_i)^2: grad[CHMC_SURR_NGRAD] */
+int chmc_surrogate_loss_grad(chmc_ctx* ctx, long long row0, int nrows, float* grad);
+/* nsteps Adam steps (lr, beta1 0.9, beta2 0.999, eps 1e-8) on batches of batch_rows consecutive rows
+ * at offsets drawn from mt19937_64(seed); loss_history[nsteps] (may be NULL) gets each batch's MSE */
+int chmc_surrogate_train(chmc_ctx* ctx, int nsteps, int batch_rows, float lr, uint64_t seed, float* loss_history);
 
 /* ---- tuning ----------------------------------------------------------------------------------- */
 /* tiling of the FP32 LJ sweep, encoded i-slots per wavefront * 10 + wavefronts per workgroup, one of

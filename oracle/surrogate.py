@@ -1,0 +1,161 @@
+"""FP64 numpy oracle of the learned surrogate (TEST INFRASTRUCTURE ONLY; no reference counterpart).
+
+Model (per atom i, neighbours j within r_c under the minimum image, L >= 2 r_c):
+  descriptor  G_k(i) = sum_j exp(-(r_ij - mu_k)^2 / (2 w^2)) fc(r_ij),  fc(r) = (cos(pi r / r_c) + 1) / 2,
+              mu_k uniform on [0.8, 2.5] (32 centres), w = 0.1
+  input       z = (G - shift) * scale
+  MLP         h1 = silu(W1 z + b1) (64), h2 = silu(W2 h1 + b2) (64), e_i = w3 . h2 + b3
+  energy      E = sum_i e_i;  gradient dE/dx_i = sum_j [sum_k (g_k(i) + g_k(j)) phi'_k(r_ij)] (x_i - x_j) / r_ij
+              with g(a) = dE/dG(a) and phi_k(r) = exp(...) fc(r)
+  training    per-atom energy targets u_i = 2 eps sum_j sr6 (sr6 - 1) (the LJ energy split evenly over
+              pairs, sum_i u_i = V), loss = mean_i (e_i - u_i)^2, gradients by reverse mode
+bf16=True rounds weights and layer inputs to bfloat16 (round to nearest even) where the GPU kernels
+store them, so that GPU and oracle differ only by FP32 summation order.
+Pinned in tests/test_surrogate_oracle.py against torch autograd (FP64, CPU) and finite differences.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NFEAT, NH = 32, 64
+MU = np.linspace(0.8, 2.5, NFEAT)
+WIDTH = 0.1
+
+
+def bf16(a):
+    """Round to bfloat16 (nearest even), returned as float64."""
+    a32 = np.asarray(a, dtype=np.float32)
+    u = a32.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    p = {
+        "W1": rng.normal(0, 1 / np.sqrt(NFEAT), (NH, NFEAT)), "b1": np.zeros(NH),
+        "W2": rng.normal(0, 1 / np.sqrt(NH), (NH, NH)), "b2": np.zeros(NH),
+        "w3": rng.normal(0, 1 / np.sqrt(NH), NH), "b3": np.zeros(1),
+        "shift": np.zeros(NFEAT), "scale": np.ones(NFEAT),
+    }
+    return p
+
+
+def pairs(x, L, rc):
+    """All ordered pairs i != j with r_ij < rc under the minimum image (valid for L >= 2 rc)."""
+    L = np.broadcast_to(np.asarray(L, float), (3,))
+    d = x[:, None, :] - x[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = (d ** 2).sum(-1)
+    np.fill_diagonal(r2, np.inf)
+    i, j = np.nonzero(r2 < rc * rc)
+    return i, j, d[i, j], np.sqrt(r2[i, j])
+
+
+def basis(r, rc):
+    """phi_k(r) and phi'_k(r), shape (npairs, 32)."""
+    e = np.exp(-((r[:, None] - MU[None, :]) ** 2) / (2 * WIDTH ** 2))
+    fc = 0.5 * (np.cos(np.pi * r / rc) + 1.0)
+    dfc = -0.5 * np.pi / rc * np.sin(np.pi * r / rc)
+    phi = e * fc[:, None]
+    dphi = e * (-(r[:, None] - MU[None, :]) / WIDTH ** 2) * fc[:, None] + e * dfc[:, None]
+    return phi, dphi
+
+
+def descriptors(x, L, rc):
+    i, j, d, r = pairs(x, L, rc)
+    phi, _ = basis(r, rc)
+    G = np.zeros((x.shape[0], NFEAT))
+    np.add.at(G, i, phi)
+    return G
+
+
+def silu(a):
+    return a / (1 + np.exp(-a))
+
+
+def dsilu(a):
+    s = 1 / (1 + np.exp(-a))
+    return s * (1 + a * (1 - s))
+
+
+def mlp(G, p, bf=False):
+    """Per-atom energies and the cache of the forward pass."""
+    q = (lambda a: bf16(a)) if bf else (lambda a: a)
+    z = q((G - p["shift"]) * p["scale"])
+    a1 = z @ q(p["W1"]).T + p["b1"]
+    h1 = q(silu(a1))
+    a2 = h1 @ q(p["W2"]).T + p["b2"]
+    h2 = q(silu(a2))
+    e = h2 @ q(p["w3"]) + p["b3"][0]
+    return e, (z, a1, h1, a2, h2)
+
+
+def input_grad(G, p, bf=False, cache=None):
+    """g = d e_i / d G(i), shape (N, 32).  With bf the back-propagated vectors are rounded to bfloat16
+    before each matrix product, as the MFMA kernel feeds them."""
+    q = (lambda a: bf16(a)) if bf else (lambda a: a)
+    if cache is None:
+        _, cache = mlp(G, p, bf)
+    z, a1, h1, a2, h2 = cache
+    d2 = q(p["w3"])[None, :] * dsilu(a2)
+    d1 = (q(d2) @ q(p["W2"])) * dsilu(a1)
+    return (q(d1) @ q(p["W1"])) * p["scale"][None, :]
+
+
+def energy_grad(x, L, rc, p, bf=False):
+    """Surrogate energy E and its gradient +dE/dx (N, 3) (the engine's force convention)."""
+    i, j, d, r = pairs(x, L, rc)
+    phi, dphi = basis(r, rc)
+    G = np.zeros((x.shape[0], NFEAT))
+    np.add.at(G, i, phi)
+    e, cache = mlp(G, p, bf)
+    g = input_grad(G, p, bf, cache)
+    # every ordered pair (i, j) appears once with i first: its term (x_i - x_j) / r_ij acts on atom i
+    coef = ((g[i] + g[j]) * dphi).sum(1) / r
+    grad = np.zeros_like(x)
+    np.add.at(grad, i, coef[:, None] * d)
+    return e.sum(), grad, e
+
+
+def lj_atom_energies(x, L, rc, eps=1.0, sigma=1.0):
+    i, j, d, r = pairs(x, L, rc)
+    sr6 = (sigma / r) ** 6
+    u = np.zeros(x.shape[0])
+    np.add.at(u, i, 2 * eps * sr6 * (sr6 - 1))
+    return u
+
+
+def loss_grad(p, G, u, bf=False):
+    """Per-atom energy MSE and its gradient w.r.t. W1, b1, W2, b2, w3, b3 (reverse mode).  With bf the
+    forward pass is the bfloat16 one of mlp() and d2 is rounded before the product with W2."""
+    q = (lambda a: bf16(a)) if bf else (lambda a: a)
+    e, (z, a1, h1, a2, h2) = mlp(G, p, bf)
+    n = G.shape[0]
+    res = e - u
+    loss = np.mean(res ** 2)
+    de = 2 * res / n
+    gw3 = h2.T @ de
+    gb3 = np.array([de.sum()])
+    d2 = de[:, None] * q(p["w3"])[None, :] * dsilu(a2)
+    gW2 = d2.T @ h1
+    gb2 = d2.sum(0)
+    d1 = (q(d2) @ q(p["W2"])) * dsilu(a1)
+    gW1 = d1.T @ z
+    gb1 = d1.sum(0)
+    return loss, {"W1": gW1, "b1": gb1, "W2": gW2, "b2": gb2, "w3": gw3, "b3": gb3}
+
+
+def adam_step(p, grads, state, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015) of the trainable tensors, in place."""
+    state["t"] = state.get("t", 0) + 1
+    t = state["t"]
+    for k, gk in grads.items():
+        m = state.setdefault("m_" + k, np.zeros_like(gk))
+        v = state.setdefault("v_" + k, np.zeros_like(gk))
+        m[...] = b1 * m + (1 - b1) * gk
+        v[...] = b2 * v + (1 - b2) * gk * gk
+        mh = m / (1 - b1 ** t)
+        vh = v / (1 - b2 ** t)
+        p[k] = p[k] - lr * mh / (np.sqrt(vh) + eps)
+    return p

@@ -72,6 +72,34 @@ def test_lattice_and_binning_bitexact(chmc, orc, N, rho, Lb):
     assert cnt2.sum() == N and np.array_equal(np.sort(perm2), np.arange(N))
 
 
+def test_binning_after_long_jumps(chmc, orc):
+    # atoms displaced by several cells since the last binning: the locality path detects it and falls back
+    # to the whole-chain scan, which must give the same stable order
+    N, rho = 4096, 0.8442
+    e, L = make(chmc, N, rho)
+    x0 = e.positions()
+    e.bin()
+    p0, _, _ = e.bins()
+    rng = np.random.default_rng(5)
+    p = rng.normal(size=(N, 3)) * 10.0
+    e.set_momenta(p)
+    e.update_positions(1.0)
+    x = e.positions()
+    e.bin()
+    perm, cnt, off = e.bins()
+    po, co, oo = orc.binning(x, L, RC, perm=p0)
+    assert np.array_equal(perm, po) and np.array_equal(cnt, co) and np.array_equal(off, oo)
+    # and the next, ordinary binning is back on the local path and still exact
+    e.set_momenta(p * 0.01)
+    e.update_positions(0.5)
+    x2 = e.positions()
+    e.bin()
+    perm2, cnt2, off2 = e.bins()
+    po2, co2, oo2 = orc.binning(x2, L, RC, perm=po)
+    assert np.array_equal(perm2, po2) and np.array_equal(cnt2, co2) and np.array_equal(off2, oo2)
+    assert not np.array_equal(x0, x)
+
+
 def test_binning_atom_id_order(chmc, orc):
     N, rho = 4096, 0.8442
     e, L = make(chmc, N, rho, atom_id_order=True)
@@ -174,6 +202,11 @@ def test_integrators_fp64_vs_oracle(chmc, orc, scheme, steps):
     s.integrate(scheme, 0.005, steps, 0.1932)
     assert np.max(np.abs(min_image(e.positions() - s.x, L))) < 1e-10
     assert np.max(np.abs(e.momenta() - s.p)) < 1e-9
+    # the same sequence of stable re-binnings ran on both sides (one per force evaluation)
+    e.bin()
+    s.create_binning()
+    perm, cnt, off = e.bins()
+    assert np.array_equal(perm, s.perm) and np.array_equal(cnt, s.bincount) and np.array_equal(off, s.binoffsets)
 
 
 def test_integrator_fp32_config2(chmc, orc):

@@ -1,0 +1,111 @@
+"""Pins the numpy surrogate oracle (oracle/surrogate.py): descriptor, MLP, analytic position gradient and
+training gradients against torch autograd (FP64, CPU) and central finite differences; bf16 rounding
+against torch's bfloat16 conversion."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import surrogate as S  # noqa: E402
+
+RC = 2.5
+
+
+def _config(N=64, rho=0.6, seed=1):
+    L = (N / rho) ** (1 / 3)
+    rng = np.random.default_rng(seed)
+    n = int(np.ceil(N ** (1 / 3)))
+    g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3)[:N]
+    x = (g + 0.5) * (L / n) + rng.uniform(-0.1, 0.1, (N, 3))
+    return x % L, L
+
+
+def _torch_energy(x, L, p):
+    xt = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+    d = xt[:, None, :] - xt[None, :, :]
+    d = d - L * torch.round(d / L)
+    r2 = (d ** 2).sum(-1) + torch.eye(len(x), dtype=torch.float64) * 1e6
+    r = torch.sqrt(r2)
+    mask = (r < RC).double()
+    mu = torch.tensor(S.MU)
+    e = torch.exp(-((r[..., None] - mu) ** 2) / (2 * S.WIDTH ** 2))
+    fc = 0.5 * (torch.cos(np.pi * r / RC) + 1) * mask
+    G = (e * fc[..., None]).sum(1)
+    T = {k: torch.tensor(v, dtype=torch.float64) for k, v in p.items()}
+    z = (G - T["shift"]) * T["scale"]
+    h1 = torch.nn.functional.silu(z @ T["W1"].T + T["b1"])
+    h2 = torch.nn.functional.silu(h1 @ T["W2"].T + T["b2"])
+    e_at = h2 @ T["w3"] + T["b3"][0]
+    E = e_at.sum()
+    E.backward()
+    return E.item(), xt.grad.numpy(), G.detach().numpy()
+
+
+def test_descriptor_energy_and_position_gradient_vs_torch():
+    x, L = _config()
+    p = S.init_params(3)
+    p["scale"] = np.full(32, 0.5)
+    p["shift"] = np.full(32, 0.3)
+    E, grad, _ = S.energy_grad(x, L, RC, p)
+    Et, gt, Gt = _torch_energy(x, L, p)
+    assert np.allclose(S.descriptors(x, L, RC), Gt, rtol=1e-12, atol=1e-12)
+    assert E == pytest.approx(Et, rel=1e-12)
+    assert np.allclose(grad, gt, rtol=1e-9, atol=1e-10)
+
+
+def test_position_gradient_finite_difference():
+    x, L = _config(N=27, rho=0.5, seed=2)
+    p = S.init_params(4)
+    _, grad, _ = S.energy_grad(x, L, RC, p)
+    h = 1e-6
+    for i, d in [(0, 0), (5, 1), (26, 2)]:
+        xp, xm = x.copy(), x.copy()
+        xp[i, d] += h
+        xm[i, d] -= h
+        num = (S.energy_grad(xp, L, RC, p)[0] - S.energy_grad(xm, L, RC, p)[0]) / (2 * h)
+        assert num == pytest.approx(grad[i, d], rel=1e-6, abs=1e-8)
+
+
+def test_training_gradients_vs_torch():
+    x, L = _config(N=64, seed=5)
+    G = S.descriptors(x, L, RC)
+    u = S.lj_atom_energies(x, L, RC)
+    p = S.init_params(7)
+    loss, grads = S.loss_grad(p, G, u)
+    T = {k: torch.tensor(v, dtype=torch.float64, requires_grad=k in grads) for k, v in p.items()}
+    z = (torch.tensor(G) - T["shift"]) * T["scale"]
+    h1 = torch.nn.functional.silu(z @ T["W1"].T + T["b1"])
+    h2 = torch.nn.functional.silu(h1 @ T["W2"].T + T["b2"])
+    e = h2 @ T["w3"] + T["b3"][0]
+    lt = torch.mean((e - torch.tensor(u)) ** 2)
+    lt.backward()
+    assert loss == pytest.approx(lt.item(), rel=1e-12)
+    for k, g in grads.items():
+        assert np.allclose(g, T[k].grad.numpy(), rtol=1e-9, atol=1e-12), k
+
+
+def test_lj_atom_energies_sum_to_potential(orc):
+    x, L = _config(N=125, rho=0.7, seed=9)
+    u = S.lj_atom_energies(x, L, RC)
+    assert u.sum() == pytest.approx(orc.potential_all(x, [L] * 3, RC), rel=1e-12)
+
+
+def test_bf16_rounding_matches_torch():
+    a = np.random.default_rng(0).normal(size=1000) * 10
+    assert np.array_equal(S.bf16(a), torch.tensor(a, dtype=torch.float32).bfloat16().double().numpy())
+
+
+def test_adam_reduces_loss():
+    x, L = _config(N=125, rho=0.7, seed=11)
+    G = S.descriptors(x, L, RC)
+    u = S.lj_atom_energies(x, L, RC)
+    p = S.init_params(1)
+    p["shift"] = G.mean(0)
+    p["scale"] = 1 / (G.std(0) + 1e-3)
+    state = {}
+    l0, _ = S.loss_grad(p, G, u)
+    for _ in range(300):
+        _, g = S.loss_grad(p, G, u)
+        S.adam_step(p, g, state, lr=1e-2)
+    l1, _ = S.loss_grad(p, G, u)
+    assert l1 < 0.1 * l0
