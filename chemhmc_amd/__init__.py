@@ -102,6 +102,8 @@ def _declare(L):
         "chmc_set_force_tiling": ([P, _I], _I),
         "chmc_get_force_tiling": ([P, C.POINTER(_I)], _I),
         "chmc_time_force": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
+        "chmc_set_trajectory": ([P, _I], _I),
+        "chmc_get_trajectory": ([P, C.POINTER(_I)], _I),
         "chmc_surrogate_set_params": ([P, C.c_void_p, _I], _I),
         "chmc_surrogate_get_params": ([P, C.c_void_p, _I], _I),
         "chmc_surrogate_eval": ([P, C.c_void_p, _I], _I),
@@ -125,7 +127,8 @@ EXPORTED = (
     "chmc_evaluate_potential", "chmc_virial", "chmc_pair_counts", "chmc_kinetic", "chmc_update_momenta", "chmc_update_positions",
     "chmc_kick", "chmc_integrate", "chmc_heatbath", "chmc_save_positions", "chmc_restore_positions",
     "chmc_hmc_run", "chmc_rdf", "chmc_potential_energy_open", "chmc_set_force_tiling",
-    "chmc_get_force_tiling", "chmc_time_force", "chmc_surrogate_set_params", "chmc_surrogate_get_params",
+    "chmc_get_force_tiling", "chmc_time_force", "chmc_set_trajectory", "chmc_get_trajectory",
+    "chmc_surrogate_set_params", "chmc_surrogate_get_params",
     "chmc_surrogate_eval", "chmc_surrogate_features", "chmc_surrogate_reserve", "chmc_surrogate_collect",
     "chmc_surrogate_training_data", "chmc_surrogate_loss_grad", "chmc_surrogate_train",
 )
@@ -306,6 +309,14 @@ class Engine:
 
     def restore_positions(self):
         self._ck(lib().chmc_restore_positions(self._h), "restore_positions")
+
+    def set_trajectory(self, istart):
+        self._ck(lib().chmc_set_trajectory(self._h, int(istart)), "set_trajectory")
+
+    def trajectory(self):
+        v = _I()
+        self._ck(lib().chmc_get_trajectory(self._h, C.byref(v)), "get_trajectory")
+        return v.value
 
     def hmc(self, ntraj, therm, scheme, dt, steps, lam=OMF2_LAMBDA_REFERENCE, seed=1, use_graph=True,
             randomize=False, surrogate=False, fallback_window=0, fallback_acc=0.0):

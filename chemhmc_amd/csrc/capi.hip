@@ -754,6 +754,23 @@ extern "C" int chmc_restore_positions(chmc_ctx* c)
     return CHMC_OK;
 }
 
+extern "C" int chmc_set_trajectory(chmc_ctx* c, int istart)
+{
+    CHECK_CTX(c);
+    if (istart < 0) return fail(c, CHMC_E_ARG, "set_trajectory: istart must be >= 0");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->v.traj, &istart, sizeof(int), hipMemcpyHostToDevice));
+    c->traj_done = istart;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_get_trajectory(const chmc_ctx* c, int* istart)
+{
+    if (!c || !istart) return CHMC_E_ARG;
+    *istart = c->traj_done;
+    return CHMC_OK;
+}
+
 // ---- HMC ----------------------------------------------------------------------------------------
 // mt19937_64 of HMC.cpp:38 (gen64), host side, used to draw the Metropolis uniforms up front
 struct Mt64 {

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_metropolis(DevView v, const double* __r
     v.state[ch] = st;
     if (rec) {
         HmcRecord h;
-        h.dh = dh; h.u = st.Vi; h.k = Ki; h.w = st.Wi; h.acc = acc; h.surr = surr;
+        h.dh = dh; h.u = st.Vi; h.k = acc ? Kf : Ki; h.w = st.Wi; h.acc = acc; h.surr = surr;
         rec[(size_t)(i - 1 - rec_base) * v.nchains + ch] = h;
     }
 }

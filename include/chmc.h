@@ -120,6 +120,11 @@ int chmc_integrate(chmc_ctx* ctx, int scheme, double dt, int steps, double lambd
 int chmc_heatbath(chmc_ctx* ctx, uint64_t seed, uint64_t traj);
 int chmc_save_positions(chmc_ctx* ctx);      /* h_x = x   (HMC.cpp:72,103,107) */
 int chmc_restore_positions(chmc_ctx* ctx);   /* x = h_x   (HMC.cpp:111) */
+/* params.istart (read_infile.hpp:17, HMC.cpp:66-67): the number of trajectories already run; the next
+ * chmc_hmc_run numbers its trajectories istart+1, ... and keys its heat bath on those numbers, so a
+ * restart that restores istart (and the seed) continues the same device random stream */
+int chmc_set_trajectory(chmc_ctx* ctx, int istart);
+int chmc_get_trajectory(const chmc_ctx* ctx, int* istart);
 
 /* HMC_class::run (HMC.cpp:61-136) for every chain of the context, all on the device: heat bath,
  * K_i, MD, V_f (FP64), K_f, dh = beta (K_f + V_f - K_i - V_i), thermalisation, Metropolis with
@@ -142,7 +147,8 @@ typedef struct chmc_hmc_params {
 
 /* per trajectory and chain, record index = traj * nchains + chain */
 typedef struct chmc_hmc_record {
-    double dh, u, k, w;   /* dH, U after the decision, K at the start, virial W after the decision */
+    double dh, u, k, w;   /* dH; U, K and virial W after the decision (K = K_f if accepted, else the
+                             heat bath's K_i: the Ki of HMC.cpp:104 that print_xyz writes) */
     int accepted;
     int surrogate;        /* 1 if the trajectory's MD used the surrogate's forces */
 } chmc_hmc_record;

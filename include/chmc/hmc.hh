@@ -37,6 +37,9 @@ struct particle_params {
     double L[3] = { 0, 0, 0 };
     double mass = 1, beta = 1, cutoff = 2.5, eps = 1, sigma = 1;
     bool fp64_forces = false;
+    // in-cell order by atom id instead of the reference's history-dependent stable order (Q8): two runs
+    // from the same positions then see the same FP32 summation order whatever ran before
+    bool order_atom_id = false;
     // RDF (identical_particles.cpp:78-86): bins of width size_bin up to Lmax
     double rdf_Lmax = 0, rdf_size_bin = 0;
     int rdf_bins = 0;
@@ -60,7 +63,8 @@ public:
         d.N = N;
         for (int i = 0; i < 3; i++) d.L[i] = L[i];
         d.cutoff = cutoff; d.sigma = sigma; d.eps = eps; d.mass = mass; d.beta = beta;
-        check(chmc_create(&ctx_, 1, &d, device, p.fp64_forces ? CHMC_FORCE_FP64 : CHMC_FORCE_FP32), nullptr, "chmc_create");
+        const int flags = (p.fp64_forces ? CHMC_FORCE_FP64 : CHMC_FORCE_FP32) | (p.order_atom_id ? CHMC_ORDER_ATOM_ID : 0);
+        check(chmc_create(&ctx_, 1, &d, device, flags), nullptr, "chmc_create");
         if (NbRDF == 0 && size_bRDF > 0) NbRDF = (int)(LmaxRDF / size_bRDF);
         RDF.assign(NbRDF > 0 ? NbRDF : 0, 0.0);
     }
@@ -287,6 +291,7 @@ public:
         hp.randomize = params.randomize_traj ? 1 : 0;
         std::vector<chmc_hmc_record> rec(params.Ntrajectories);
         int nacc = 0;
+        check(chmc_set_trajectory(P->ctx(), params.istart), P->ctx(), "set_trajectory");
         check(chmc_hmc_run(P->ctx(), &hp, rec.data(), &nacc), P->ctx(), "hmc_run");
         acceptance += nacc;
         return rec;

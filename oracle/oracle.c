@@ -749,7 +749,7 @@ int orc_hmc_run(orc_system* s, const orc_hmc_params* hp, double* rec_dh, int* re
         if (rec_dh) rec_dh[t] = dh;
         if (rec_acc) rec_acc[t] = accepted;
         if (rec_u) rec_u[t] = Vi;
-        if (rec_k) rec_k[t] = Ki;
+        if (rec_k) rec_k[t] = accepted ? Kf : Ki;      /* Ki after HMC.cpp:104 (what print_xyz writes) */
         if (rec_w) rec_w[t] = Wi;
     }
     return acc;

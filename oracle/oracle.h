@@ -103,7 +103,7 @@ typedef struct {
 } orc_hmc_params;
 
 /* records (length ntraj each, any may be NULL): dH, accepted, U after the decision,
- * K at the start of the trajectory, virial W of the current configuration */
+ * K after the decision (K_f if accepted, else K_i), virial W of the current configuration */
 int orc_hmc_run(orc_system* s, const orc_hmc_params* hp, double* rec_dh, int* rec_acc,
                 double* rec_u, double* rec_k, double* rec_w);
 

@@ -4,6 +4,9 @@ and g(r) (200 bins to L/2).  Errors are tau_int-corrected (Sokal windowing); the
 scalar observable and a mean z^2 over the populated g(r) bins below 2 (z-scores are printed; the
 measured values are recorded in the design notes).  Both chains start from the same seeded lattice
 and diverge (FP32 vs FP64 forces), so they are independent samples of the same distribution.
+Burn-in accepts every trajectory (the reference's thermalization_steps): from the lattice the first
+trajectory carries dH ~ 7 (pairs of the sqrt(5) a = 2.41 shell leaving the unshifted cutoff, Q3), so a
+Metropolis chain started there can stay rejected for hundreds of trajectories.
 """
 import numpy as np
 import pytest
@@ -22,7 +25,7 @@ BURN, SEG, NSEG = 300, 50, 50
 def _run_gpu(chmc):
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
     e.init_lattice(1, 0.05)
-    e.hmc(BURN, 0, chmc.LEAPFROG, DT, STEPS, seed=100, use_graph=True)
+    e.hmc(BURN, BURN + 1, chmc.LEAPFROG, DT, STEPS, seed=100, use_graph=True)    # accept-all burn-in
     recs, g = [], []
     for k in range(NSEG):
         r, _ = e.hmc(SEG, 0, chmc.LEAPFROG, DT, STEPS, seed=1000 + 7919 * k, use_graph=True)
@@ -35,7 +38,7 @@ def _run_gpu(chmc):
 def _run_oracle(orc):
     s = orc.Sim(N, [L] * 3, cutoff=RC, beta=1 / T)
     s.x[:] = orc.lattice_init(N, [L] * 3, 1, 0, 0.05)
-    s.hmc(BURN, 0, orc.LEAPFROG, DT, STEPS, seed=200)
+    s.hmc(BURN, BURN + 1, orc.LEAPFROG, DT, STEPS, seed=200)
     u, w, a, g = [], [], [], []
     for k in range(NSEG):
         r = s.hmc(SEG, 0, orc.LEAPFROG, DT, STEPS, seed=2000 + 7919 * k)

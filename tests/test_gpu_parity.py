@@ -276,6 +276,29 @@ def test_hmc_graph_equals_direct_and_batch_independence(chmc):
     assert np.array_equal(rb2[:, 0], rb[:, 0]) and np.array_equal(rb2[:, 2], rb[:, 2])
 
 
+@pytest.mark.parametrize("fp64", [True, False])
+def test_hmc_records_vs_oracle(chmc, orc, fp64):
+    """Whole trajectories (heat bath, LEAP MD, FP64 V and K, Metropolis, restore on reject) against the
+    oracle chain from the same start, seed and Metropolis stream.  FP64 forces: dH within 1e-8 and the
+    same decisions for every trajectory.  FP32 forces: dH within 1e-3 over the first 10 trajectories
+    (the chains then drift apart through the FP32 force rounding)."""
+    N, rho, T = 512, 0.8, 1.5
+    e, L = make(chmc, N, rho, T=T, fp64_forces=fp64)
+    s = orc.Sim(N, [L] * 3, cutoff=RC, beta=1 / T)
+    s.x[:] = e.positions()
+    ntraj = 30 if fp64 else 10
+    r, nacc = e.hmc(ntraj, 3, 0, 0.005, 20, seed=4, use_graph=True)
+    ro = s.hmc(ntraj, 3, orc.LEAPFROG, 0.005, 20, seed=4)
+    r = r[:, 0]
+    print("dh gpu   ", np.array2string(r["dh"], precision=4))
+    print("dh oracle", np.array2string(ro["dh"], precision=4))
+    tol = 1e-8 if fp64 else 1e-3
+    assert np.max(np.abs(r["dh"] - ro["dh"])) < tol
+    assert np.array_equal(r["accepted"], ro["acc"])
+    assert nacc[0] == ro["nacc"]
+    assert np.max(np.abs(r["u"] - ro["u"])) < tol * N
+
+
 def test_error_paths(chmc):
     with pytest.raises(chmc.ChmcError):
         chmc.Engine(chmc.Chain(N=10, L=2.0, cutoff=2.5))            # box <= cutoff
