@@ -8,9 +8,10 @@ the state resident in HBM and ends after the device has finished the K-th trajec
 
   value      atom-steps/s = N * 10 MD steps * K trajectories / seconds
   ess_per_s  ESS/s of U/N over the timed trajectories (Sokal-windowed tau_int)
-  roofline   the dominant kernel, the FP32 LJ sweep: algorithmic flops per launch (8 N_cand + 13 N_int
-             with N_cand, N_int counted on the device for the current configuration) / its average
-             duration measured with HIP events on the context's stream
+  roofline   the dominant kernel, the FP32 neighbour-list sweep (force + kick + drift): algorithmic flops
+             per launch (8 per list entry + 13 per interacting pair, both counted on the device for the
+             current configuration) / its average duration measured with HIP events on the context's
+             stream
   cpu_baseline  the FP64 C oracle (a port of the reference algorithm, cpu-opt binning) timed on this
              host's cores for a bounded sample of the same workload
 """
@@ -57,6 +58,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000, help="timed HMC trajectories")
     ap.add_argument("--warmup", type=int, default=100, help="untimed trajectories (also thermalisation)")
+    ap.add_argument("--tune", type=int, default=40, help="trajectories per skin in the skin autotuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     args = ap.parse_args()
@@ -71,10 +73,19 @@ def main():
     L = (N / RHO) ** (1 / 3)
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
     e.init_lattice(1, 0.05)
-    e.set_force_tiling(0)                       # autotune the sweep tiling on this configuration
-    islots = e.force_tiling()
-    if args.warmup > 0:
+    if args.warmup > 0:     # accept-all thermalisation away from the lattice (thermalization_steps)
         e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=True)
+    # skin autotuner: wall time of a short run per skin on the equilibrated state
+    skins = {}
+    for k, sk in enumerate((0.2, 0.3, 0.4, 0.5)):
+        e.set_skin(sk)
+        e.hmc(4, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=20 + k, use_graph=True)
+        t0 = time.perf_counter()
+        e.hmc(args.tune, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=30 + k, use_graph=True)
+        skins[sk] = (time.perf_counter() - t0) / args.tune * 1e3
+    skin = min(skins, key=skins.get)
+    e.set_skin(skin)
+    _, _, rebuilds0 = e.list_stats()
 
     t0 = time.perf_counter()
     rec, nacc = e.hmc(args.steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=True)
@@ -86,11 +97,14 @@ def main():
     tau, _ = stats.tau_int(u)
     ess = args.steps / (2 * tau)
 
-    # dominant kernel: FP32 LJ sweep; algorithmic flops of this configuration
-    ncand, nint = e.pair_counts(0)
-    flops = 8.0 * ncand + 13.0 * nint
-    ms_bin, ms_force = e.time_force(reps=50)
-    achieved = flops / (ms_force * 1e-3) / 1e12
+    # dominant kernel: the neighbour-list FP32 sweep (force + kick + drift); algorithmic flops of this
+    # configuration: 8 per list entry (candidate) + 13 per interacting pair
+    entries, nint, rebuilds1 = e.list_stats()
+    rebuilds_per_traj = (rebuilds1 - rebuilds0 - 1) / args.steps       # list_stats itself rebuilds once
+    flops = 8.0 * entries + 13.0 * nint
+    ms_sweep, ms_rebuild = e.time_list(reps=50)
+    achieved = flops / (ms_sweep * 1e-3) / 1e12
+    ncand_cell, _ = e.pair_counts(0)
 
     out = {
         "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N",
@@ -99,13 +113,15 @@ def main():
         "dtype": "f32 pair forces, f64 x/p/dH", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": "plain HMC N=32768 rho=0.8442 T=0.722 rc=2.5 OMF2(lambda=0.1932) 10x0.01",
                    "N": N, "trajectories": args.steps, "md_steps_per_traj": NSTEPS, "force_evals_per_traj": 2 * NSTEPS + 1,
-                   "force_tiling_islots": islots},
+                   "md_path": "neighbour lists", "skin": skin, "ms_per_traj_by_skin": skins},
         "acceptance": float(nacc[0]) / args.steps,
         "u_per_n": float(u.mean()), "tau_int_u": tau, "ess_per_s": ess / el,
-        "roofline": {"bound": "valu", "kernel": "k_lj<float> force+kick sweep", "achieved": achieved,
+        "list_rebuilds_per_traj": rebuilds_per_traj,
+        "roofline": {"bound": "valu", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": None, "flops_per_launch": flops, "ms_per_launch": ms_force,
-                     "n_cand_per_atom": ncand / N, "n_int_per_atom": nint / N, "ms_binning": ms_bin},
+                     "traffic": None, "flops_per_launch": flops, "ms_per_launch": ms_sweep,
+                     "list_entries_per_atom": entries / N, "n_int_per_atom": nint / N,
+                     "cell_candidates_per_atom": ncand_cell / N, "ms_rebuild": ms_rebuild},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libchmc.so")
 
 LEAPFROG, OMF4, EULER, RUTH, OMF2 = 0, 2, 4, 5, 6
-FORCE_FP32, FORCE_FP64, ORDER_ATOM_ID = 0, 1, 2
+FORCE_FP32, FORCE_FP64, ORDER_ATOM_ID, CELL_MD = 0, 1, 2, 4
 OMF2_LAMBDA_REFERENCE = 0.1938       # code/modules/integrator.cpp:72
 
 _D, _I, _U64 = C.c_double, C.c_int, C.c_uint64
@@ -102,6 +102,10 @@ def _declare(L):
         "chmc_set_force_tiling": ([P, _I], _I),
         "chmc_get_force_tiling": ([P, C.POINTER(_I)], _I),
         "chmc_time_force": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
+        "chmc_set_skin": ([P, _D], _I),
+        "chmc_get_skin": ([P, C.POINTER(_D)], _I),
+        "chmc_list_stats": ([P, _I, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], _I),
+        "chmc_time_list": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
         "chmc_set_trajectory": ([P, _I], _I),
         "chmc_get_trajectory": ([P, C.POINTER(_I)], _I),
         "chmc_surrogate_set_params": ([P, C.c_void_p, _I], _I),
@@ -127,7 +131,8 @@ EXPORTED = (
     "chmc_evaluate_potential", "chmc_virial", "chmc_pair_counts", "chmc_kinetic", "chmc_update_momenta", "chmc_update_positions",
     "chmc_kick", "chmc_integrate", "chmc_heatbath", "chmc_save_positions", "chmc_restore_positions",
     "chmc_hmc_run", "chmc_rdf", "chmc_potential_energy_open", "chmc_set_force_tiling",
-    "chmc_get_force_tiling", "chmc_time_force", "chmc_set_trajectory", "chmc_get_trajectory",
+    "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_list_stats",
+    "chmc_time_list", "chmc_set_trajectory", "chmc_get_trajectory",
     "chmc_surrogate_set_params", "chmc_surrogate_get_params",
     "chmc_surrogate_eval", "chmc_surrogate_features", "chmc_surrogate_reserve", "chmc_surrogate_collect",
     "chmc_surrogate_training_data", "chmc_surrogate_loss_grad", "chmc_surrogate_train",
@@ -182,12 +187,13 @@ class Engine:
     """A context of one or more chains on one GPU (all calls act on every chain at once)."""
 
     def __init__(self, chains: Sequence[Chain] | Chain, device: int = -1, fp64_forces: bool = False,
-                 atom_id_order: bool = False):
+                 atom_id_order: bool = False, cell_md: bool = False):
         if isinstance(chains, Chain):
             chains = [chains]
         self.chains = list(chains)
         arr = (ChainDesc * len(self.chains))(*[c.to_c() for c in self.chains])
-        flags = (FORCE_FP64 if fp64_forces else 0) | (ORDER_ATOM_ID if atom_id_order else 0)
+        flags = ((FORCE_FP64 if fp64_forces else 0) | (ORDER_ATOM_ID if atom_id_order else 0)
+                 | (CELL_MD if cell_md else 0))
         self._h = C.c_void_p()
         rc = lib().chmc_create(C.byref(self._h), len(self.chains), arr, int(device), flags)
         if rc != 0:
@@ -390,6 +396,24 @@ class Engine:
         v = _I()
         self._ck(lib().chmc_get_force_tiling(self._h, C.byref(v)), "get_force_tiling")
         return v.value
+
+    def set_skin(self, skin):
+        self._ck(lib().chmc_set_skin(self._h, float(skin)), "set_skin")
+
+    def skin(self):
+        v = _D()
+        self._ck(lib().chmc_get_skin(self._h, C.byref(v)), "get_skin")
+        return v.value
+
+    def list_stats(self, chain=0):
+        a, b, r = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        self._ck(lib().chmc_list_stats(self._h, int(chain), C.byref(a), C.byref(b), C.byref(r)), "list_stats")
+        return a.value, b.value, r.value
+
+    def time_list(self, reps=20):
+        a, b = _D(), _D()
+        self._ck(lib().chmc_time_list(self._h, int(reps), C.byref(a), C.byref(b)), "time_list")
+        return a.value, b.value
 
     def time_force(self, reps=20):
         b, f = _D(), _D()

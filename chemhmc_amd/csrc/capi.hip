@@ -54,6 +54,12 @@ struct chmc_ctx {
     float* sloss = nullptr;
     int adam_step = 0;
     bool md_surr = false;        // MD kicks use the surrogate's forces (set for the trajectories of an HMC run)
+    // neighbour-list MD (allocated on first use)
+    double skin = 0.3;
+    bool list_ready = false;
+    int lpar = 0;                // list parity: which FP32 coordinate buffer / rebuild flag is current
+    std::vector<void*> lallocs;
+    std::vector<Chunk> htiles;
 };
 
 static thread_local std::string g_create_error;
@@ -103,6 +109,7 @@ static void destroy_ctx(chmc_ctx* c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void* p : c->allocs) hipFree(p);
+    for (void* p : c->lallocs) hipFree(p);
     if (c->d_uniforms) hipFree(c->d_uniforms);
     if (c->d_rec) hipFree(c->d_rec);
     if (c->d_rdf) hipFree(c->d_rdf);
@@ -636,6 +643,178 @@ extern "C" int chmc_kick(chmc_ctx* c, double dt)
     return CHMC_OK;
 }
 
+// ---- neighbour-list MD -------------------------------------------------------------------------
+static bool list_md(const chmc_ctx* c) { return !fp64_md(c) && (c->flags & CHMC_CELL_MD) == 0; }
+
+template <typename T>
+static hipError_t lalloc(chmc_ctx* c, T** p, size_t n)
+{
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, sizeof(T) * (n > 0 ? n : 1));
+    if (e == hipSuccess) c->lallocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return e;
+}
+
+// per-chain list grids (cells of side >= rc + skin; the skin shrinks where a box side is shorter than
+// rc + skin), 32-slot tiles, and the list buffers
+static int list_setup(chmc_ctx* c, double skin)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (void* p : c->lallocs) hipFree(p);
+    c->lallocs.clear();
+    c->list_ready = false;
+    c->htiles.clear();
+    long long lcells = 0, dense = 0;
+    double nbmax = 0.0;
+    std::vector<int> hlc;
+    for (int k = 0; k < c->nchains; k++) {
+        ChainDev& cd = c->hchains[k];
+        double rl = cd.rc + skin;
+        for (int d = 0; d < 3; d++) rl = std::min(rl, cd.L[d]);
+        const double sk = rl - cd.rc;
+        double vol = 1.0;
+        for (int d = 0; d < 3; d++) {
+            cd.lnbin[d] = std::max(1, (int)std::floor(cd.L[d] / rl));
+            cd.lsize[d] = cd.L[d] / cd.lnbin[d];
+            cd.lsize_f[d] = (float)cd.lsize[d];
+            vol *= cd.lsize[d] / cd.sigma;
+        }
+        cd.half_skin2 = 0.25 * sk * sk;
+        cd.rl2_f = (float)(rl * rl * (1.0 + 1e-5));   // a margin over the FP32 rounding of r^2 at the rebuild
+        // capacities from a local density bound of max(1.5 / sigma^3, 1.5 x the chain's mean density)
+        const double rho = cd.natoms / (cd.L[0] * cd.L[1] * cd.L[2]) * cd.sigma * cd.sigma * cd.sigma;
+        const double dens = std::max(1.5, 1.5 * rho);
+        cd.lcap = (int)std::ceil(dens * vol) + 16;
+        if (cd.lcap > 512)
+            return fail(c, CHMC_E_GEOMETRY, "chain " + std::to_string(k) + ": list cells too large for the list sweep (use CHMC_CELL_MD)");
+        const double rls = rl / cd.sigma;
+        nbmax = std::max(nbmax, 4.0 / 3.0 * M_PI * rls * rls * rls * dens + 16.0);
+        cd.lcell_off = (int)lcells;
+        cd.lncells = cd.lnbin[0] * cd.lnbin[1] * cd.lnbin[2];
+        cd.ldense_off = (int)dense;
+        for (int i = 0; i < cd.lncells; i++) hlc.push_back(k);
+        lcells += cd.lncells;
+        dense += (long long)cd.lncells * cd.lcap;
+        cd.tile_first = (int)c->htiles.size();
+        for (int b = 0; b < cd.natoms; b += kTile)
+            c->htiles.push_back(Chunk{ k, cd.atom_off + b, cd.atom_off + std::min(cd.natoms, b + kTile), 0 });
+        cd.ntiles = (int)c->htiles.size() - cd.tile_first;
+    }
+    if (lcells > (1LL << 28) || dense > (1LL << 30)) return fail(c, CHMC_E_ARG, "list grids too large");
+    const int nbcap = ((int)std::ceil(nbmax) + 31) / 32 * 32;
+    const size_t N = (size_t)c->Ntot;
+    if ((double)N * nbcap > 4.0e9) return fail(c, CHMC_E_ARG, "neighbour lists too large (reduce the skin)");
+    DevView& v = c->v;
+    v.Cl = (int)lcells;
+    v.ntiles = (int)c->htiles.size();
+    v.nbcap = nbcap;
+    int* lcell_chain = nullptr;
+    Chunk* tiles = nullptr;
+    HIPCHK(c, lalloc(c, &lcell_chain, (size_t)lcells));
+    HIPCHK(c, lalloc(c, &tiles, (size_t)v.ntiles));
+    HIPCHK(c, lalloc(c, &v.lcount, (size_t)lcells));
+    HIPCHK(c, lalloc(c, &v.loffs, (size_t)lcells + 1));
+    HIPCHK(c, lalloc(c, &v.lmem, (size_t)dense));
+    HIPCHK(c, lalloc(c, &v.lperm, N));
+    HIPCHK(c, lalloc(c, &v.linv, N));
+    HIPCHK(c, lalloc(c, &v.nb, N * nbcap));
+    HIPCHK(c, lalloc(c, &v.nbn, N));
+    HIPCHK(c, lalloc(c, &v.lflag, 4));
+    HIPCHK(c, lalloc(c, &v.lreq, 2 * (size_t)c->nchains));
+    HIPCHK(c, lalloc(c, &v.lscan_tmp, (size_t)lcells));
+    HIPCHK(c, lalloc(c, &v.x0, 3 * N));
+    HIPCHK(c, lalloc(c, &v.r0, 3 * N));
+    HIPCHK(c, lalloc(c, &v.rel64, 3 * N));
+    HIPCHK(c, lalloc(c, &v.xsl, 2 * N));
+    HIPCHK(c, lalloc(c, &v.tile_part, 4 * (size_t)v.ntiles));
+    v.lcell_chain = lcell_chain;
+    v.tiles = tiles;
+    HIPCHK(c, hipMemcpy(lcell_chain, hlc.data(), sizeof(int) * hlc.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(tiles, c->htiles.data(), sizeof(Chunk) * c->htiles.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_chains, c->hchains.data(), sizeof(ChainDev) * c->nchains, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(v.lcount, 0, sizeof(int) * lcells));
+    HIPCHK(c, hipMemset(v.lflag, 0, sizeof(int) * 4));
+    HIPCHK(c, hipMemset(v.lreq, 0, sizeof(int) * 2 * c->nchains));
+    // loffs of a chain's first cell is its atom offset; the end entry is Ntot
+    std::vector<int> lo0(lcells + 1, 0);
+    for (const ChainDev& cd : c->hchains)
+        for (int i = 0; i < cd.lncells; i++) lo0[cd.lcell_off + i] = cd.atom_off;
+    lo0[lcells] = c->Ntot;
+    HIPCHK(c, hipMemcpy(v.loffs, lo0.data(), sizeof(int) * lo0.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(v.nbn, 0, sizeof(int) * N));
+    // valid atom ids in every slot before the first rebuild (a slot is rewritten by each rebuild)
+    std::vector<int> iota(N);
+    std::vector<float4> xs0(2 * N);
+    for (size_t i = 0; i < N; i++) {
+        iota[i] = (int)i;
+        int ii = (int)i;
+        float w;
+        std::memcpy(&w, &ii, sizeof(float));
+        xs0[i] = xs0[N + i] = make_float4(0.f, 0.f, 0.f, w);
+    }
+    HIPCHK(c, hipMemcpy(v.lperm, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(v.linv, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(v.xsl, xs0.data(), sizeof(float4) * 2 * N, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(v.x0, 0, sizeof(double) * 3 * N));
+    HIPCHK(c, hipMemset(v.r0, 0, sizeof(double) * 3 * N));
+    HIPCHK(c, hipDeviceSynchronize());
+    c->skin = skin;
+    c->lpar = 0;
+    c->list_ready = true;
+    return CHMC_OK;
+}
+
+static int list_ensure(chmc_ctx* c)
+{
+    return c->list_ready ? CHMC_OK : list_setup(c, c->skin);
+}
+
+// request a rebuild before the next list sweep (positions changed outside the list path)
+static int list_invalidate(chmc_ctx* c)
+{
+    HIPCHK(c, launch_fill_int(c->v.lreq + (size_t)c->lpar * c->nchains, 1, c->nchains, c->stream));
+    return CHMC_OK;
+}
+
+static int list_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift, double drift_dt)
+{
+    HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));
+    const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0) | (fuse_drift ? LJ_DRIFT : 0);
+    HIPCHK(c, launch_list_sweep(c->v, mode, c->lpar, dt, fuse_drift ? drift_dt : 0.0, c->stream));
+    if (fuse_drift) c->lpar ^= 1;
+    c->force_kind = (storef && !fuse_drift) ? 1 : 0;
+    c->bin_state = 2;
+    return CHMC_OK;
+}
+
+static int list_drift(chmc_ctx* c, double dt)
+{
+    HIPCHK(c, launch_list_drift(c->v, c->lpar, dt, c->stream));
+    c->lpar ^= 1;
+    c->force_kind = 0;
+    c->bin_state = 2;
+    return CHMC_OK;
+}
+
+// FP64 energy / virial / kinetic per tile at the current positions (lists brought up to date first)
+static int list_energy(chmc_ctx* c, bool sums)
+{
+    HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));
+    HIPCHK(c, launch_list_energy(c->v, c->lpar, c->stream));
+    if (sums) HIPCHK(c, launch_reduce(c->v, SUM_TILES, c->stream));
+    return CHMC_OK;
+}
+
+static int list_check_overflow(chmc_ctx* c)
+{
+    int f[4] = { 0, 0, 0, 0 };
+    HIPCHK(c, hipMemcpyAsync(f, c->v.lflag, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (f[2]) return fail(c, CHMC_E_STATE, "neighbour list capacity exceeded (cell members or list entries); results invalid");
+    return CHMC_OK;
+}
+
 // integrator_type::integrate (integrator.cpp:52-131; code-carsten integrator.hh:200-240) as a list of
 // update_momenta (M) / update_positions (X) calls.  Every scheme alternates M and X; an M followed by an
 // X runs as one sweep (kick, then the drift and the count of the next binning); a trailing X is
@@ -688,6 +867,24 @@ static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double l
 {
     std::vector<MdOp> ops;
     if (build_ops(scheme, dt, steps, lambda, ops) != 0) return fail(c, CHMC_E_ARG, "unknown integrator scheme");
+    if (list_md(c) && !c->md_surr) {
+        // neighbour-list MD from a fresh rebuild of the starting positions
+        int rc = flush_drift(c);
+        if (rc) return rc;
+        if ((rc = list_ensure(c))) return rc;
+        if ((rc = list_invalidate(c))) return rc;
+        for (size_t k = 0; k < ops.size(); k++) {
+            if (ops[k].kick) {
+                const bool next_x = k + 1 < ops.size() && !ops[k + 1].kick;
+                rc = list_force_kick(c, ops[k].dt, storef, next_x, next_x ? ops[k + 1].dt : 0.0);
+                if (next_x) k++;
+            } else {
+                rc = list_drift(c, ops[k].dt);
+            }
+            if (rc) return rc;
+        }
+        return CHMC_OK;
+    }
     for (size_t k = 0; k < ops.size(); k++) {
         int rc;
         if (ops[k].kick && c->md_surr) {
@@ -720,6 +917,7 @@ extern "C" int chmc_integrate(chmc_ctx* c, int scheme, double dt, int steps, dou
     rc = flush_drift(c);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (list_md(c)) return list_check_overflow(c);
     return CHMC_OK;
 }
 
@@ -823,6 +1021,19 @@ struct Mt64 {
 static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps, int ufirst, int rec_base)
 {
     int rc;
+    if (list_md(c) && !c->md_surr) {
+        // heat bath + K_i; neighbour-list MD (fresh rebuild); FP64 V_f, W_f, K_f over the lists
+        HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, false, c->stream));
+        if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false))) return rc;
+        if (steps == 0 && (rc = list_invalidate(c))) return rc;
+        if ((rc = list_energy(c, false))) return rc;
+        HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, 0, SUM_TILES,
+                                    c->stream));
+        HIPCHK(c, launch_save_restore(c->v, 2, c->stream));
+        c->force_kind = 0;
+        c->bin_state = 2;
+        return CHMC_OK;
+    }
     // heat bath + K_i + the count phase of the binning of the (accepted or restored) positions
     HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, true, c->stream));
     c->bin_state = 1;
@@ -832,7 +1043,7 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
     // (euler/ruth) it is redone -- the reference rebins in either case (identical_particles.cpp:379)
     if ((rc = do_energy(c, false, false))) return rc;               // V_f, W_f and K_f partials
     HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, c->md_surr ? 1 : 0,
-                                c->stream));
+                                SUM_CELLS, c->stream));
     HIPCHK(c, launch_save_restore(c->v, 2, c->stream));
     c->force_kind = 0;
     c->bin_state = 2;
@@ -842,7 +1053,7 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
 // a captured trajectory sequence: valid for replays that start at binning parity `par` in MD mode `surr`
 struct TrajGraph {
     hipGraphExec_t exec;
-    int par, per_graph;
+    int par, lpar, per_graph;
     bool surr;
 };
 
@@ -862,7 +1073,7 @@ static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count
     int done = 0;
     const TrajGraph* g = nullptr;
     for (const TrajGraph& k : graphs)
-        if (k.par == c->v.par && k.surr == c->md_surr) g = &k;
+        if (k.par == c->v.par && k.lpar == c->lpar && k.surr == c->md_surr) g = &k;
     if (!g) {
         // one trajectory per graph when it performs an even number of placements (the binning parity
         // baked into the captured launches then repeats), two otherwise.  The host state after the
@@ -870,11 +1081,11 @@ static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        const int par0 = c->v.par;
+        const int par0 = c->v.par, lpar0 = c->lpar;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         rc = enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base);
         int per_graph = 1;
-        if (!rc && c->v.par != par0) {
+        if (!rc && (c->v.par != par0 || c->lpar != lpar0)) {
             rc = enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base);
             per_graph = 2;
         }
@@ -884,12 +1095,13 @@ static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count
         hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         HIPCHK(c, ei);
-        graphs.push_back(TrajGraph{ exec, par0, per_graph, c->md_surr });
+        graphs.push_back(TrajGraph{ exec, par0, lpar0, per_graph, c->md_surr });
         g = &graphs.back();
         if (count < per_graph) {
             // nothing of the capture ran; the host state is restored by running the leftover directly
-            // from the capture's start parity
+            // from the capture's start parities
             c->v.par = par0;
+            c->lpar = lpar0;
         } else {
             HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
             done = per_graph;
@@ -908,6 +1120,7 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     if (hp->ntraj < 1 || hp->steps < 1 || !scheme_ok(hp->scheme) || hp->therm < 0)
         return fail(c, CHMC_E_ARG, "hmc_run: ntraj >= 1, steps >= 1, therm >= 0 and a known scheme are required");
     if (hp->surrogate && !c->surr_params) return fail(c, CHMC_E_STATE, "hmc_run: surrogate parameters not set");
+    if (list_md(c) && (rc = list_ensure(c))) return rc;          // allocations stay outside graph capture
     if (hp->surrogate && (rc = surr_geometry(c))) return rc;
     const int nch = c->nchains, ntraj = hp->ntraj;
     rc = flush_drift(c);
@@ -946,7 +1159,11 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     HIPCHK(c, hipMemcpyAsync(c->d_uniforms, u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice, c->stream));
 
     // ---- V_i, W_i of the start configuration (HMC.cpp:64), h_x = x (HMC.cpp:72) ----
-    if ((rc = do_energy(c, true))) return rc;
+    if (list_md(c) && !hp->surrogate) {
+        if ((rc = list_ensure(c)) || (rc = list_invalidate(c)) || (rc = list_energy(c, true))) return rc;
+    } else if ((rc = do_energy(c, true))) {
+        return rc;
+    }
     std::vector<double> red((size_t)nch * RED_NSLOT);
     HIPCHK(c, hipMemcpyAsync(red.data(), c->v.red, sizeof(double) * red.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -982,6 +1199,7 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     c->md_surr = false;
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (list_md(c) && (rc = list_check_overflow(c))) return rc;
     c->traj_done += ntraj;
     c->bin_state = 2;
     if (records) HIPCHK(c, hipMemcpy(records, c->d_rec, sizeof(HmcRecord) * nrec, hipMemcpyDeviceToHost));
@@ -1367,4 +1585,71 @@ extern "C" int chmc_time_force(chmc_ctx* c, int reps, double* ms_bin, double* ms
     CHECK_CTX(c);
     if (reps < 1) return CHMC_E_ARG;
     return time_force_is(c, c->islots, reps, ms_bin, ms_force);
+}
+
+extern "C" int chmc_set_skin(chmc_ctx* c, double skin)
+{
+    CHECK_CTX(c);
+    if (!(skin > 0.0) || skin > 2.0) return fail(c, CHMC_E_ARG, "set_skin: 0 < skin <= 2 required");
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    return list_setup(c, skin);
+}
+
+extern "C" int chmc_get_skin(const chmc_ctx* c, double* skin)
+{
+    if (!c || !skin) return CHMC_E_ARG;
+    *skin = c->skin;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_list_stats(chmc_ctx* c, int chain, long long* entries, long long* nint, long long* rebuilds)
+{
+    if (!c || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    if ((rc = list_ensure(c)) || (rc = list_invalidate(c)) || (rc = list_energy(c, true))) return rc;
+    std::vector<double> nintv(c->nchains);
+    if ((rc = read_red(c, RED_NINT, nintv.data()))) return rc;
+    const ChainDev& cd = c->hchains[chain];
+    std::vector<int> n(cd.natoms);
+    HIPCHK(c, hipMemcpy(n.data(), c->v.nbn + cd.atom_off, sizeof(int) * cd.natoms, hipMemcpyDeviceToHost));
+    int f[4];
+    HIPCHK(c, hipMemcpy(f, c->v.lflag, sizeof(f), hipMemcpyDeviceToHost));
+    long long tot = 0;
+    for (int k : n) tot += k;
+    if (entries) *entries = tot;
+    if (nint) *nint = (long long)std::llround(nintv[chain]);
+    if (rebuilds) *rebuilds = f[3];
+    return f[2] ? fail(c, CHMC_E_STATE, "neighbour list capacity exceeded") : CHMC_OK;
+}
+
+extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* ms_rebuild)
+{
+    CHECK_CTX(c);
+    if (reps < 1) return CHMC_E_ARG;
+    if (fp64_md(c)) return fail(c, CHMC_E_STATE, "time_list: the list sweep is the FP32 path");
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    if ((rc = list_ensure(c))) return rc;
+    float tb = 0.f, ts = 0.f;
+    for (int r = 0; r < reps + 1; r++) {
+        if ((rc = list_invalidate(c))) return rc;
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        // the trajectory's MD step: force, kick and drift with dt = 0 (positions and momenta unchanged)
+        HIPCHK(c, launch_list_sweep(c->v, LJ_FORCE | LJ_KICK | LJ_DRIFT, c->lpar, 0.0, 0.0, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev2, c->stream));
+        c->lpar ^= 1;
+        HIPCHK(c, hipEventSynchronize(c->ev2));
+        float a = 0.f, b = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a, c->ev0, c->ev1));
+        HIPCHK(c, hipEventElapsedTime(&b, c->ev1, c->ev2));
+        if (r > 0) { tb += a; ts += b; }      // the first pass warms up
+    }
+    c->force_kind = 0;
+    if (ms_sweep) *ms_sweep = ts / reps;
+    if (ms_rebuild) *ms_rebuild = tb / reps;
+    return list_check_overflow(c);
 }

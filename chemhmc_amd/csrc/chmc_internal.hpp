@@ -18,6 +18,23 @@
 //   cell_chain    : int [Ctot]
 // Chains are contiguous in both the atom and the cell index spaces, so the global scan of the
 // per-cell counts equals chain atom offset + the chain's own exclusive scan.
+//
+// Neighbour-list MD (FP32 forces between rebuilds; cells of side >= rc + skin on a second grid):
+//   lcount        : int [Cl]                  (zero between rebuilds)
+//   loffs         : int [Cl + 1]              (exclusive scan: first compact slot of each list cell)
+//   lmem          : int [sum_chains lncells * lcap]  (members of each cell in arrival order)
+//   lperm / linv  : int [Ntot]                (compact slot -> atom / atom -> slot, atom-id order
+//                                              inside a cell)
+//   x0, r0        : double SoA [3][Ntot] by slot (position at the rebuild; minus its cell origin)
+//   xsl           : float4 [2][Ntot] by slot  (current position relative to the rebuild cell origin,
+//                                              FP32, atom id in w; double-buffered across drifts)
+//   nb            : int [Ntot][nbcap]         (j slot << 5 | image code 0..26 of the 27-cell stencil)
+//   nbn           : int [Ntot]
+//   lreq          : int [2][nchains]          (chain needs a rebuild before its next sweep at list
+//                                              parity q; chains rebuild independently)
+//   lflag         : int [4]                   ([2]: capacity overflow, [3]: chain rebuilds run)
+//   lscan_tmp     : int [Cl]                  (global exclusive scan of the counts, rebuild scratch)
+//   tiles         : 32-slot ranges that never straddle chains (sweep / energy workgroups)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -29,9 +46,13 @@ struct ChainDev {
     int nbin[3];
     int chunk_first, nchunks;      // deterministic reduction chunks of this chain
     int pad_;
+    int lnbin[3], lcell_off, lncells, lcap, ldense_off, tile_first, ntiles;
+    int lpad_;
     double L[3], sizebin[3];
     double rc, rc2, sigma, sigma2, eps, mass, beta;
     float  sizebin_f[3], rc2_f, sigma2_f, eps48_f;
+    double lsize[3], half_skin2;   // list cell side, (skin / 2)^2
+    float  lsize_f[3], rl2_f;      // list radius^2 = (rc + skin)^2
 };
 
 // per-chain reduction slots
@@ -52,7 +73,8 @@ struct HmcRecord {
 struct Chunk { int chain, begin, end, pad_; };
 
 constexpr int kMaxJ = 768;         // j-list entries staged in LDS per pass
-constexpr int kRedChunk = 1024;    // atoms per deterministic reduction chunk
+constexpr int kRedChunk = 256;     // atoms per deterministic reduction chunk
+constexpr int kTile = 32;          // slots per neighbour-list sweep workgroup (8 lanes per slot)
 
 // ---- launch wrappers (each defined in the TU that owns the kernel) ----------------------
 struct DevView {
@@ -71,6 +93,14 @@ struct DevView {
     double* cell_part;    // [Ctot * 4]: energy, virial, interacting pairs, kinetic energy
     ChainState* state;    // [nchains]
     int* traj;            // [1] device trajectory counter (1-based index of the last finished one)
+    // neighbour-list MD
+    int Cl, ntiles, nbcap, lpad_;
+    const int* lcell_chain;
+    const Chunk* tiles;
+    int *lcount, *loffs, *lmem, *lperm, *linv, *nb, *nbn, *lflag, *lreq, *lscan_tmp;
+    double *x0, *r0, *rel64;
+    float4* xsl;
+    double* tile_part;    // [ntiles * 4]: energy, virial, interacting pairs, kinetic energy
 };
 
 enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8, LJ_DRIFT = 16 };
@@ -89,7 +119,7 @@ hipError_t launch_scan(const DevView& v, hipStream_t s);
 hipError_t launch_place(const DevView& v, bool local, bool atom_id_order, hipStream_t s);
 
 // per-chain sums of the per-cell (V, W, pairs, K) and per-chunk (K) partials into red[]
-enum { SUM_CELLS = 1, SUM_CHUNKS_KI = 2, SUM_CHUNKS_K = 4 };
+enum { SUM_CELLS = 1, SUM_CHUNKS_KI = 2, SUM_CHUNKS_K = 4, SUM_TILES = 8 };
 hipError_t launch_reduce(const DevView& v, int what, hipStream_t s);
 hipError_t launch_drift(const DevView& v, double dt, hipStream_t s);
 hipError_t launch_kick(const DevView& v, double dt, bool fp64, hipStream_t s);
@@ -98,8 +128,16 @@ hipError_t launch_lj(const DevView& v, int mode, bool fp64, int tiling, double k
 hipError_t launch_kinetic(const DevView& v, hipStream_t s);
 hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, hipStream_t s);
 hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustride, int therm, int ufirst,
-                             HmcRecord* rec, int rec_base, int surr, hipStream_t s);
+                             HmcRecord* rec, int rec_base, int surr, int what, hipStream_t s);
 hipError_t launch_save_restore(const DevView& v, int mode, hipStream_t s);
+
+// neighbour-list MD: the three rebuild kernels (each a no-op unless lflag[q] is set), the FP32
+// force sweep (LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT; with LJ_DRIFT the parity advances to q ^ 1),
+// a standalone drift (q -> q ^ 1) and the FP64 energy / virial / kinetic sums into tile_part
+hipError_t launch_list_rebuild(const DevView& v, int q, hipStream_t s);
+hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, hipStream_t s);
+hipError_t launch_list_drift(const DevView& v, int q, double dt, hipStream_t s);
+hipError_t launch_list_energy(const DevView& v, int q, hipStream_t s);
 hipError_t launch_rdf(const DevView& v, int chain, int natoms, int nbins, double size_bin,
                       unsigned long long* counts, hipStream_t s);
 hipError_t launch_open_potential(const double* xyz_soa, int n, int nexp, double a, double b,

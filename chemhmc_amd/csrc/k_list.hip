@@ -1,0 +1,496 @@
+// Neighbour-list MD for the FP32 force path: the same truncated LJ forces as the cell sweep of
+// k_lj.hip (every pair with r^2 < rc^2 under the 27-cell image rules of identical_particles.cpp:
+// 584-600), but the candidate set is a per-atom list of the pairs within rc + skin, rebuilt only when
+// an atom has moved more than skin / 2 since the last rebuild (so no pair can have entered rc
+// unlisted).  The reference re-bins on every force evaluation (identical_particles.cpp:379,560); here
+// a force evaluation between rebuilds is one sweep over ~(4/3) pi (rc + skin)^3 rho candidates per atom
+// instead of 27 cells' worth.
+//
+// Rebuild (three kernels that skip every chain whose request flag of the current list parity is clear,
+// so that they can sit in a captured trajectory and cost a launch when idle; chains rebuild
+// independently, so a chain of a batch evolves exactly as it does alone):
+//   k_lcount  atoms -> list cells (floor(x / lsize), clamped), arrival order into a dense member array
+//   k_lscan   one workgroup: exclusive scan of the cell counts -> first compact slot per cell
+//   k_lbuild  one workgroup per list cell: members sorted by atom id give the compact slots; the cell
+//             writes its own atoms' slot data (x0, r0 = x0 - cell origin, FP32 xsl) and their lists,
+//             staging the 27 neighbour cells (image shift dd * lsize folded in) the way k_lj does, with
+//             each neighbour's slot recomputed from its sorted rank, so no workgroup reads data another
+//             writes in the same launch
+// Sweep: 32 slots x 8 lanes per workgroup; each lane walks every 8th list entry, gathers the neighbour's
+//   FP32 coordinates, and the 8 partial sums meet in a butterfly.  Lane 0 of an atom kicks p in FP64,
+//   optionally drifts x in FP64 (identical_particles.cpp:651-673), writes the FP32 coordinates of the
+//   other parity and raises the rebuild request when the atom is beyond skin / 2 from x0.
+// Energy: FP64 coordinates relative to the rebuild cell origins (rel64 = r0 + wrap(x - x0)), the
+//   potential's r < rc test (identical_particles.cpp:432-435), per-tile fixed-order partials.
+#include "chmc_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace chmc {
+
+__device__ __forceinline__ int clamp_l(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// list cell of a position (global list-cell index), and its grid coordinates
+__device__ __forceinline__ int list_cell(const ChainDev& C, const double x3[3], int b[3])
+{
+#pragma unroll
+    for (int d = 0; d < 3; d++) b[d] = clamp_l((int)floor(x3[d] / C.lsize[d]), 0, C.lnbin[d] - 1);
+    return C.lcell_off + b[0] + C.lnbin[0] * (b[1] + b[2] * C.lnbin[1]);
+}
+
+// ---- rebuild -------------------------------------------------------------------------------------
+__device__ __forceinline__ int* req(const DevView& v, int q, int chain) { return v.lreq + q * v.nchains + chain; }
+
+__global__ __launch_bounds__(256) void k_lcount(DevView v, int q)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= v.Ntot) return;
+    const int ch = v.atom_chain[a];
+    if (*req(v, q, ch) == 0) return;
+    const ChainDev& C = v.chains[ch];
+    const int N = v.Ntot;
+    const double x3[3] = { v.x[a], v.x[N + a], v.x[2 * N + a] };
+    int b[3];
+    const int c = list_cell(C, x3, b);
+    const int pos = atomicAdd(&v.lcount[c], 1);
+    if (pos < C.lcap) v.lmem[C.ldense_off + (c - C.lcell_off) * C.lcap + pos] = a;
+    else atomicOr(&v.lflag[2], 1);
+}
+
+// global exclusive scan of the counts (zero in chains that did not count), then each rebuilding chain's
+// cells get atom_off + the scan relative to the chain's first cell; other chains keep their offsets
+__global__ __launch_bounds__(1024) void k_lscan(DevView v, int q)
+{
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < v.nchains; k += blockDim.x)
+        if (*req(v, q, k)) atomicAdd(&any, 1);
+    __syncthreads();
+    if (any == 0) return;
+    __shared__ int wsum[16];
+    const int nt = blockDim.x, tid = threadIdx.x;
+    const int n = v.Cl;
+    const int per = (n + nt - 1) / nt;
+    const int beg = min(tid * per, n), end = min(beg + per, n);
+    int s = 0;
+    for (int c = beg; c < end; c++) s += v.lcount[c];
+    const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
+    int incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+        int w = lane < nw ? wsum[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int t = __shfl_up(w, o, 64);
+            if (lane >= o) w += t;
+        }
+        if (lane < nw) wsum[lane] = w;
+    }
+    __syncthreads();
+    int run = incl - s + (wid > 0 ? wsum[wid - 1] : 0);
+    for (int c = beg; c < end; c++) {
+        v.lscan_tmp[c] = run;
+        run += v.lcount[c];
+    }
+    __syncthreads();
+    for (int c = beg; c < end; c++) {
+        const int k = v.lcell_chain[c];
+        if (*req(v, q, k) == 0) continue;
+        const ChainDev& C = v.chains[k];
+        v.loffs[c] = C.atom_off + v.lscan_tmp[c] - v.lscan_tmp[C.lcell_off];
+        v.lcount[c] = 0;
+    }
+    if (tid == 0) v.lflag[3] += any;        // chain rebuilds run (statistics)
+}
+
+constexpr int kLMaxJ = 768;       // staged neighbour entries per pass
+constexpr int kLMaxI = 512;       // members of one list cell (lcap never exceeds this)
+
+// rank of atom `a` among the members [base, base + n) of a cell, by atom id
+__device__ __forceinline__ int member_rank(const int* __restrict__ mem, int base, int n, int a)
+{
+    int r = 0;
+    for (int m = 0; m < n; m++) r += mem[base + m] < a ? 1 : 0;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
+{
+    if (*req(v, q, v.lcell_chain[blockIdx.x]) == 0) return;
+    constexpr int IS = 4, JG = 16, NT = 256, W = 4;
+    __shared__ float4 sj[kLMaxJ];          // staged j: coordinates relative to the i-cell origin, slot in w
+    __shared__ unsigned char scode[kLMaxJ];
+    __shared__ float4 si[kLMaxI];          // own atoms by rank: FP32 r0, slot in w
+    __shared__ int icount[kLMaxI];
+    __shared__ int seg_beg[27], seg_pref[28], seg_slot0[27], seg_cnt[27];
+    __shared__ float seg_sh[27][3];
+    __shared__ double seg_org[27][3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = blockIdx.x;
+    const ChainDev& C = v.chains[v.lcell_chain[c]];
+    const int nx = C.lnbin[0], ny = C.lnbin[1], nz = C.lnbin[2];
+    const int lc = c - C.lcell_off;
+    const int bz = lc / (nx * ny), by = (lc - bz * nx * ny) / nx, bx = lc - nx * (by + bz * ny);
+    // member counts are clamped to the dense capacity: after an overflow (flagged by k_lcount) the
+    // missing atoms are simply absent and the host rejects the results
+    const int ibeg = v.loffs[c], ni = min(v.loffs[c + 1] - ibeg, C.lcap);
+    if (ni == 0) return;
+    const int N = v.Ntot;
+    const int* mem = v.lmem;
+    const int ibase = C.ldense_off + lc * C.lcap;
+    const int nbcap = v.nbcap;
+
+    if (wave == 0) {
+        int cnt = 0;
+        if (lane < 27) {
+            const int dd[3] = { lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1 };
+            const int b3[3] = { bx, by, bz }, n3[3] = { nx, ny, nz };
+            int j3[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                j3[d] = b3[d] + dd[d];
+                if (j3[d] < 0 || j3[d] >= n3[d]) j3[d] = (b3[d] + n3[d] + dd[d]) % n3[d];
+                seg_sh[lane][d] = (float)dd[d] * C.lsize_f[d];
+                seg_org[lane][d] = (double)j3[d] * C.lsize[d];
+            }
+            const int jl = j3[0] + nx * (j3[1] + j3[2] * ny);
+            const int jc = C.lcell_off + jl;
+            seg_beg[lane] = C.ldense_off + jl * C.lcap;
+            seg_slot0[lane] = v.loffs[jc];
+            cnt = min(v.loffs[jc + 1] - v.loffs[jc], C.lcap);
+            seg_cnt[lane] = cnt;
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane < 27) seg_pref[lane + 1] = incl;
+        if (lane == 0) seg_pref[0] = 0;
+    }
+    // own atoms: compact slot by rank, rebuild reference data, FP32 coordinates of this parity
+    for (int m = tid; m < ni; m += NT) {
+        const int a = mem[ibase + m];
+        const int r = member_rank(mem, ibase, ni, a);
+        const int slot = ibeg + r;
+        const double o3[3] = { (double)bx * C.lsize[0], (double)by * C.lsize[1], (double)bz * C.lsize[2] };
+        float f3[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double xv = v.x[d * N + a];
+            const double rv = xv - o3[d];
+            v.x0[d * N + slot] = xv;
+            v.r0[d * N + slot] = rv;
+            f3[d] = (float)rv;
+        }
+        const float4 xf = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
+        v.xsl[(size_t)q * N + slot] = xf;
+        v.lperm[slot] = a;
+        v.linv[a] = slot;
+        si[r] = make_float4(f3[0], f3[1], f3[2], __int_as_float(slot));
+        icount[r] = 0;
+    }
+    __syncthreads();
+    const int nj = seg_pref[27];
+    const float rl2 = C.rl2_f;
+    const int ii = lane % IS, g = lane / IS;
+    unsigned long long pattern = 0;
+#pragma unroll
+    for (int k = 0; k < JG; k++) pattern |= 1ull << (k * IS + ii);
+    const int nchunk = (ni + IS - 1) / IS;
+    for (int jlo = 0; jlo < nj; jlo += kLMaxJ) {
+        const int jhi = min(nj, jlo + kLMaxJ);
+        if (jlo > 0) __syncthreads();
+        // every member goes to position seg_pref[k] + (its rank in its cell), so that the staged order,
+        // and with it the list order and the FP32 summation order, is independent of the arrival order
+        // of k_lcount's atomics; with several passes each pass scans all entries and keeps its window
+        for (int t = tid; t < nj; t += NT) {
+            int k = 0;
+#pragma unroll
+            for (int step = 16; step > 0; step >>= 1)
+                if (k + step < 27 && seg_pref[k + step] <= t) k += step;
+            const int a = mem[seg_beg[k] + (t - seg_pref[k])];
+            const int r = member_rank(mem, seg_beg[k], seg_cnt[k], a);
+            const int dst = seg_pref[k] + r;
+            if (dst < jlo || dst >= jhi) continue;
+            float f3[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) f3[d] = (float)(v.x[d * N + a] - seg_org[k][d]);   // its xsl value
+            sj[dst - jlo] = make_float4(f3[0] + seg_sh[k][0], f3[1] + seg_sh[k][1], f3[2] + seg_sh[k][2],
+                                        __int_as_float(seg_slot0[k] + r));
+            scode[dst - jlo] = (unsigned char)k;
+        }
+        __syncthreads();
+        const int jn = jhi - jlo;
+        for (int chunk = wave; chunk < nchunk; chunk += W) {
+            const int i = chunk * IS + ii;
+            const bool ival = i < ni;
+            const float4 xi = si[ival ? i : ni - 1];
+            const int slot_i = __float_as_int(xi.w);
+            int count = icount[ival ? i : ni - 1];
+            for (int j0 = 0; j0 < jn; j0 += JG) {
+                const int j = j0 + g;
+                bool hit = false;
+                if (ival && j < jn) {
+                    const float4 p = sj[j];
+                    const float dx = xi.x - p.x, dy = xi.y - p.y, dz = xi.z - p.z;
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    hit = r2 < rl2 && r2 > 0.f;
+                }
+                const unsigned long long m = __ballot(hit) & pattern;
+                if (hit) {
+                    const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
+                    if (pos < nbcap)
+                        v.nb[(size_t)slot_i * nbcap + pos] = (__float_as_int(sj[j].w) << 5) | (int)scode[j];
+                }
+                count += __popcll(m);
+            }
+            if (ival && g == 0) icount[i] = count;
+        }
+    }
+    __syncthreads();
+    for (int r = tid; r < ni; r += NT) {
+        const int n = icount[r];
+        v.nbn[ibeg + r] = min(n, nbcap);
+        if (n > nbcap) atomicOr(&v.lflag[2], 2);
+    }
+}
+
+hipError_t launch_list_rebuild(const DevView& v, int q, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lcount, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, q);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_lscan, dim3(1), dim3(1024), 0, s, v, q);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_lbuild, dim3(v.Cl), dim3(256), 0, s, v, q);
+    return hipGetLastError();
+}
+
+// ---- sweep ---------------------------------------------------------------------------------------
+// drift of one atom by dt (FP64, wrap into [0, L)), its FP32 list coordinates of the next parity and
+// the rebuild request when it is beyond skin / 2 from its rebuild position
+__device__ __forceinline__ void list_drift_atom(const DevView& v, const ChainDev& C, int atom, int slot, int q,
+                                                double dt, const double pn[3])
+{
+    const int N = v.Ntot;
+    const double coeff_x = C.beta / C.mass;       // compute_coeff_position, identical_particles.cpp:646-648
+    float f3[3];
+    double disp2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double xv = v.x[d * N + atom];
+        xv += dt * coeff_x * pn[d];
+        xv -= C.L[d] * floor(xv / C.L[d]);
+        v.x[d * N + atom] = xv;
+        double e = xv - v.x0[d * N + slot];
+        e -= C.L[d] * rint(e / C.L[d]);
+        disp2 += e * e;
+        f3[d] = (float)(v.r0[d * N + slot] + e);
+    }
+    v.xsl[(size_t)(q ^ 1) * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(atom));
+    if (disp2 > C.half_skin2) atomicOr(req(v, q ^ 1, v.atom_chain[atom]), 1);
+}
+
+__device__ __forceinline__ void load_shifts(float4* sh, const ChainDev& C)
+{
+    const int t = threadIdx.x;
+    if (t < 27)
+        sh[t] = make_float4((float)(t % 3 - 1) * C.lsize_f[0], (float)((t / 3) % 3 - 1) * C.lsize_f[1],
+                            (float)(t / 9 - 1) * C.lsize_f[2], 0.f);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q)
+{
+    __shared__ float4 sh[27];
+    const Chunk tl = v.tiles[blockIdx.x];
+    const ChainDev& C = v.chains[tl.chain];
+    load_shifts(sh, C);
+    if (tl.begin == C.atom_off && threadIdx.x == 0) *req(v, q, tl.chain) = 0;   // this evaluation's rebuild is done
+    __syncthreads();
+    const int N = v.Ntot;
+    const int t = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    const int s = tl.begin + t;
+    const bool val = s < tl.end;
+    const float4* __restrict__ xq = v.xsl + (size_t)q * N;
+    const float rc2 = C.rc2_f, sig2 = C.sigma2_f;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (val) {
+        xi = xq[s];
+        const int n = v.nbn[s];
+        const int* __restrict__ nbs = v.nb + (size_t)s * v.nbcap;
+#pragma unroll 4
+        for (int k = sub; k < n; k += 8) {
+            const int e = nbs[k];
+            const float4 pj = xq[e >> 5];
+            const float4 shf = sh[e & 31];
+            const float dx = xi.x - (pj.x + shf.x), dy = xi.y - (pj.y + shf.y), dz = xi.z - (pj.z + shf.z);
+            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            const float inv = __builtin_amdgcn_rcpf(r2);
+            const float s2 = sig2 * inv;
+            const float s6 = s2 * s2 * s2;
+            float gq = s6 * (0.5f - s6) * inv;            // sr6 (0.5 - sr6) / r^2, identical_particles.cpp:611-613
+            gq = (r2 < rc2 && r2 > 0.f) ? gq : 0.f;
+            ax = __builtin_fmaf(gq, dx, ax);
+            ay = __builtin_fmaf(gq, dy, ay);
+            az = __builtin_fmaf(gq, dz, az);
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) {
+        ax += __shfl_xor(ax, m, 64);
+        ay += __shfl_xor(ay, m, 64);
+        az += __shfl_xor(az, m, 64);
+    }
+    if (!val || sub != 0) return;
+    const int atom = __float_as_int(xi.w);
+    const float f3[3] = { C.eps48_f * ax, C.eps48_f * ay, C.eps48_f * az };
+    const double kdt = kick_dt * C.beta;               // dt * coeff_p, coeff_p = beta (:642-644)
+    double pn[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if constexpr ((MODE & LJ_STOREF) != 0) v.f32[d * N + atom] = f3[d];
+        pn[d] = v.p[d * N + atom];
+        if constexpr ((MODE & LJ_KICK) != 0) {
+            pn[d] = pn[d] - kdt * (double)f3[d];
+            v.p[d * N + atom] = pn[d];
+        }
+    }
+    if constexpr ((MODE & LJ_DRIFT) != 0) list_drift_atom(v, C, atom, s, q, drift_dt, pn);
+}
+
+__global__ __launch_bounds__(256) void k_ldrift(DevView v, double dt, int q)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.Ntot) return;
+    const int atom = v.lperm[s];
+    const int ch = v.atom_chain[atom];
+    const ChainDev& C = v.chains[ch];
+    if (s == C.atom_off) {
+        // a pending request of parity q carries over to q ^ 1 (nothing consumed it)
+        const int pend = *req(v, q, ch);
+        *req(v, q, ch) = 0;
+        if (pend) atomicOr(req(v, q ^ 1, ch), pend);
+    }
+    const int N = v.Ntot;
+    const double pn[3] = { v.p[atom], v.p[N + atom], v.p[2 * N + atom] };
+    list_drift_atom(v, C, atom, s, q, dt, pn);
+}
+
+// ---- FP64 energy ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_lrel64(DevView v)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.Ntot) return;
+    const int atom = v.lperm[s];
+    const ChainDev& C = v.chains[v.atom_chain[atom]];
+    const int N = v.Ntot;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double e = v.x[d * N + atom] - v.x0[d * N + s];
+        e -= C.L[d] * rint(e / C.L[d]);
+        v.rel64[d * N + s] = v.r0[d * N + s] + e;
+    }
+}
+
+__device__ __forceinline__ double tile_sum_256(double s, double* part)
+{
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) s += __shfl_xor(s, m, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    return ((part[0] + part[1]) + part[2]) + part[3];
+}
+
+__global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
+{
+    __shared__ double wpart[4];
+    const Chunk tl = v.tiles[blockIdx.x];
+    const ChainDev& C = v.chains[tl.chain];
+    if (tl.begin == C.atom_off && threadIdx.x == 0) *req(v, q, tl.chain) = 0;
+    const int N = v.Ntot;
+    const int t = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    const int s = tl.begin + t;
+    const bool val = s < tl.end;
+    double au = 0.0, aw = 0.0, an = 0.0, ak = 0.0;
+    if (val) {
+        const double xi = v.rel64[s], yi = v.rel64[N + s], zi = v.rel64[2 * N + s];
+        const int n = v.nbn[s];
+        const int* __restrict__ nbs = v.nb + (size_t)s * v.nbcap;
+        for (int k = sub; k < n; k += 8) {
+            const int e = nbs[k];
+            const int j = e >> 5, code = e & 31;
+            const double dx = xi - (v.rel64[j] + (double)(code % 3 - 1) * C.lsize[0]);
+            const double dy = yi - (v.rel64[N + j] + (double)((code / 3) % 3 - 1) * C.lsize[1]);
+            const double dz = zi - (v.rel64[2 * N + j] + (double)(code / 9 - 1) * C.lsize[2]);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (sqrt(r2) < C.rc && r2 > 0.0) {                 // identical_particles.cpp:432-435
+                const double s2 = C.sigma2 / r2;
+                const double s6 = s2 * s2 * s2;
+                au += s6 * (s6 - 1.0);
+                aw += s6 * (s6 - 0.5);
+                an += 1.0;
+            }
+        }
+        if (sub == 0) {
+            const int atom = v.lperm[s];
+            const double px = v.p[atom], py = v.p[N + atom], pz = v.p[2 * N + atom];
+            ak = (px * px + py * py + pz * pz) / (2 * C.mass);
+        }
+    }
+    au = tile_sum_256(au, wpart);
+    aw = tile_sum_256(aw, wpart);
+    an = tile_sum_256(an, wpart);
+    ak = tile_sum_256(ak, wpart);
+    if (threadIdx.x == 0) {
+        double* tp = &v.tile_part[4 * blockIdx.x];
+        tp[0] = au; tp[1] = aw; tp[2] = an; tp[3] = ak;
+    }
+}
+
+// ---- launchers -----------------------------------------------------------------------------------
+hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, hipStream_t s)
+{
+    const dim3 g(v.ntiles), b(256);
+    switch (mode) {
+    case LJ_FORCE | LJ_KICK: hipLaunchKernelGGL(k_lsweep<LJ_FORCE | LJ_KICK>, g, b, 0, s, v, kick_dt, drift_dt, q); break;
+    case LJ_FORCE | LJ_KICK | LJ_STOREF:
+        hipLaunchKernelGGL(k_lsweep<LJ_FORCE | LJ_KICK | LJ_STOREF>, g, b, 0, s, v, kick_dt, drift_dt, q); break;
+    case LJ_FORCE | LJ_KICK | LJ_DRIFT:
+        hipLaunchKernelGGL(k_lsweep<LJ_FORCE | LJ_KICK | LJ_DRIFT>, g, b, 0, s, v, kick_dt, drift_dt, q); break;
+    case LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT:
+        hipLaunchKernelGGL(k_lsweep<LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT>, g, b, 0, s, v, kick_dt, drift_dt, q); break;
+    case LJ_FORCE | LJ_STOREF:
+        hipLaunchKernelGGL(k_lsweep<LJ_FORCE | LJ_STOREF>, g, b, 0, s, v, kick_dt, drift_dt, q); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_list_drift(const DevView& v, int q, double dt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ldrift, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, dt, q);
+    return hipGetLastError();
+}
+
+hipError_t launch_list_energy(const DevView& v, int q, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lrel64, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_lenergy, dim3(v.ntiles), dim3(256), 0, s, v, q);
+    return hipGetLastError();
+}
+
+}  // namespace chmc

@@ -100,6 +100,23 @@ __device__ void chain_sums(const DevView& v, int ch, int what)
             red[RED_K] = sk;
         }
     }
+    if (what & SUM_TILES) {
+        double su = 0.0, sw = 0.0, sn = 0.0, sk = 0.0;
+        for (int q = threadIdx.x; q < C.ntiles; q += 256) {
+            const double* tp = &v.tile_part[4 * (C.tile_first + q)];
+            su += tp[0]; sw += tp[1]; sn += tp[2]; sk += tp[3];
+        }
+        su = block_sum_256(su);
+        sw = block_sum_256(sw);
+        sn = block_sum_256(sn);
+        sk = block_sum_256(sk);
+        if (threadIdx.x == 0) {
+            red[RED_V] = 2 * C.eps * su;
+            red[RED_W] = 24 * C.eps * sw;
+            red[RED_NINT] = sn;
+            red[RED_K] = sk;
+        }
+    }
     if (what & (SUM_CHUNKS_KI | SUM_CHUNKS_K)) {
         double q = 0.0;
         for (int k = threadIdx.x; k < C.nchunks; k += 256) q += v.chunk_part[C.chunk_first + k];
@@ -117,10 +134,10 @@ __global__ __launch_bounds__(256) void k_reduce(DevView v, int what)
 // then the Metropolis decision.
 __global__ __launch_bounds__(256) void k_metropolis(DevView v, const double* __restrict__ uniforms, int ustride,
                                                     int therm, int ufirst, HmcRecord* __restrict__ rec, int rec_base,
-                                                    int surr)
+                                                    int surr, int what)
 {
     const int ch = blockIdx.x;
-    chain_sums(v, ch, SUM_CELLS | SUM_CHUNKS_KI);
+    chain_sums(v, ch, what | SUM_CHUNKS_KI);
     if (threadIdx.x != 0) return;
     const int i = *v.traj + 1;                 // 1-based index of this trajectory (first_traj = istart + 1)
     const ChainDev& C = v.chains[ch];
@@ -199,10 +216,10 @@ hipError_t launch_reduce(const DevView& v, int what, hipStream_t s)
 }
 
 hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustride, int therm, int ufirst,
-                             HmcRecord* rec, int rec_base, int surr, hipStream_t s)
+                             HmcRecord* rec, int rec_base, int surr, int what, hipStream_t s)
 {
     hipLaunchKernelGGL(k_metropolis, dim3(v.nchains), dim3(256), 0, s, v, uniforms, ustride, therm, ufirst, rec, rec_base,
-                       surr);
+                       surr, what);
     return hipGetLastError();
 }
 

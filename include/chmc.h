@@ -51,7 +51,10 @@ typedef struct chmc_chain_desc {
 enum {
     CHMC_FORCE_FP32 = 0,       /* MD forces: FP32 pairs, FP32 per-atom sums (default) */
     CHMC_FORCE_FP64 = 1 << 0,  /* MD forces in FP64 (the reference's arithmetic) */
-    CHMC_ORDER_ATOM_ID = 1 << 1/* in-cell order by atom id instead of the reference's stable order */
+    CHMC_ORDER_ATOM_ID = 1 << 1,/* in-cell order by atom id instead of the reference's stable order */
+    CHMC_CELL_MD = 1 << 2      /* FP32 MD of chmc_integrate / chmc_hmc_run through the 27-cell sweep with
+                                  a re-binning per force evaluation (the reference's cost model) instead
+                                  of the skin neighbour lists */
 };
 
 /* integrator schemes: the numbering of code-carsten/integrator.hh:14 (set_integrator) */
@@ -201,6 +204,21 @@ int chmc_set_force_tiling(chmc_ctx* ctx, int tiling);
 int chmc_get_force_tiling(const chmc_ctx* ctx, int* tiling);
 /* average device time (ms) of one force evaluation (binning + FP32 sweep + kick) over reps */
 int chmc_time_force(chmc_ctx* ctx, int reps, double* ms_bin, double* ms_force);
+
+/* Neighbour-list MD (FP32 contexts without CHMC_CELL_MD): chmc_integrate and chmc_hmc_run keep per-atom
+ * lists of the pairs within cutoff + skin and rebuild them (on the device, inside the captured
+ * trajectory) only when an atom has moved more than skin / 2 since the last rebuild, so every pair
+ * within the cutoff is always listed; each integration / trajectory starts from a fresh rebuild.
+ * No reference counterpart (the reference re-bins on every evaluation, identical_particles.cpp:379,560).
+ * set_skin reallocates the lists (default 0.3 sigma-units of length).  list_stats: list entries and
+ * interacting pairs (within the cutoff) of chain `chain` at the current positions (rebuilds first),
+ * and the number of rebuilds run since the context was created. */
+int chmc_set_skin(chmc_ctx* ctx, double skin);
+int chmc_get_skin(const chmc_ctx* ctx, double* skin);
+int chmc_list_stats(chmc_ctx* ctx, int chain, long long* entries, long long* nint, long long* rebuilds);
+/* average device time (ms) of the list sweep with kick + drift (the MD step of the trajectory) and of
+ * one rebuild, over reps, at the current positions (the positions do not change: dt = 0) */
+int chmc_time_list(chmc_ctx* ctx, int reps, double* ms_sweep, double* ms_rebuild);
 
 #ifdef __cplusplus
 }

@@ -228,6 +228,45 @@ def test_integrator_fp32_config2(chmc, orc):
     assert np.max(dx) < 2e-4, np.max(dx)
 
 
+@pytest.mark.parametrize("N,rho,Lb", [(32768, 0.8442, None), (125, 0.58, None), (51, 0.8, None),
+                                      (100, None, (8.0, 6.0, 4.5))])
+def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
+    """Neighbour-list MD (skin lists, rebuilt on displacement) against the 27-cell sweep with a
+    re-binning per evaluation and against the FP64 oracle: 10 OMF2 steps from the same start.  Both FP32
+    paths differ from the oracle only by FP32 rounding (same bars as the FP32 integrator test); the
+    small boxes put one or two list cells along a side, where the lists hold several images of a pair
+    (the anisotropic box holds 100 atoms: its 300-atom lattice of the force tests, at rho = 1.39 with
+    0.64 sigma spacing, blows up under OMF2 dt = 0.01 in any arithmetic)."""
+    T = 0.722 if N == 32768 else 1.0
+    e, L = _eng(chmc, N, rho, Lb, T=T)
+    ec, _ = _eng(chmc, N, rho, Lb, T=T, cell_md=True)
+    e.heatbath(9, 1)
+    ec.set_momenta(e.momenta())
+    x, p = e.positions(), e.momenta()
+    s = orc.Sim(N, np.broadcast_to(np.asarray(L, float), (3,)).tolist(), cutoff=RC, beta=1 / T)
+    s.x[:] = x
+    s.p[:] = p
+    e.integrate(6, 0.01, 10, 0.1932)
+    ec.integrate(6, 0.01, 10, 0.1932)
+    s.integrate(6, 0.01, 10, 0.1932)
+    entries, nint, rebuilds = e.list_stats()
+    print(f"list entries/atom {entries / N:.1f}, interacting/atom {nint / N:.1f}, rebuilds so far {rebuilds}")
+    for name, eng in (("list", e), ("cell", ec)):
+        dx = np.abs(min_image(eng.positions() - s.x, L))
+        print(f"{name}: rms {np.sqrt(np.mean(dx ** 2)):.2e} max {dx.max():.2e}")
+        assert np.sqrt(np.mean(dx ** 2)) < 1e-6
+        assert dx.max() < 2e-4
+    # interacting pairs over the lists equal the cell count (FP64 r < rc on both sides)
+    assert nint == e.pair_counts(0)[1]
+    # the stored force of the list path equals the cell path's at the same positions
+    e.integrate(6, 0.0, 1, 0.1932)
+    ec.set_positions(e.positions())
+    ec.compute_force()
+    fl, fc = e.force(), ec.force()
+    frms = np.sqrt(np.mean(fc ** 2))
+    assert np.max(np.abs(fl - fc)) < 5e-5 * frms
+
+
 @pytest.mark.parametrize("N,rho,nb", [(512, 0.8, 200), (4000, 0.8442, 500)])
 def test_rdf_counts_bitexact(chmc, orc, N, rho, nb):
     e, L = make(chmc, N, rho)
