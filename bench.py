@@ -85,6 +85,8 @@ def main():
         skins[sk] = (time.perf_counter() - t0) / args.tune * 1e3
     skin = min(skins, key=skins.get)
     e.set_skin(skin)
+    e.set_list_lanes(0)                         # lanes per atom of the list sweep (autotuned)
+    lanes = e.list_lanes()
     _, _, rebuilds0 = e.list_stats()
 
     t0 = time.perf_counter()
@@ -113,7 +115,7 @@ def main():
         "dtype": "f32 pair forces, f64 x/p/dH", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": "plain HMC N=32768 rho=0.8442 T=0.722 rc=2.5 OMF2(lambda=0.1932) 10x0.01",
                    "N": N, "trajectories": args.steps, "md_steps_per_traj": NSTEPS, "force_evals_per_traj": 2 * NSTEPS + 1,
-                   "md_path": "neighbour lists", "skin": skin, "ms_per_traj_by_skin": skins},
+                   "md_path": "neighbour lists", "skin": skin, "list_lanes": lanes, "ms_per_traj_by_skin": skins},
         "acceptance": float(nacc[0]) / args.steps,
         "u_per_n": float(u.mean()), "tau_int_u": tau, "ess_per_s": ess / el,
         "list_rebuilds_per_traj": rebuilds_per_traj,

@@ -35,7 +35,7 @@ class ChainDesc(C.Structure):
 class HmcParams(C.Structure):
     _fields_ = [("ntraj", _I), ("therm", _I), ("scheme", _I), ("steps", _I), ("dt", _D), ("lam", _D),
                 ("seed", _U64), ("use_graph", _I), ("randomize", _I), ("surrogate", _I),
-                ("fallback_window", _I), ("fallback_acc", _D)]
+                ("fallback_window", _I), ("fallback_acc", _D), ("uniforms", C.c_void_p), ("traj_steps", C.c_void_p)]
 
 
 HMC_RECORD_DTYPE = np.dtype([("dh", "f8"), ("u", "f8"), ("k", "f8"), ("w", "f8"), ("accepted", "i4"),
@@ -106,6 +106,8 @@ def _declare(L):
         "chmc_get_skin": ([P, C.POINTER(_D)], _I),
         "chmc_list_stats": ([P, _I, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], _I),
         "chmc_time_list": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
+        "chmc_set_list_lanes": ([P, _I], _I),
+        "chmc_get_list_lanes": ([P, C.POINTER(_I)], _I),
         "chmc_set_trajectory": ([P, _I], _I),
         "chmc_get_trajectory": ([P, C.POINTER(_I)], _I),
         "chmc_surrogate_set_params": ([P, C.c_void_p, _I], _I),
@@ -132,7 +134,7 @@ EXPORTED = (
     "chmc_kick", "chmc_integrate", "chmc_heatbath", "chmc_save_positions", "chmc_restore_positions",
     "chmc_hmc_run", "chmc_rdf", "chmc_potential_energy_open", "chmc_set_force_tiling",
     "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_list_stats",
-    "chmc_time_list", "chmc_set_trajectory", "chmc_get_trajectory",
+    "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_trajectory", "chmc_get_trajectory",
     "chmc_surrogate_set_params", "chmc_surrogate_get_params",
     "chmc_surrogate_eval", "chmc_surrogate_features", "chmc_surrogate_reserve", "chmc_surrogate_collect",
     "chmc_surrogate_training_data", "chmc_surrogate_loss_grad", "chmc_surrogate_train",
@@ -409,6 +411,14 @@ class Engine:
         a, b, r = C.c_longlong(), C.c_longlong(), C.c_longlong()
         self._ck(lib().chmc_list_stats(self._h, int(chain), C.byref(a), C.byref(b), C.byref(r)), "list_stats")
         return a.value, b.value, r.value
+
+    def set_list_lanes(self, lanes=0):
+        self._ck(lib().chmc_set_list_lanes(self._h, int(lanes)), "set_list_lanes")
+
+    def list_lanes(self):
+        v = _I()
+        self._ck(lib().chmc_get_list_lanes(self._h, C.byref(v)), "get_list_lanes")
+        return v.value
 
     def time_list(self, reps=20):
         a, b = _D(), _D()

@@ -761,6 +761,7 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, hipDeviceSynchronize());
     c->skin = skin;
     c->lpar = 0;
+    if (v.lanes == 0) v.lanes = 8;
     c->list_ready = true;
     return CHMC_OK;
 }
@@ -1132,17 +1133,25 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     for (int k = 0; k < nch; k++) gens.emplace_back(hp->seed + (uint64_t)k);
     std::vector<int> rsteps;
     std::vector<double> u((size_t)nch * ntraj, 2.0);
-    if (!hp->randomize) {
+    if (hp->randomize && nch != 1) return fail(c, CHMC_E_ARG, "randomize_trajectory needs a single-chain context");
+    if (hp->uniforms) {
+        std::copy(hp->uniforms, hp->uniforms + (size_t)nch * ntraj, u.begin());
+        if (hp->randomize) {
+            if (!hp->traj_steps) return fail(c, CHMC_E_ARG, "hmc_run: caller uniforms with randomize need traj_steps");
+            rsteps.assign(hp->traj_steps, hp->traj_steps + ntraj);
+        }
+    } else if (!hp->randomize) {
         for (int k = 0; k < nch; k++)
             for (int t = 0; t < ntraj; t++) u[(size_t)k * ntraj + t] = gens[k].uniform();
     } else {
-        if (nch != 1) return fail(c, CHMC_E_ARG, "randomize_trajectory needs a single-chain context");
         int q = 0;
         for (int t = 0; t < ntraj; t++) {
-            rsteps.push_back(gens[0].binomial(hp->steps));
+            rsteps.push_back(hp->traj_steps ? hp->traj_steps[t] : gens[0].binomial(hp->steps));
             if (rec_base + t + 1 >= hp->therm) u[q++] = gens[0].uniform();
         }
     }
+    for (int st : rsteps)
+        if (st < 0) return fail(c, CHMC_E_ARG, "hmc_run: negative trajectory length");
     if (c->uniforms_cap < u.size()) {
         if (c->d_uniforms) hipFree(c->d_uniforms);
         c->d_uniforms = nullptr;
@@ -1622,6 +1631,38 @@ extern "C" int chmc_list_stats(chmc_ctx* c, int chain, long long* entries, long 
     if (nint) *nint = (long long)std::llround(nintv[chain]);
     if (rebuilds) *rebuilds = f[3];
     return f[2] ? fail(c, CHMC_E_STATE, "neighbour list capacity exceeded") : CHMC_OK;
+}
+
+extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* ms_rebuild);
+
+extern "C" int chmc_set_list_lanes(chmc_ctx* c, int lanes)
+{
+    CHECK_CTX(c);
+    if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16)
+        return fail(c, CHMC_E_ARG, "set_list_lanes: 0 (autotune), 4, 8 or 16");
+    int rc = list_ensure(c);
+    if (rc) return rc;
+    if (lanes) {
+        c->v.lanes = lanes;
+        return CHMC_OK;
+    }
+    double best = 1e30;
+    int pick = 8;
+    for (int lp : { 4, 8, 16 }) {
+        c->v.lanes = lp;
+        double ms = 0.0;
+        if ((rc = chmc_time_list(c, 20, &ms, nullptr))) return rc;
+        if (ms < best) { best = ms; pick = lp; }
+    }
+    c->v.lanes = pick;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_get_list_lanes(const chmc_ctx* c, int* lanes)
+{
+    if (!c || !lanes) return CHMC_E_ARG;
+    *lanes = c->v.lanes ? c->v.lanes : 8;
+    return CHMC_OK;
 }
 
 extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* ms_rebuild)

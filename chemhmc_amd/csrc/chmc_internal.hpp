@@ -94,7 +94,7 @@ struct DevView {
     ChainState* state;    // [nchains]
     int* traj;            // [1] device trajectory counter (1-based index of the last finished one)
     // neighbour-list MD
-    int Cl, ntiles, nbcap, lpad_;
+    int Cl, ntiles, nbcap, lanes;   // lanes: list-sweep lanes per slot (4, 8, 16)
     const int* lcell_chain;
     const Chunk* tiles;
     int *lcount, *loffs, *lmem, *lperm, *linv, *nb, *nbn, *lflag, *lreq, *lscan_tmp;

@@ -16,10 +16,11 @@
 //             staging the 27 neighbour cells (image shift dd * lsize folded in) the way k_lj does, with
 //             each neighbour's slot recomputed from its sorted rank, so no workgroup reads data another
 //             writes in the same launch
-// Sweep: 32 slots x 8 lanes per workgroup; each lane walks every 8th list entry, gathers the neighbour's
-//   FP32 coordinates, and the 8 partial sums meet in a butterfly.  Lane 0 of an atom kicks p in FP64,
-//   optionally drifts x in FP64 (identical_particles.cpp:651-673), writes the FP32 coordinates of the
-//   other parity and raises the rebuild request when the atom is beyond skin / 2 from x0.
+// Sweep: 32 slots x LP lanes per workgroup (LP = 4, 8 or 16, tuned); each lane walks every LP-th list
+//   entry, gathers the neighbour's FP32 coordinates, and the partial sums meet in a butterfly.  Three
+//   lanes of an atom then kick p in FP64, optionally drift x in FP64 (identical_particles.cpp:651-673),
+//   and lane 0 writes the FP32 coordinates of the other parity and raises the rebuild request when the
+//   atom is beyond skin / 2 from x0.
 // Energy: FP64 coordinates relative to the rebuild cell origins (rel64 = r0 + wrap(x - x0)), the
 //   potential's r < rc test (identical_particles.cpp:432-435), per-tile fixed-order partials.
 #include "chmc_internal.hpp"
@@ -308,8 +309,13 @@ __device__ __forceinline__ void load_shifts(float4* sh, const ChainDev& C)
                             (float)(t / 9 - 1) * C.lsize_f[2], 0.f);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q)
+// LP lanes per slot, kTile slots per workgroup.  Each lane first loads up to kBatch of its list entries
+// and gathers their coordinates, so that one memory latency covers most lists; the per-atom tail (kick,
+// drift, next-parity coordinates, displacement test) runs on three lanes, one per dimension.
+constexpr int kBatch = 8;
+
+template <int MODE, int LP>
+__global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q)
 {
     __shared__ float4 sh[27];
     const Chunk tl = v.tiles[blockIdx.x];
@@ -318,55 +324,87 @@ __global__ __launch_bounds__(256) void k_lsweep(DevView v, double kick_dt, doubl
     if (tl.begin == C.atom_off && threadIdx.x == 0) *req(v, q, tl.chain) = 0;   // this evaluation's rebuild is done
     __syncthreads();
     const int N = v.Ntot;
-    const int t = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    const int t = threadIdx.x / LP, sub = threadIdx.x % LP;
     const int s = tl.begin + t;
     const bool val = s < tl.end;
     const float4* __restrict__ xq = v.xsl + (size_t)q * N;
     const float rc2 = C.rc2_f, sig2 = C.sigma2_f;
     float ax = 0.f, ay = 0.f, az = 0.f;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto pair = [&](const float4 pj, int code) {
+        const float4 shf = sh[code];
+        const float dx = xi.x - (pj.x + shf.x), dy = xi.y - (pj.y + shf.y), dz = xi.z - (pj.z + shf.z);
+        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const float inv = __builtin_amdgcn_rcpf(r2);
+        const float s2 = sig2 * inv;
+        const float s6 = s2 * s2 * s2;
+        float gq = s6 * (0.5f - s6) * inv;            // sr6 (0.5 - sr6) / r^2, identical_particles.cpp:611-613
+        gq = (r2 < rc2 && r2 > 0.f) ? gq : 0.f;
+        ax = __builtin_fmaf(gq, dx, ax);
+        ay = __builtin_fmaf(gq, dy, ay);
+        az = __builtin_fmaf(gq, dz, az);
+    };
     if (val) {
         xi = xq[s];
         const int n = v.nbn[s];
         const int* __restrict__ nbs = v.nb + (size_t)s * v.nbcap;
-#pragma unroll 4
-        for (int k = sub; k < n; k += 8) {
-            const int e = nbs[k];
-            const float4 pj = xq[e >> 5];
-            const float4 shf = sh[e & 31];
-            const float dx = xi.x - (pj.x + shf.x), dy = xi.y - (pj.y + shf.y), dz = xi.z - (pj.z + shf.z);
-            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-            const float inv = __builtin_amdgcn_rcpf(r2);
-            const float s2 = sig2 * inv;
-            const float s6 = s2 * s2 * s2;
-            float gq = s6 * (0.5f - s6) * inv;            // sr6 (0.5 - sr6) / r^2, identical_particles.cpp:611-613
-            gq = (r2 < rc2 && r2 > 0.f) ? gq : 0.f;
-            ax = __builtin_fmaf(gq, dx, ax);
-            ay = __builtin_fmaf(gq, dy, ay);
-            az = __builtin_fmaf(gq, dz, az);
+        int e[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int k = sub + u * LP;
+            e[u] = k < n ? nbs[k] : 0;
+        }
+        float4 pj[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) pj[u] = xq[e[u] >> 5];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+            if (sub + u * LP < n) pair(pj[u], e[u] & 31);
+        for (int k = sub + kBatch * LP; k < n; k += LP) {
+            const int ee = nbs[k];
+            pair(xq[ee >> 5], ee & 31);
         }
     }
 #pragma unroll
-    for (int m = 1; m < 8; m <<= 1) {
+    for (int m = 1; m < LP; m <<= 1) {
         ax += __shfl_xor(ax, m, 64);
         ay += __shfl_xor(ay, m, 64);
         az += __shfl_xor(az, m, 64);
     }
-    if (!val || sub != 0) return;
+    // per-dimension tail on lanes sub = 0, 1, 2 of the slot
+    const int d = sub < 3 ? sub : 0;
     const int atom = __float_as_int(xi.w);
-    const float f3[3] = { C.eps48_f * ax, C.eps48_f * ay, C.eps48_f * az };
-    const double kdt = kick_dt * C.beta;               // dt * coeff_p, coeff_p = beta (:642-644)
-    double pn[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        if constexpr ((MODE & LJ_STOREF) != 0) v.f32[d * N + atom] = f3[d];
-        pn[d] = v.p[d * N + atom];
+    double e = 0.0;
+    float rel = 0.f;
+    if (val && sub < 3) {
+        const float fd = C.eps48_f * (d == 0 ? ax : (d == 1 ? ay : az));
+        if constexpr ((MODE & LJ_STOREF) != 0) v.f32[d * N + atom] = fd;
+        double pn = v.p[d * N + atom];
         if constexpr ((MODE & LJ_KICK) != 0) {
-            pn[d] = pn[d] - kdt * (double)f3[d];
-            v.p[d * N + atom] = pn[d];
+            const double kdt = kick_dt * C.beta;       // dt * coeff_p, coeff_p = beta (:642-644)
+            pn = pn - kdt * (double)fd;
+            v.p[d * N + atom] = pn;
+        }
+        if constexpr ((MODE & LJ_DRIFT) != 0) {
+            const double coeff_x = C.beta / C.mass;    // compute_coeff_position, identical_particles.cpp:646-648
+            double xv = v.x[d * N + atom];
+            xv += drift_dt * coeff_x * pn;
+            xv -= C.L[d] * floor(xv / C.L[d]);
+            v.x[d * N + atom] = xv;
+            e = xv - v.x0[d * N + s];
+            e -= C.L[d] * rint(e / C.L[d]);
+            rel = (float)(v.r0[d * N + s] + e);
         }
     }
-    if constexpr ((MODE & LJ_DRIFT) != 0) list_drift_atom(v, C, atom, s, q, drift_dt, pn);
+    if constexpr ((MODE & LJ_DRIFT) != 0) {
+        const int base = threadIdx.x & ~(LP - 1) & 63;
+        const float ry = __shfl(rel, base + 1, 64), rz = __shfl(rel, base + 2, 64);
+        const double ey = __shfl(e, base + 1, 64), ez = __shfl(e, base + 2, 64);
+        if (val && sub == 0) {
+            v.xsl[(size_t)(q ^ 1) * N + s] = make_float4(rel, ry, rz, __int_as_float(atom));
+            if (e * e + ey * ey + ez * ez > C.half_skin2) atomicOr(req(v, q ^ 1, tl.chain), 1);
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_ldrift(DevView v, double dt, int q)
@@ -460,22 +498,30 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
 }
 
 // ---- launchers -----------------------------------------------------------------------------------
-hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, hipStream_t s)
+template <int MODE>
+static hipError_t sweep_lp(const DevView& v, int lp, int q, double kick_dt, double drift_dt, hipStream_t s)
 {
-    const dim3 g(v.ntiles), b(256);
-    switch (mode) {
-    case LJ_FORCE | LJ_KICK: hipLaunchKernelGGL(k_lsweep<LJ_FORCE | LJ_KICK>, g, b, 0, s, v, kick_dt, drift_dt, q); break;
-    case LJ_FORCE | LJ_KICK | LJ_STOREF:
-        hipLaunchKernelGGL(k_lsweep<LJ_FORCE | LJ_KICK | LJ_STOREF>, g, b, 0, s, v, kick_dt, drift_dt, q); break;
-    case LJ_FORCE | LJ_KICK | LJ_DRIFT:
-        hipLaunchKernelGGL(k_lsweep<LJ_FORCE | LJ_KICK | LJ_DRIFT>, g, b, 0, s, v, kick_dt, drift_dt, q); break;
-    case LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT:
-        hipLaunchKernelGGL(k_lsweep<LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT>, g, b, 0, s, v, kick_dt, drift_dt, q); break;
-    case LJ_FORCE | LJ_STOREF:
-        hipLaunchKernelGGL(k_lsweep<LJ_FORCE | LJ_STOREF>, g, b, 0, s, v, kick_dt, drift_dt, q); break;
+    switch (lp) {
+    case 4: hipLaunchKernelGGL((k_lsweep<MODE, 4>), dim3(v.ntiles), dim3(kTile * 4), 0, s, v, kick_dt, drift_dt, q); break;
+    case 8: hipLaunchKernelGGL((k_lsweep<MODE, 8>), dim3(v.ntiles), dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q); break;
+    case 16: hipLaunchKernelGGL((k_lsweep<MODE, 16>), dim3(v.ntiles), dim3(kTile * 16), 0, s, v, kick_dt, drift_dt, q); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, hipStream_t s)
+{
+    const int lp = v.lanes;
+    switch (mode) {
+    case LJ_FORCE | LJ_KICK: return sweep_lp<LJ_FORCE | LJ_KICK>(v, lp, q, kick_dt, drift_dt, s);
+    case LJ_FORCE | LJ_KICK | LJ_STOREF: return sweep_lp<LJ_FORCE | LJ_KICK | LJ_STOREF>(v, lp, q, kick_dt, drift_dt, s);
+    case LJ_FORCE | LJ_KICK | LJ_DRIFT: return sweep_lp<LJ_FORCE | LJ_KICK | LJ_DRIFT>(v, lp, q, kick_dt, drift_dt, s);
+    case LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT:
+        return sweep_lp<LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT>(v, lp, q, kick_dt, drift_dt, s);
+    case LJ_FORCE | LJ_STOREF: return sweep_lp<LJ_FORCE | LJ_STOREF>(v, lp, q, kick_dt, drift_dt, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_list_drift(const DevView& v, int q, double dt, hipStream_t s)

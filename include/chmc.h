@@ -146,6 +146,13 @@ typedef struct chmc_hmc_params {
     int surrogate;
     int fallback_window;
     double fallback_acc;
+    /* optional caller-drawn randomness (NULL: drawn inside from mt19937_64(seed + chain)):
+     * uniforms[chain * ntraj + q] is the Metropolis uniform of the chain's q-th post-thermalisation
+     * trajectory of this call (gen_random, HMC.cpp:48-50); traj_steps[t] the MD steps of trajectory t
+     * when randomize is set (set_binomial_steps, integrator.cpp:37-49).  A caller that keeps one host
+     * mt19937_64 across calls (and saves it, HMC.cpp:52-59) reproduces the reference's single stream. */
+    const double* uniforms;
+    const int* traj_steps;
 } chmc_hmc_params;
 
 /* per trajectory and chain, record index = traj * nchains + chain */
@@ -219,6 +226,10 @@ int chmc_list_stats(chmc_ctx* ctx, int chain, long long* entries, long long* nin
 /* average device time (ms) of the list sweep with kick + drift (the MD step of the trajectory) and of
  * one rebuild, over reps, at the current positions (the positions do not change: dt = 0) */
 int chmc_time_list(chmc_ctx* ctx, int reps, double* ms_sweep, double* ms_rebuild);
+/* lanes per atom of the list sweep: 4, 8 or 16; 0 runs the autotuner (times each at the current
+ * positions and keeps the fastest) */
+int chmc_set_list_lanes(chmc_ctx* ctx, int lanes);
+int chmc_get_list_lanes(const chmc_ctx* ctx, int* lanes);
 
 #ifdef __cplusplus
 }

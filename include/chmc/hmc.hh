@@ -275,11 +275,25 @@ public:
         }
     }
 
-    // the same chain with every stage on the device (hipGraph-captured trajectory)
+    // the same chain with every stage on the device (hipGraph-captured trajectory); the Metropolis
+    // uniforms and binomial trajectory lengths come from this object's gen64 in the order run() draws
+    // them, so run() and run_device() consume one stream identically
     std::vector<chmc_hmc_record> run_device(bool use_graph = true)
     {
         identical_particles* P = integrator->particles;
+        std::vector<double> u(params.Ntrajectories, 2.0);
+        std::vector<int> st;
+        int q = 0;
+        for (int t = 0; t < params.Ntrajectories; t++) {
+            if (params.randomize_traj) {
+                integrator->set_binomial_steps(gen64);
+                st.push_back(integrator->steps);
+            }
+            if (params.istart + 1 + t >= params.thermalization_steps) u[q++] = gen_random();
+        }
         chmc_hmc_params hp{};
+        hp.uniforms = u.data();
+        hp.traj_steps = params.randomize_traj ? st.data() : nullptr;
         hp.ntraj = params.Ntrajectories;
         hp.therm = params.thermalization_steps;
         hp.scheme = integrator->scheme();
