@@ -53,6 +53,54 @@ def cpu_baseline(seconds_budget=20.0):
             "sample": f"{ntraj} OMF2 trajectories of the N=32768 workload (FP64 oracle, O(N) binning), {el:.1f} s"}
 
 
+def other_workload(args):
+    """--workload c3 (256 batched chains, LEAP 15 x 0.005) or c5 (N = 1,048,576, OMF2 10 x 0.01): the
+    same measurement on those configurations, one JSON line."""
+    import chemhmc_amd as chmc
+    from chemhmc_amd import configs, stats
+
+    w = configs.WORKLOADS[args.workload]()
+    e = chmc.Engine(w.chains)
+    e.init_lattice(1, 0.05)
+    nat = w.natoms
+    steps = args.steps if args.steps_given else (200 if args.workload == "c3" else 50)
+    warm = args.warmup if args.warmup_given else 30
+    t0 = time.perf_counter()
+    e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=True)
+    t_warm = time.perf_counter() - t0
+    e.set_list_lanes(0)
+    t0 = time.perf_counter()
+    rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=True)
+    el = time.perf_counter() - t0
+    value = nat * w.steps * steps / el
+    ent = nint = 0
+    for k in range(len(w.chains)):
+        a, b, _ = e.list_stats(k)
+        ent += a
+        nint += b
+    ms_sweep, ms_rebuild = e.time_list(reps=20)
+    flops = 8.0 * ent + 13.0 * nint
+    achieved = flops / (ms_sweep * 1e-3) / 1e12
+    u = rec["u"] / np.array([c.N for c in w.chains])[None, :]
+    ess = sum(steps / (2 * stats.tau_int(u[:, k])[0]) for k in range(len(w.chains)))
+    out = {
+        "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N", "value": value, "unit": "atom-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": el / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair forces, f64 x/p/dH",
+        "data": "synthetic (seeded lattice + jitter, Philox momenta)",
+        "config": {"workload": f"{args.workload}: {len(w.chains)} chain(s), {nat} atoms, "
+                               f"{'LEAP' if w.scheme == chmc.LEAPFROG else 'OMF2'} {w.steps}x{w.dt}",
+                   "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "skin": e.skin(),
+                   "warmup_seconds": t_warm},
+        "acceptance": float(nacc.sum()) / (steps * len(w.chains)), "ess_per_s_sum_over_chains": ess / el,
+        "roofline": {"bound": "valu", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops, "ms_per_launch": ms_sweep, "ms_rebuild": ms_rebuild,
+                     "list_entries_per_atom": ent / nat, "n_int_per_atom": nint / nat},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,11 +109,19 @@ def main():
     ap.add_argument("--tune", type=int, default=40, help="trajectories per skin in the skin autotuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"],
+                    help="c2 (default, the headline line), c3 batched chains, c5 N=1M")
     args = ap.parse_args()
+    args.steps_given = "--steps" in sys.argv
+    args.warmup_given = "--warmup" in sys.argv
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.workload != "c2":
+        other_workload(args)
+        return
 
     import chemhmc_amd as chmc
     from chemhmc_amd import stats

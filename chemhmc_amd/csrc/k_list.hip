@@ -130,6 +130,7 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
     __shared__ unsigned char scode[kLMaxJ];
     __shared__ float4 si[kLMaxI];          // own atoms by rank: FP32 r0, slot in w
     __shared__ int icount[kLMaxI];
+    __shared__ int sid[kLMaxJ];            // member atom ids of the 27 cells (when they fit)
     __shared__ int seg_beg[27], seg_pref[28], seg_slot0[27], seg_cnt[27];
     __shared__ float seg_sh[27][3];
     __shared__ double seg_org[27][3];
@@ -177,10 +178,34 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
         if (lane < 27) seg_pref[lane + 1] = incl;
         if (lane == 0) seg_pref[0] = 0;
     }
-    // own atoms: compact slot by rank, rebuild reference data, FP32 coordinates of this parity
+    __syncthreads();
+    const int nj = seg_pref[27];
+    // members of the 27 cells in LDS when they fit (ranks then come from LDS), else from global memory
+    const bool lds_ids = nj <= kLMaxJ;
+    if (lds_ids) {
+        for (int t = tid; t < nj; t += NT) {
+            int k = 0;
+#pragma unroll
+            for (int step = 16; step > 0; step >>= 1)
+                if (k + step < 27 && seg_pref[k + step] <= t) k += step;
+            sid[t] = mem[seg_beg[k] + (t - seg_pref[k])];
+        }
+        __syncthreads();
+    }
+    auto rank_in = [&](int k, int a) -> int {
+        if (lds_ids) {
+            const int b = seg_pref[k], n = seg_cnt[k];
+            int r = 0;
+            for (int m = 0; m < n; m++) r += sid[b + m] < a ? 1 : 0;
+            return r;
+        }
+        return member_rank(mem, seg_beg[k], seg_cnt[k], a);
+    };
+    // own atoms (segment 13, dd = 0): compact slot by rank, rebuild reference data, FP32 coordinates
+    // of this parity
     for (int m = tid; m < ni; m += NT) {
-        const int a = mem[ibase + m];
-        const int r = member_rank(mem, ibase, ni, a);
+        const int a = lds_ids ? sid[seg_pref[13] + m] : mem[ibase + m];
+        const int r = rank_in(13, a);
         const int slot = ibeg + r;
         const double o3[3] = { (double)bx * C.lsize[0], (double)by * C.lsize[1], (double)bz * C.lsize[2] };
         float f3[3];
@@ -200,7 +225,6 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
         icount[r] = 0;
     }
     __syncthreads();
-    const int nj = seg_pref[27];
     const float rl2 = C.rl2_f;
     const int ii = lane % IS, g = lane / IS;
     unsigned long long pattern = 0;
@@ -218,8 +242,8 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
 #pragma unroll
             for (int step = 16; step > 0; step >>= 1)
                 if (k + step < 27 && seg_pref[k + step] <= t) k += step;
-            const int a = mem[seg_beg[k] + (t - seg_pref[k])];
-            const int r = member_rank(mem, seg_beg[k], seg_cnt[k], a);
+            const int a = lds_ids ? sid[t] : mem[seg_beg[k] + (t - seg_pref[k])];
+            const int r = rank_in(k, a);
             const int dst = seg_pref[k] + r;
             if (dst < jlo || dst >= jhi) continue;
             float f3[3];
