@@ -34,8 +34,21 @@ PEAK_FP32_TFLOPS = 157.3        # MI355X vector FP32 (vendor figure; packed FP32
 PEAK_HBM_GBS = 8000.0
 
 
+def _omp_threads(n):
+    """Set the OpenMP team size of the oracle library (libgomp) for the next parallel regions."""
+    import ctypes
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+        return True
+    except OSError:
+        return False
+
+
 def cpu_baseline(seconds_budget=20.0):
-    """FP64 oracle (OpenMP) on the host: OMF2 trajectories of the same configuration until ~budget."""
+    """FP64 oracle (OpenMP) on the host, on the N=32768 workload: OMF2 trajectories with the O(N)
+    counting-sort binning on all cores until ~budget (the reported value), one trajectory on one core,
+    and one trajectory with the reference's O(N * cells) count/set binning (its cost model) on all
+    cores."""
     import oracle
     L = (N / RHO) ** (1 / 3)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -49,8 +62,22 @@ def cpu_baseline(seconds_budget=20.0):
         el = time.perf_counter() - t0
         if el > seconds_budget or ntraj >= 50:
             break
-    return {"value": N * NSTEPS * ntraj / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{ntraj} OMF2 trajectories of the N=32768 workload (FP64 oracle, O(N) binning), {el:.1f} s"}
+    out = {"value": N * NSTEPS * ntraj / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+           "sample": f"{ntraj} OMF2 trajectories of the N=32768 workload (FP64 oracle, O(N) binning), {el:.1f} s"}
+    if _omp_threads(1):
+        t0 = time.perf_counter()
+        s.hmc(1, 0, oracle.OMF2, DT, NSTEPS, lam=LAM, seed=999)
+        el1 = time.perf_counter() - t0
+        out["one_core"] = {"value": N * NSTEPS / el1, "sample": f"1 trajectory on 1 core, {el1:.1f} s"}
+        _omp_threads(threads)
+    r = oracle.Sim(N, [L] * 3, cutoff=RC, beta=1 / T, refalgo=True)
+    r.x[:] = s.x
+    t0 = time.perf_counter()
+    r.hmc(1, 0, oracle.OMF2, DT, NSTEPS, lam=LAM, seed=998)
+    elr = time.perf_counter() - t0
+    out["ref_algo"] = {"value": N * NSTEPS / elr, "cores": threads,
+                       "sample": f"1 trajectory with the reference's O(N*cells) binning (22 binnings), {elr:.1f} s"}
+    return out
 
 
 def other_workload(args):
@@ -66,11 +93,11 @@ def other_workload(args):
     steps = args.steps if args.steps_given else (200 if args.workload == "c3" else 50)
     warm = args.warmup if args.warmup_given else 30
     t0 = time.perf_counter()
-    e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=True)
+    e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
     t_warm = time.perf_counter() - t0
     e.set_list_lanes(0)
     t0 = time.perf_counter()
-    rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=True)
+    rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=not args.no_graph)
     el = time.perf_counter() - t0
     value = nat * w.steps * steps / el
     ent = nint = 0
@@ -109,6 +136,8 @@ def main():
     ap.add_argument("--tune", type=int, default=40, help="trajectories per skin in the skin autotuner")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every trajectory directly instead of replaying the captured graph")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"],
                     help="c2 (default, the headline line), c3 batched chains, c5 N=1M")
     args = ap.parse_args()
@@ -130,14 +159,14 @@ def main():
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
     e.init_lattice(1, 0.05)
     if args.warmup > 0:     # accept-all thermalisation away from the lattice (thermalization_steps)
-        e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=True)
+        e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
     # skin autotuner: wall time of a short run per skin on the equilibrated state
     skins = {}
     for k, sk in enumerate((0.2, 0.3, 0.4, 0.5)):
         e.set_skin(sk)
-        e.hmc(4, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=20 + k, use_graph=True)
+        e.hmc(4, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=20 + k, use_graph=not args.no_graph)
         t0 = time.perf_counter()
-        e.hmc(args.tune, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=30 + k, use_graph=True)
+        e.hmc(args.tune, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=30 + k, use_graph=not args.no_graph)
         skins[sk] = (time.perf_counter() - t0) / args.tune * 1e3
     skin = min(skins, key=skins.get)
     e.set_skin(skin)
@@ -146,7 +175,7 @@ def main():
     _, _, rebuilds0 = e.list_stats()
 
     t0 = time.perf_counter()
-    rec, nacc = e.hmc(args.steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=True)
+    rec, nacc = e.hmc(args.steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph)
     el = time.perf_counter() - t0
     ms = el / args.steps * 1e3
     value = N * NSTEPS * args.steps / el

@@ -58,6 +58,7 @@ struct chmc_ctx {
     double skin = 0.3;
     bool list_ready = false;
     int lpar = 0;                // list parity: which FP32 coordinate buffer / rebuild flag is current
+    bool list_valid = false;     // the lists and their rebuild flags account for every position change
     std::vector<void*> lallocs;
     std::vector<Chunk> htiles;
 };
@@ -316,6 +317,7 @@ static int flush_drift(chmc_ctx* c);
 extern "C" int chmc_set_positions(chmc_ctx* c, int chain, const double* xyz)
 {
     if (!c || !xyz || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    c->list_valid = false;             // positions change outside the lists
     const ChainDev& cd = c->hchains[chain];
     // check_in_volume (identical_particles.cpp:228-236): every coordinate in [0, L)
     for (int i = 0; i < cd.natoms; i++)
@@ -366,6 +368,7 @@ static int lattice_side(int N)
 extern "C" int chmc_init_lattice(chmc_ctx* c, uint64_t seed, double jitter)
 {
     CHECK_CTX(c);
+    c->list_valid = false;             // positions change outside the lists
     if (!(jitter >= 0.0)) return CHMC_E_ARG;
     for (int k = 0; k < c->nchains; k++) {
         const ChainDev& cd = c->hchains[k];
@@ -440,6 +443,7 @@ static int do_bin(chmc_ctx* c)
 
 static int do_drift(chmc_ctx* c, double dt)
 {
+    c->list_valid = false;             // positions change outside the lists
     int rc = flush_drift(c);
     if (rc) return rc;
     if ((rc = settle_bins(c))) return rc;
@@ -721,7 +725,7 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, lalloc(c, &v.nb, N * nbcap));
     HIPCHK(c, lalloc(c, &v.nbn, N));
     HIPCHK(c, lalloc(c, &v.lflag, 4));
-    HIPCHK(c, lalloc(c, &v.lreq, 2 * (size_t)c->nchains));
+    HIPCHK(c, lalloc(c, &v.lreq, 2 * ((size_t)c->nchains + 1)));
     HIPCHK(c, lalloc(c, &v.lscan_tmp, (size_t)lcells));
     HIPCHK(c, lalloc(c, &v.x0, 3 * N));
     HIPCHK(c, lalloc(c, &v.r0, 3 * N));
@@ -735,7 +739,7 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, hipMemcpy(c->d_chains, c->hchains.data(), sizeof(ChainDev) * c->nchains, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(v.lcount, 0, sizeof(int) * lcells));
     HIPCHK(c, hipMemset(v.lflag, 0, sizeof(int) * 4));
-    HIPCHK(c, hipMemset(v.lreq, 0, sizeof(int) * 2 * c->nchains));
+    HIPCHK(c, hipMemset(v.lreq, 0, sizeof(int) * 2 * (c->nchains + 1)));
     // loffs of a chain's first cell is its atom offset; the end entry is Ntot
     std::vector<int> lo0(lcells + 1, 0);
     for (const ChainDev& cd : c->hchains)
@@ -761,6 +765,7 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, hipDeviceSynchronize());
     c->skin = skin;
     c->lpar = 0;
+    c->list_valid = false;
     if (v.lanes == 0) v.lanes = 8;
     c->list_ready = true;
     return CHMC_OK;
@@ -774,7 +779,7 @@ static int list_ensure(chmc_ctx* c)
 // request a rebuild before the next list sweep (positions changed outside the list path)
 static int list_invalidate(chmc_ctx* c)
 {
-    HIPCHK(c, launch_fill_int(c->v.lreq + (size_t)c->lpar * c->nchains, 1, c->nchains, c->stream));
+    HIPCHK(c, launch_fill_int(c->v.lreq + (size_t)c->lpar * (c->nchains + 1), 1, c->nchains + 1, c->stream));
     return CHMC_OK;
 }
 
@@ -873,7 +878,8 @@ static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double l
         int rc = flush_drift(c);
         if (rc) return rc;
         if ((rc = list_ensure(c))) return rc;
-        if ((rc = list_invalidate(c))) return rc;
+        if (!c->list_valid && (rc = list_invalidate(c))) return rc;
+        c->list_valid = true;
         for (size_t k = 0; k < ops.size(); k++) {
             if (ops[k].kick) {
                 const bool next_x = k + 1 < ops.size() && !ops[k + 1].kick;
@@ -886,6 +892,7 @@ static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double l
         }
         return CHMC_OK;
     }
+    c->list_valid = false;
     for (size_t k = 0; k < ops.size(); k++) {
         int rc;
         if (ops[k].kick && c->md_surr) {
@@ -935,7 +942,7 @@ extern "C" int chmc_save_positions(chmc_ctx* c)
     CHECK_CTX(c);
     int rc = flush_drift(c);
     if (rc) return rc;
-    HIPCHK(c, launch_save_restore(c->v, 0, c->stream));
+    HIPCHK(c, launch_save_restore(c->v, 0, -1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
 }
@@ -943,10 +950,11 @@ extern "C" int chmc_save_positions(chmc_ctx* c)
 extern "C" int chmc_restore_positions(chmc_ctx* c)
 {
     CHECK_CTX(c);
+    c->list_valid = false;             // positions change outside the lists
     int rc = flush_drift(c);
     if (rc) return rc;
     if ((rc = settle_bins(c))) return rc;
-    HIPCHK(c, launch_save_restore(c->v, 1, c->stream));
+    HIPCHK(c, launch_save_restore(c->v, 1, -1, c->stream));
     c->bin_state = 2;
     c->force_kind = 0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1026,11 +1034,12 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
         // heat bath + K_i; neighbour-list MD (fresh rebuild); FP64 V_f, W_f, K_f over the lists
         HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, false, c->stream));
         if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false))) return rc;
-        if (steps == 0 && (rc = list_invalidate(c))) return rc;
+        if (steps == 0 && !c->list_valid && (rc = list_invalidate(c))) return rc;
+        c->list_valid = true;
         if ((rc = list_energy(c, false))) return rc;
         HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, 0, SUM_TILES,
                                     c->stream));
-        HIPCHK(c, launch_save_restore(c->v, 2, c->stream));
+        HIPCHK(c, launch_save_restore(c->v, 2, c->lpar, c->stream));
         c->force_kind = 0;
         c->bin_state = 2;
         return CHMC_OK;
@@ -1045,7 +1054,9 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
     if ((rc = do_energy(c, false, false))) return rc;               // V_f, W_f and K_f partials
     HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, c->md_surr ? 1 : 0,
                                 SUM_CELLS, c->stream));
-    HIPCHK(c, launch_save_restore(c->v, 2, c->stream));
+    HIPCHK(c, launch_save_restore(c->v, 2, -1, c->stream));
+    // the positions moved outside the list path: lists rebuild before the next list sweep
+    if (list_md(c) && c->list_ready && (rc = list_invalidate(c))) return rc;
     c->force_kind = 0;
     c->bin_state = 2;
     return CHMC_OK;
@@ -1169,7 +1180,8 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
 
     // ---- V_i, W_i of the start configuration (HMC.cpp:64), h_x = x (HMC.cpp:72) ----
     if (list_md(c) && !hp->surrogate) {
-        if ((rc = list_ensure(c)) || (rc = list_invalidate(c)) || (rc = list_energy(c, true))) return rc;
+        if ((rc = list_ensure(c)) || (!c->list_valid && (rc = list_invalidate(c))) || (rc = list_energy(c, true))) return rc;
+        c->list_valid = true;
     } else if ((rc = do_energy(c, true))) {
         return rc;
     }
@@ -1183,7 +1195,7 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
         st[k].Wi = red[(size_t)k * RED_NSLOT + RED_W];
     }
     HIPCHK(c, hipMemcpyAsync(c->v.state, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, launch_save_restore(c->v, 0, c->stream));
+    HIPCHK(c, launch_save_restore(c->v, 0, -1, c->stream));
 
     // ---- trajectories, in windows when the surrogate drives the MD (acceptance-based fallback) ----
     std::vector<TrajGraph> graphs;

@@ -30,8 +30,9 @@
 //                                              FP32, atom id in w; double-buffered across drifts)
 //   nb            : int [Ntot][nbcap]         (j slot << 5 | image code 0..26 of the 27-cell stencil)
 //   nbn           : int [Ntot]
-//   lreq          : int [2][nchains]          (chain needs a rebuild before its next sweep at list
-//                                              parity q; chains rebuild independently)
+//   lreq          : int [2][1 + nchains]      (parity q: [0] any chain, [1 + k] chain k needs a
+//                                              rebuild before its next sweep; chains rebuild
+//                                              independently)
 //   lflag         : int [4]                   ([2]: capacity overflow, [3]: chain rebuilds run)
 //   lscan_tmp     : int [Cl]                  (global exclusive scan of the counts, rebuild scratch)
 //   tiles         : 32-slot ranges that never straddle chains (sweep / energy workgroups)
@@ -129,7 +130,8 @@ hipError_t launch_kinetic(const DevView& v, hipStream_t s);
 hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, hipStream_t s);
 hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustride, int therm, int ufirst,
                              HmcRecord* rec, int rec_base, int surr, int what, hipStream_t s);
-hipError_t launch_save_restore(const DevView& v, int mode, hipStream_t s);
+// list_q >= 0: neighbour-list MD at that list parity; the chains restored by mode 2 request a rebuild
+hipError_t launch_save_restore(const DevView& v, int mode, int list_q, hipStream_t s);
 
 // neighbour-list MD: the three rebuild kernels (each a no-op unless lflag[q] is set), the FP32
 // force sweep (LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT; with LJ_DRIFT the parity advances to q ^ 1),

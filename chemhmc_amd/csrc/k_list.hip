@@ -40,10 +40,18 @@ __device__ __forceinline__ int list_cell(const ChainDev& C, const double x3[3], 
 }
 
 // ---- rebuild -------------------------------------------------------------------------------------
-__device__ __forceinline__ int* req(const DevView& v, int q, int chain) { return v.lreq + q * v.nchains + chain; }
+// request flags of parity q: [0] any chain, [1 + k] chain k
+__device__ __forceinline__ int* req(const DevView& v, int q, int chain) { return v.lreq + q * (v.nchains + 1) + 1 + chain; }
+__device__ __forceinline__ int* req_any(const DevView& v, int q) { return v.lreq + q * (v.nchains + 1); }
+__device__ __forceinline__ void request(const DevView& v, int q, int chain)
+{
+    atomicOr(req(v, q, chain), 1);
+    atomicOr(req_any(v, q), 1);
+}
 
 __global__ __launch_bounds__(256) void k_lcount(DevView v, int q)
 {
+    if (*req_any(v, q) == 0) return;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= v.Ntot) return;
     const int ch = v.atom_chain[a];
@@ -62,13 +70,13 @@ __global__ __launch_bounds__(256) void k_lcount(DevView v, int q)
 // cells get atom_off + the scan relative to the chain's first cell; other chains keep their offsets
 __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q)
 {
+    if (*req_any(v, q) == 0) return;
     __shared__ int any;
     if (threadIdx.x == 0) any = 0;
     __syncthreads();
     for (int k = threadIdx.x; k < v.nchains; k += blockDim.x)
         if (*req(v, q, k)) atomicAdd(&any, 1);
     __syncthreads();
-    if (any == 0) return;
     __shared__ int wsum[16];
     const int nt = blockDim.x, tid = threadIdx.x;
     const int n = v.Cl;
@@ -124,7 +132,7 @@ __device__ __forceinline__ int member_rank(const int* __restrict__ mem, int base
 
 __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
 {
-    if (*req(v, q, v.lcell_chain[blockIdx.x]) == 0) return;
+    if (*req_any(v, q) == 0 || *req(v, q, v.lcell_chain[blockIdx.x]) == 0) return;
     constexpr int IS = 4, JG = 16, NT = 256, W = 4;
     __shared__ float4 sj[kLMaxJ];          // staged j: coordinates relative to the i-cell origin, slot in w
     __shared__ unsigned char scode[kLMaxJ];
@@ -322,7 +330,7 @@ __device__ __forceinline__ void list_drift_atom(const DevView& v, const ChainDev
         f3[d] = (float)(v.r0[d * N + slot] + e);
     }
     v.xsl[(size_t)(q ^ 1) * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(atom));
-    if (disp2 > C.half_skin2) atomicOr(req(v, q ^ 1, v.atom_chain[atom]), 1);
+    if (disp2 > C.half_skin2) request(v, q ^ 1, v.atom_chain[atom]);
 }
 
 __device__ __forceinline__ void load_shifts(float4* sh, const ChainDev& C)
@@ -345,7 +353,10 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
     const Chunk tl = v.tiles[blockIdx.x];
     const ChainDev& C = v.chains[tl.chain];
     load_shifts(sh, C);
-    if (tl.begin == C.atom_off && threadIdx.x == 0) *req(v, q, tl.chain) = 0;   // this evaluation's rebuild is done
+    if (threadIdx.x == 0) {                               // this evaluation's rebuild is done
+        if (tl.begin == C.atom_off) *req(v, q, tl.chain) = 0;
+        if (blockIdx.x == 0) *req_any(v, q) = 0;
+    }
     __syncthreads();
     const int N = v.Ntot;
     const int t = threadIdx.x / LP, sub = threadIdx.x % LP;
@@ -426,7 +437,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         const double ey = __shfl(e, base + 1, 64), ez = __shfl(e, base + 2, 64);
         if (val && sub == 0) {
             v.xsl[(size_t)(q ^ 1) * N + s] = make_float4(rel, ry, rz, __int_as_float(atom));
-            if (e * e + ey * ey + ez * ez > C.half_skin2) atomicOr(req(v, q ^ 1, tl.chain), 1);
+            if (e * e + ey * ey + ez * ez > C.half_skin2) request(v, q ^ 1, tl.chain);
         }
     }
 }
@@ -438,11 +449,12 @@ __global__ __launch_bounds__(256) void k_ldrift(DevView v, double dt, int q)
     const int atom = v.lperm[s];
     const int ch = v.atom_chain[atom];
     const ChainDev& C = v.chains[ch];
+    if (s == 0) *req_any(v, q) = 0;
     if (s == C.atom_off) {
         // a pending request of parity q carries over to q ^ 1 (nothing consumed it)
         const int pend = *req(v, q, ch);
         *req(v, q, ch) = 0;
-        if (pend) atomicOr(req(v, q ^ 1, ch), pend);
+        if (pend) request(v, q ^ 1, ch);
     }
     const int N = v.Ntot;
     const double pn[3] = { v.p[atom], v.p[N + atom], v.p[2 * N + atom] };
@@ -480,7 +492,10 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
     __shared__ double wpart[4];
     const Chunk tl = v.tiles[blockIdx.x];
     const ChainDev& C = v.chains[tl.chain];
-    if (tl.begin == C.atom_off && threadIdx.x == 0) *req(v, q, tl.chain) = 0;
+    if (threadIdx.x == 0) {
+        if (tl.begin == C.atom_off) *req(v, q, tl.chain) = 0;
+        if (blockIdx.x == 0) *req_any(v, q) = 0;
+    }
     const int N = v.Ntot;
     const int t = threadIdx.x >> 3, sub = threadIdx.x & 7;
     const int s = tl.begin + t;

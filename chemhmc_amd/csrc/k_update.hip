@@ -167,13 +167,22 @@ __global__ __launch_bounds__(256) void k_metropolis(DevView v, const double* __r
 
 // mode 0: save (x_saved = x), 1: restore (x = x_saved), 2: per chain by the Metropolis decision, which
 // also closes the trajectory on the device counter (read by the next heat bath and Metropolis)
-__global__ __launch_bounds__(256) void k_save_restore(DevView v, int mode)
+// list_q >= 0 (neighbour-list MD at list parity list_q): a rejected chain's restored positions get a
+// list rebuild before its next sweep
+__global__ __launch_bounds__(256) void k_save_restore(DevView v, int mode, int list_q)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (mode == 2 && a == 0) *v.traj = *v.traj + 1;
     if (a >= v.Ntot) return;
     int save = mode == 0;
-    if (mode == 2) save = v.state[v.atom_chain[a]].accepted;
+    if (mode == 2) {
+        const int ch = v.atom_chain[a];
+        save = v.state[ch].accepted;
+        if (!save && list_q >= 0 && a == v.chains[ch].atom_off) {
+            atomicOr(v.lreq + list_q * (v.nchains + 1) + 1 + ch, 1);
+            atomicOr(v.lreq + list_q * (v.nchains + 1), 1);
+        }
+    }
     const int N = v.Ntot;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -223,9 +232,9 @@ hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustri
     return hipGetLastError();
 }
 
-hipError_t launch_save_restore(const DevView& v, int mode, hipStream_t s)
+hipError_t launch_save_restore(const DevView& v, int mode, int list_q, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_save_restore, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, mode);
+    hipLaunchKernelGGL(k_save_restore, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, mode, list_q);
     return hipGetLastError();
 }
 
