@@ -6,16 +6,16 @@
 // a force evaluation between rebuilds is one sweep over ~(4/3) pi (rc + skin)^3 rho candidates per atom
 // instead of 27 cells' worth.
 //
-// Rebuild (three kernels that skip every chain whose request flag of the current list parity is clear,
+// Rebuild (four kernels that skip every chain whose request flag of the current list parity is clear,
 // so that they can sit in a captured trajectory and cost a launch when idle; chains rebuild
 // independently, so a chain of a batch evolves exactly as it does alone):
 //   k_lcount  atoms -> list cells (floor(x / lsize), clamped), arrival order into a dense member array
 //   k_lscan   one workgroup: exclusive scan of the cell counts -> first compact slot per cell
-//   k_lbuild  one workgroup per list cell: members sorted by atom id give the compact slots; the cell
-//             writes its own atoms' slot data (x0, r0 = x0 - cell origin, FP32 xsl) and their lists,
-//             staging the 27 neighbour cells (image shift dd * lsize folded in) the way k_lj does, with
-//             each neighbour's slot recomputed from its sorted rank, so no workgroup reads data another
-//             writes in the same launch
+//   k_lsort   one wavefront per list cell: members sorted by atom id give the compact slots; the cell
+//             writes its atoms' slot data (x0, r0 = x0 - cell origin, FP32 xsl)
+//   k_lbuild  one workgroup per list cell: the 27 neighbour cells' FP32 coordinates staged in LDS (image
+//             shift dd * lsize folded in, the way k_lj stages its j-list) and each own atom's list of
+//             the entries within rc + skin, in slot order
 // Sweep: 32 slots x LP lanes per workgroup (LP = 4, 8 or 16, tuned); each lane walks every LP-th list
 //   entry, gathers the neighbour's FP32 coordinates, and the partial sums meet in a butterfly.  Three
 //   lanes of an atom then kick p in FP64, optionally drift x in FP64 (identical_particles.cpp:651-673),
@@ -122,39 +122,70 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q)
 constexpr int kLMaxJ = 768;       // staged neighbour entries per pass
 constexpr int kLMaxI = 512;       // members of one list cell (lcap never exceeds this)
 
-// rank of atom `a` among the members [base, base + n) of a cell, by atom id
-__device__ __forceinline__ int member_rank(const int* __restrict__ mem, int base, int n, int a)
+// one wavefront per list cell: members ranked by atom id give the compact slots; the cell writes its
+// atoms' slot data (x0, r0 = x0 - cell origin, FP32 coordinates of this parity, lperm, linv)
+__global__ __launch_bounds__(64) void k_lsort(DevView v, int q)
 {
-    int r = 0;
-    for (int m = 0; m < n; m++) r += mem[base + m] < a ? 1 : 0;
-    return r;
+    if (*req_any(v, q) == 0) return;
+    const int c = blockIdx.x;
+    const ChainDev& C = v.chains[v.lcell_chain[c]];
+    if (*req(v, q, v.lcell_chain[c]) == 0) return;
+    __shared__ int sid[kLMaxI];
+    // member counts are clamped to the dense capacity: after an overflow (flagged by k_lcount) the
+    // missing atoms are simply absent and the host rejects the results
+    const int ibeg = v.loffs[c], ni = min(v.loffs[c + 1] - ibeg, C.lcap);
+    if (ni == 0) return;
+    const int lc = c - C.lcell_off;
+    const int nx = C.lnbin[0], ny = C.lnbin[1];
+    const int bz = lc / (nx * ny), by = (lc - bz * nx * ny) / nx, bx = lc - nx * (by + bz * ny);
+    const int ibase = C.ldense_off + lc * C.lcap;
+    for (int m = threadIdx.x; m < ni; m += 64) sid[m] = v.lmem[ibase + m];
+    __syncthreads();
+    const int N = v.Ntot;
+    const double o3[3] = { (double)bx * C.lsize[0], (double)by * C.lsize[1], (double)bz * C.lsize[2] };
+    for (int m = threadIdx.x; m < ni; m += 64) {
+        const int a = sid[m];
+        int r = 0;
+        for (int k = 0; k < ni; k++) r += sid[k] < a ? 1 : 0;
+        const int slot = ibeg + r;
+        float f3[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double xv = v.x[d * N + a];
+            const double rv = xv - o3[d];
+            v.x0[d * N + slot] = xv;
+            v.r0[d * N + slot] = rv;
+            f3[d] = (float)rv;
+        }
+        v.xsl[(size_t)q * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
+        v.lperm[slot] = a;
+        v.linv[a] = slot;
+    }
 }
 
+// one workgroup per list cell: the 27 neighbour cells' sorted FP32 coordinates are staged in LDS with
+// the image shift folded in (the way k_lj stages its j-list) and each own atom appends the entries
+// within rc + skin in staging order, so the list order is fixed by the slot order
 __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
 {
     if (*req_any(v, q) == 0 || *req(v, q, v.lcell_chain[blockIdx.x]) == 0) return;
     constexpr int IS = 4, JG = 16, NT = 256, W = 4;
     __shared__ float4 sj[kLMaxJ];          // staged j: coordinates relative to the i-cell origin, slot in w
     __shared__ unsigned char scode[kLMaxJ];
-    __shared__ float4 si[kLMaxI];          // own atoms by rank: FP32 r0, slot in w
+    __shared__ float4 si[kLMaxI];          // own atoms by slot: FP32 coordinates, slot in w
     __shared__ int icount[kLMaxI];
-    __shared__ int sid[kLMaxJ];            // member atom ids of the 27 cells (when they fit)
-    __shared__ int seg_beg[27], seg_pref[28], seg_slot0[27], seg_cnt[27];
+    __shared__ int seg_beg[27], seg_pref[28];
     __shared__ float seg_sh[27][3];
-    __shared__ double seg_org[27][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = blockIdx.x;
     const ChainDev& C = v.chains[v.lcell_chain[c]];
     const int nx = C.lnbin[0], ny = C.lnbin[1], nz = C.lnbin[2];
     const int lc = c - C.lcell_off;
     const int bz = lc / (nx * ny), by = (lc - bz * nx * ny) / nx, bx = lc - nx * (by + bz * ny);
-    // member counts are clamped to the dense capacity: after an overflow (flagged by k_lcount) the
-    // missing atoms are simply absent and the host rejects the results
     const int ibeg = v.loffs[c], ni = min(v.loffs[c + 1] - ibeg, C.lcap);
     if (ni == 0) return;
     const int N = v.Ntot;
-    const int* mem = v.lmem;
-    const int ibase = C.ldense_off + lc * C.lcap;
+    const float4* __restrict__ xq = v.xsl + (size_t)q * N;
     const int nbcap = v.nbcap;
 
     if (wave == 0) {
@@ -168,14 +199,10 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
                 j3[d] = b3[d] + dd[d];
                 if (j3[d] < 0 || j3[d] >= n3[d]) j3[d] = (b3[d] + n3[d] + dd[d]) % n3[d];
                 seg_sh[lane][d] = (float)dd[d] * C.lsize_f[d];
-                seg_org[lane][d] = (double)j3[d] * C.lsize[d];
             }
-            const int jl = j3[0] + nx * (j3[1] + j3[2] * ny);
-            const int jc = C.lcell_off + jl;
-            seg_beg[lane] = C.ldense_off + jl * C.lcap;
-            seg_slot0[lane] = v.loffs[jc];
+            const int jc = C.lcell_off + j3[0] + nx * (j3[1] + j3[2] * ny);
+            seg_beg[lane] = v.loffs[jc];
             cnt = min(v.loffs[jc + 1] - v.loffs[jc], C.lcap);
-            seg_cnt[lane] = cnt;
         }
         int incl = cnt;
 #pragma unroll
@@ -186,53 +213,13 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
         if (lane < 27) seg_pref[lane + 1] = incl;
         if (lane == 0) seg_pref[0] = 0;
     }
-    __syncthreads();
-    const int nj = seg_pref[27];
-    // members of the 27 cells in LDS when they fit (ranks then come from LDS), else from global memory
-    const bool lds_ids = nj <= kLMaxJ;
-    if (lds_ids) {
-        for (int t = tid; t < nj; t += NT) {
-            int k = 0;
-#pragma unroll
-            for (int step = 16; step > 0; step >>= 1)
-                if (k + step < 27 && seg_pref[k + step] <= t) k += step;
-            sid[t] = mem[seg_beg[k] + (t - seg_pref[k])];
-        }
-        __syncthreads();
-    }
-    auto rank_in = [&](int k, int a) -> int {
-        if (lds_ids) {
-            const int b = seg_pref[k], n = seg_cnt[k];
-            int r = 0;
-            for (int m = 0; m < n; m++) r += sid[b + m] < a ? 1 : 0;
-            return r;
-        }
-        return member_rank(mem, seg_beg[k], seg_cnt[k], a);
-    };
-    // own atoms (segment 13, dd = 0): compact slot by rank, rebuild reference data, FP32 coordinates
-    // of this parity
-    for (int m = tid; m < ni; m += NT) {
-        const int a = lds_ids ? sid[seg_pref[13] + m] : mem[ibase + m];
-        const int r = rank_in(13, a);
-        const int slot = ibeg + r;
-        const double o3[3] = { (double)bx * C.lsize[0], (double)by * C.lsize[1], (double)bz * C.lsize[2] };
-        float f3[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const double xv = v.x[d * N + a];
-            const double rv = xv - o3[d];
-            v.x0[d * N + slot] = xv;
-            v.r0[d * N + slot] = rv;
-            f3[d] = (float)rv;
-        }
-        const float4 xf = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
-        v.xsl[(size_t)q * N + slot] = xf;
-        v.lperm[slot] = a;
-        v.linv[a] = slot;
-        si[r] = make_float4(f3[0], f3[1], f3[2], __int_as_float(slot));
+    for (int r = tid; r < ni; r += NT) {
+        const float4 p = xq[ibeg + r];
+        si[r] = make_float4(p.x, p.y, p.z, __int_as_float(ibeg + r));
         icount[r] = 0;
     }
     __syncthreads();
+    const int nj = seg_pref[27];
     const float rl2 = C.rl2_f;
     const int ii = lane % IS, g = lane / IS;
     unsigned long long pattern = 0;
@@ -242,24 +229,15 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
     for (int jlo = 0; jlo < nj; jlo += kLMaxJ) {
         const int jhi = min(nj, jlo + kLMaxJ);
         if (jlo > 0) __syncthreads();
-        // every member goes to position seg_pref[k] + (its rank in its cell), so that the staged order,
-        // and with it the list order and the FP32 summation order, is independent of the arrival order
-        // of k_lcount's atomics; with several passes each pass scans all entries and keeps its window
-        for (int t = tid; t < nj; t += NT) {
+        for (int t = jlo + tid; t < jhi; t += NT) {
             int k = 0;
 #pragma unroll
             for (int step = 16; step > 0; step >>= 1)
                 if (k + step < 27 && seg_pref[k + step] <= t) k += step;
-            const int a = lds_ids ? sid[t] : mem[seg_beg[k] + (t - seg_pref[k])];
-            const int r = rank_in(k, a);
-            const int dst = seg_pref[k] + r;
-            if (dst < jlo || dst >= jhi) continue;
-            float f3[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) f3[d] = (float)(v.x[d * N + a] - seg_org[k][d]);   // its xsl value
-            sj[dst - jlo] = make_float4(f3[0] + seg_sh[k][0], f3[1] + seg_sh[k][1], f3[2] + seg_sh[k][2],
-                                        __int_as_float(seg_slot0[k] + r));
-            scode[dst - jlo] = (unsigned char)k;
+            const int slot = seg_beg[k] + (t - seg_pref[k]);
+            const float4 p = xq[slot];
+            sj[t - jlo] = make_float4(p.x + seg_sh[k][0], p.y + seg_sh[k][1], p.z + seg_sh[k][2], __int_as_float(slot));
+            scode[t - jlo] = (unsigned char)k;
         }
         __syncthreads();
         const int jn = jhi - jlo;
@@ -272,8 +250,9 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
             for (int j0 = 0; j0 < jn; j0 += JG) {
                 const int j = j0 + g;
                 bool hit = false;
+                float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (ival && j < jn) {
-                    const float4 p = sj[j];
+                    p = sj[j];
                     const float dx = xi.x - p.x, dy = xi.y - p.y, dz = xi.z - p.z;
                     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                     hit = r2 < rl2 && r2 > 0.f;
@@ -281,8 +260,7 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
                 const unsigned long long m = __ballot(hit) & pattern;
                 if (hit) {
                     const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
-                    if (pos < nbcap)
-                        v.nb[(size_t)slot_i * nbcap + pos] = (__float_as_int(sj[j].w) << 5) | (int)scode[j];
+                    if (pos < nbcap) v.nb[(size_t)slot_i * nbcap + pos] = (__float_as_int(p.w) << 5) | (int)scode[j];
                 }
                 count += __popcll(m);
             }
@@ -303,6 +281,8 @@ hipError_t launch_list_rebuild(const DevView& v, int q, hipStream_t s)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_lscan, dim3(1), dim3(1024), 0, s, v, q);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_lsort, dim3(v.Cl), dim3(64), 0, s, v, q);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_lbuild, dim3(v.Cl), dim3(256), 0, s, v, q);
     return hipGetLastError();
@@ -344,7 +324,7 @@ __device__ __forceinline__ void load_shifts(float4* sh, const ChainDev& C)
 // LP lanes per slot, kTile slots per workgroup.  Each lane first loads up to kBatch of its list entries
 // and gathers their coordinates, so that one memory latency covers most lists; the per-atom tail (kick,
 // drift, next-parity coordinates, displacement test) runs on three lanes, one per dimension.
-constexpr int kBatch = 8;
+constexpr int kBatch = 12;
 
 template <int MODE, int LP>
 __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q)
@@ -362,10 +342,13 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
     const int t = threadIdx.x / LP, sub = threadIdx.x % LP;
     const int s = tl.begin + t;
     const bool val = s < tl.end;
+    const int d = sub < 3 ? sub : 0;                      // the per-dimension tail runs on lanes 0, 1, 2
     const float4* __restrict__ xq = v.xsl + (size_t)q * N;
     const float rc2 = C.rc2_f, sig2 = C.sigma2_f;
     float ax = 0.f, ay = 0.f, az = 0.f;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    int atom = 0;
+    double p_d = 0.0, x_d = 0.0, x0_d = 0.0, r0_d = 0.0;
     auto pair = [&](const float4 pj, int code) {
         const float4 shf = sh[code];
         const float dx = xi.x - (pj.x + shf.x), dy = xi.y - (pj.y + shf.y), dz = xi.z - (pj.z + shf.z);
@@ -380,8 +363,18 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         az = __builtin_fmaf(gq, dz, az);
     };
     if (val) {
+        atom = v.lperm[s];
         xi = xq[s];
         const int n = v.nbn[s];
+        // the tail's operands are fetched now, alongside the list, not after the butterfly
+        if (sub < 3) {
+            p_d = v.p[d * N + atom];
+            if constexpr ((MODE & LJ_DRIFT) != 0) {
+                x_d = v.x[d * N + atom];
+                x0_d = v.x0[d * N + s];
+                r0_d = v.r0[d * N + s];
+            }
+        }
         const int* __restrict__ nbs = v.nb + (size_t)s * v.nbcap;
         int e[kBatch];
 #pragma unroll
@@ -406,15 +399,12 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         ay += __shfl_xor(ay, m, 64);
         az += __shfl_xor(az, m, 64);
     }
-    // per-dimension tail on lanes sub = 0, 1, 2 of the slot
-    const int d = sub < 3 ? sub : 0;
-    const int atom = __float_as_int(xi.w);
     double e = 0.0;
     float rel = 0.f;
     if (val && sub < 3) {
         const float fd = C.eps48_f * (d == 0 ? ax : (d == 1 ? ay : az));
         if constexpr ((MODE & LJ_STOREF) != 0) v.f32[d * N + atom] = fd;
-        double pn = v.p[d * N + atom];
+        double pn = p_d;
         if constexpr ((MODE & LJ_KICK) != 0) {
             const double kdt = kick_dt * C.beta;       // dt * coeff_p, coeff_p = beta (:642-644)
             pn = pn - kdt * (double)fd;
@@ -422,13 +412,13 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         }
         if constexpr ((MODE & LJ_DRIFT) != 0) {
             const double coeff_x = C.beta / C.mass;    // compute_coeff_position, identical_particles.cpp:646-648
-            double xv = v.x[d * N + atom];
+            double xv = x_d;
             xv += drift_dt * coeff_x * pn;
             xv -= C.L[d] * floor(xv / C.L[d]);
             v.x[d * N + atom] = xv;
-            e = xv - v.x0[d * N + s];
+            e = xv - x0_d;
             e -= C.L[d] * rint(e / C.L[d]);
-            rel = (float)(v.r0[d * N + s] + e);
+            rel = (float)(r0_d + e);
         }
     }
     if constexpr ((MODE & LJ_DRIFT) != 0) {
