@@ -8,10 +8,10 @@ the state resident in HBM and ends after the device has finished the K-th trajec
 
   value      atom-steps/s = N * 10 MD steps * K trajectories / seconds
   ess_per_s  ESS/s of U/N over the timed trajectories (Sokal-windowed tau_int)
-  roofline   the dominant kernel, the FP32 neighbour-list sweep (force + kick + drift): algorithmic flops
-             per launch (8 per list entry + 13 per interacting pair, both counted on the device for the
-             current configuration) / its average duration measured with HIP events on the context's
-             stream
+  roofline   the dominant kernel, the FP32 neighbour-list sweep (force + kick + drift), HBM-bound by its
+             arithmetic intensity: algorithmic bytes per launch (4 per list entry + 184 per atom, list
+             entries counted on the device for the current configuration) / its average duration
+             measured with HIP events on the context's stream (50 launches back to back)
   cpu_baseline  the FP64 C oracle (a port of the reference algorithm, cpu-opt binning) timed on this
              host's cores for a bounded sample of the same workload
 """
@@ -32,6 +32,9 @@ N, RHO, T, RC = 32768, 0.8442, 0.722, 2.5
 DT, NSTEPS, LAM = 0.01, 10, 0.1932
 PEAK_FP32_TFLOPS = 157.3        # MI355X vector FP32 (vendor figure; packed FP32)
 PEAK_HBM_GBS = 8000.0
+# list sweep bytes per atom besides its list row: lperm 4, own FP32 coordinates 16, count 4, p 24 + 24,
+# x 24 + 24, x0 24, r0 24, next FP32 coordinates 16
+SWEEP_BYTES_PER_ATOM = 184.0
 
 
 def _omp_threads(n):
@@ -107,7 +110,8 @@ def other_workload(args):
         nint += b
     ms_sweep, ms_rebuild = e.time_list(reps=20)
     flops = 8.0 * ent + 13.0 * nint
-    achieved = flops / (ms_sweep * 1e-3) / 1e12
+    nbytes = 4.0 * ent + SWEEP_BYTES_PER_ATOM * nat
+    achieved = nbytes / (ms_sweep * 1e-3) / 1e9
     u = rec["u"] / np.array([c.N for c in w.chains])[None, :]
     ess = sum(steps / (2 * stats.tau_int(u[:, k])[0]) for k in range(len(w.chains)))
     out = {
@@ -120,11 +124,113 @@ def other_workload(args):
                    "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "skin": e.skin(),
                    "warmup_seconds": t_warm},
         "acceptance": float(nacc.sum()) / (steps * len(w.chains)), "ess_per_s_sum_over_chains": ess / el,
-        "roofline": {"bound": "valu", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "flops_per_launch": flops, "ms_per_launch": ms_sweep, "ms_rebuild": ms_rebuild,
+        "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "bytes_per_launch": nbytes, "flops_per_launch": flops, "tflops": flops / (ms_sweep * 1e-3) / 1e12,
+                     "ms_per_launch": ms_sweep, "ms_rebuild": ms_rebuild,
                      "list_entries_per_atom": ent / nat, "n_int_per_atom": nint / nat},
     }
+    print(json.dumps(out))
+
+
+def surrogate_workload(args):
+    """--workload c4: the learned surrogate trained on configurations of plain-HMC chains of the N=512
+    system (rho*=0.8, T*=1.5; 5000 configurations of 512 atoms), then HMC at N=32768, rho*=0.8442,
+    T*=0.722 (OMF2 10 x 0.01) with surrogate-driven MD and the exact FP64 LJ accept step, against plain
+    HMC from the same state: acceptance, ms per trajectory and ESS/s of U/N of both."""
+    import chemhmc_amd as chmc
+    from chemhmc_amd import stats
+
+    rng = np.random.default_rng(4)
+    nch, nconf_per = 20, 250                       # 20 chains x 250 snapshots = 5000 configurations
+    Lc1 = (512 / 0.8) ** (1 / 3)
+    tr = chmc.Engine([chmc.Chain(N=512, L=Lc1, cutoff=RC, beta=1 / 1.5)] * nch)
+    tr.init_lattice(3, 0.05)
+    tr.hmc(100, 101, chmc.LEAPFROG, 0.005, 20, seed=5)          # accept-all burn-in
+    p0 = {"W1": rng.normal(0, 1 / np.sqrt(32), (64, 32)), "b1": np.zeros(64),
+          "W2": rng.normal(0, 1 / np.sqrt(64), (64, 64)), "b2": np.zeros(64),
+          "w3": rng.normal(0, 1 / np.sqrt(64), 64), "b3": np.zeros(1), "shift": np.zeros(32), "scale": np.ones(32)}
+    tr.surrogate_set_params(p0)
+    tr.surrogate_reserve(nconf_per)
+    t0 = time.perf_counter()
+    for k in range(nconf_per):
+        tr.hmc(2, 0, chmc.LEAPFROG, 0.005, 20, seed=100 + k)
+        rows = tr.surrogate_collect()
+    t_collect = time.perf_counter() - t0
+    G, u = tr.surrogate_training_data(0, min(rows, 200000))
+    p0["shift"] = G.mean(0).astype(np.float64)
+    p0["scale"] = 1.0 / (G.std(0).astype(np.float64) + 1e-3)
+    p0["b3"] = np.array([float(u.mean())])
+    tr.surrogate_set_params(p0)
+    t0 = time.perf_counter()
+    hist = tr.surrogate_train(args.train_steps, 4096, lr=1e-3, seed=6)
+    t_train = time.perf_counter() - t0
+    params = tr.surrogate_params()
+    tr.close()
+
+    L = (N / RHO) ** (1 / 3)
+    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
+    e.init_lattice(1, 0.05)
+    e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11)
+    e.surrogate_set_params(params)
+    x0 = e.positions()
+    steps = args.steps if args.steps_given else 500
+    out = {}
+    for name, surr in (("plain", False), ("surrogate", True)):
+        e.set_positions(x0)
+        e.set_trajectory(args.warmup)
+        e.kinetic()
+        t0 = time.perf_counter()
+        rec, nacc = e.hmc(steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph,
+                          surrogate=surr, fallback_window=0)
+        el = time.perf_counter() - t0
+        uu = rec["u"][:, 0] / N
+        tau, _ = stats.tau_int(uu)
+        out[name] = {"acceptance": float(nacc[0]) / steps, "ms_per_traj": el / steps * 1e3,
+                     "u_per_n": float(uu.mean()), "tau_int_u": tau, "ess_per_s": steps / (2 * tau) / el,
+                     "value": N * NSTEPS * steps / el}
+    res = {
+        "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N", "value": out["surrogate"]["value"],
+        "unit": "atom-steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": out["surrogate"]["ms_per_traj"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 MFMA surrogate MD, f64 LJ accept", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
+        "config": {"workload": "c4: surrogate HMC N=32768 rho=0.8442 T=0.722 OMF2(0.1932) 10x0.01; surrogate 32-64-64-1 "
+                               "trained on 5000 N=512 rho=0.8 T=1.5 configurations",
+                   "train_rows": rows, "train_steps": args.train_steps, "train_batch": 4096, "lr": 1e-3,
+                   "collect_seconds": t_collect, "train_seconds": t_train,
+                   "train_mse_first": float(hist[:20].mean()), "train_mse_last": float(hist[-100:].mean())},
+        "plain": out["plain"], "surrogate": out["surrogate"],
+        "ess_per_s_ratio_surrogate_over_plain": out["surrogate"]["ess_per_s"] / out["plain"]["ess_per_s"],
+    }
+    print(json.dumps(res))
+
+
+def cell_md_run(args, e, chmc, stats):
+    """The same c2 measurement through the 27-cell sweep with a re-binning per force evaluation (the
+    reference's cost model, CHMC_CELL_MD) instead of the neighbour lists."""
+    if args.warmup > 0:
+        e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
+    e.set_force_tiling(0)
+    t0 = time.perf_counter()
+    rec, nacc = e.hmc(args.steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph)
+    el = time.perf_counter() - t0
+    u = rec["u"][:, 0] / N
+    tau, _ = stats.tau_int(u)
+    ncand, nint = e.pair_counts(0)
+    flops = 8.0 * ncand + 13.0 * nint
+    ms_bin, ms_force = e.time_force(reps=50)
+    achieved = flops / (ms_force * 1e-3) / 1e12
+    out = {"metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N", "value": N * NSTEPS * args.steps / el,
+           "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 pair forces, f64 x/p/dH", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
+           "config": {"workload": "plain HMC N=32768 rho=0.8442 T=0.722 rc=2.5 OMF2(lambda=0.1932) 10x0.01",
+                      "md_path": "27-cell sweep, re-binning per evaluation", "force_tiling": e.force_tiling()},
+           "acceptance": float(nacc[0]) / args.steps, "tau_int_u": tau, "ess_per_s": args.steps / (2 * tau) / el,
+           "roofline": {"bound": "valu", "kernel": "k_lj<float> force+kick sweep", "achieved": achieved,
+                        "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                        "traffic": None, "flops_per_launch": flops, "ms_per_launch": ms_force, "ms_binning": ms_bin,
+                        "n_cand_per_atom": ncand / N, "n_int_per_atom": nint / N}}
     print(json.dumps(out))
 
 
@@ -138,8 +244,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every trajectory directly instead of replaying the captured graph")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"],
-                    help="c2 (default, the headline line), c3 batched chains, c5 N=1M")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2 (default, the headline line), c3 batched chains, c4 surrogate vs plain, c5 N=1M")
+    ap.add_argument("--train-steps", type=int, default=4000, help="c4: Adam steps of the surrogate")
+    ap.add_argument("--cell-md", action="store_true",
+                    help="c2 through the 27-cell sweep with a re-binning per evaluation (CHMC_CELL_MD)")
     args = ap.parse_args()
     args.steps_given = "--steps" in sys.argv
     args.warmup_given = "--warmup" in sys.argv
@@ -148,6 +257,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.workload == "c4":
+        surrogate_workload(args)
+        return
     if args.workload != "c2":
         other_workload(args)
         return
@@ -156,8 +268,11 @@ def main():
     from chemhmc_amd import stats
 
     L = (N / RHO) ** (1 / 3)
-    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
+    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T), cell_md=args.cell_md)
     e.init_lattice(1, 0.05)
+    if args.cell_md:
+        cell_md_run(args, e, chmc, stats)
+        return
     if args.warmup > 0:     # accept-all thermalisation away from the lattice (thermalization_steps)
         e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
     # skin autotuner: wall time of a short run per skin on the equilibrated state
@@ -184,13 +299,18 @@ def main():
     tau, _ = stats.tau_int(u)
     ess = args.steps / (2 * tau)
 
-    # dominant kernel: the neighbour-list FP32 sweep (force + kick + drift); algorithmic flops of this
-    # configuration: 8 per list entry (candidate) + 13 per interacting pair
+    # dominant kernel: the neighbour-list FP32 sweep (force + kick + drift).  Its arithmetic intensity
+    # (~2.6 flop/B) is below the ridge (157.3 TF/s / 8 TB/s = 19.7 flop/B), so the roofline that bounds
+    # it is HBM: algorithmic bytes per launch = 4 B per list entry + SWEEP_BYTES_PER_ATOM per atom
+    # (own coordinates, count, atom id, rebuild reference, p and x read and written, next coordinates);
+    # the neighbour coordinates it gathers are cache hits and not counted.  Flops: 8 per list entry
+    # (candidate) + 13 per interacting pair.
     entries, nint, rebuilds1 = e.list_stats()
     rebuilds_per_traj = (rebuilds1 - rebuilds0 - 1) / args.steps       # list_stats itself rebuilds once
     flops = 8.0 * entries + 13.0 * nint
+    nbytes = 4.0 * entries + SWEEP_BYTES_PER_ATOM * N
     ms_sweep, ms_rebuild = e.time_list(reps=50)
-    achieved = flops / (ms_sweep * 1e-3) / 1e12
+    achieved = nbytes / (ms_sweep * 1e-3) / 1e9
     ncand_cell, _ = e.pair_counts(0)
 
     out = {
@@ -204,9 +324,10 @@ def main():
         "acceptance": float(nacc[0]) / args.steps,
         "u_per_n": float(u.mean()), "tau_int_u": tau, "ess_per_s": ess / el,
         "list_rebuilds_per_traj": rebuilds_per_traj,
-        "roofline": {"bound": "valu", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": None, "flops_per_launch": flops, "ms_per_launch": ms_sweep,
+        "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": None, "bytes_per_launch": nbytes, "flops_per_launch": flops,
+                     "tflops": flops / (ms_sweep * 1e-3) / 1e12, "ms_per_launch": ms_sweep,
                      "list_entries_per_atom": entries / N, "n_int_per_atom": nint / N,
                      "cell_candidates_per_atom": ncand_cell / N, "ms_rebuild": ms_rebuild},
     }

@@ -747,6 +747,7 @@ static int list_setup(chmc_ctx* c, double skin)
     lo0[lcells] = c->Ntot;
     HIPCHK(c, hipMemcpy(v.loffs, lo0.data(), sizeof(int) * lo0.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(v.nbn, 0, sizeof(int) * N));
+    HIPCHK(c, hipMemset(v.nb, 0, sizeof(int) * N * nbcap));    // every entry a valid slot (see k_lsweep)
     // valid atom ids in every slot before the first rebuild (a slot is rewritten by each rebuild)
     std::vector<int> iota(N);
     std::vector<float4> xs0(2 * N);
@@ -1685,24 +1686,28 @@ extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* m
     int rc = flush_drift(c);
     if (rc) return rc;
     if ((rc = list_ensure(c))) return rc;
-    float tb = 0.f, ts = 0.f;
-    for (int r = 0; r < reps + 1; r++) {
+    // rebuilds: `reps` forced rebuilds back to back (request flags set by a fill kernel inside the span)
+    if ((rc = list_invalidate(c))) return rc;
+    HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));            // warm-up
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; r++) {
         if ((rc = list_invalidate(c))) return rc;
-        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-        // the trajectory's MD step: force, kick and drift with dt = 0 (positions and momenta unchanged)
-        HIPCHK(c, launch_list_sweep(c->v, LJ_FORCE | LJ_KICK | LJ_DRIFT, c->lpar, 0.0, 0.0, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev2, c->stream));
-        c->lpar ^= 1;
-        HIPCHK(c, hipEventSynchronize(c->ev2));
-        float a = 0.f, b = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&a, c->ev0, c->ev1));
-        HIPCHK(c, hipEventElapsedTime(&b, c->ev1, c->ev2));
-        if (r > 0) { tb += a; ts += b; }      // the first pass warms up
     }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    // the trajectory's MD step alone: force, kick and drift with dt = 0 (positions and momenta
+    // unchanged, so the lists stay valid), `reps` sweeps back to back
+    for (int r = 0; r < reps; r++) {
+        HIPCHK(c, launch_list_sweep(c->v, LJ_FORCE | LJ_KICK | LJ_DRIFT, c->lpar, 0.0, 0.0, c->stream));
+        c->lpar ^= 1;
+    }
+    HIPCHK(c, hipEventRecord(c->ev2, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev2));
+    float a = 0.f, b = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&a, c->ev0, c->ev1));
+    HIPCHK(c, hipEventElapsedTime(&b, c->ev1, c->ev2));
     c->force_kind = 0;
-    if (ms_sweep) *ms_sweep = ts / reps;
-    if (ms_rebuild) *ms_rebuild = tb / reps;
+    if (ms_sweep) *ms_sweep = b / reps;
+    if (ms_rebuild) *ms_rebuild = a / reps;
     return list_check_overflow(c);
 }

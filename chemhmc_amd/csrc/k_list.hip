@@ -363,6 +363,15 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         az = __builtin_fmaf(gq, dz, az);
     };
     if (val) {
+        const int* __restrict__ nbs = v.nb + (size_t)s * v.nbcap;
+        // the list entries are read before the count is known (entries past the count are older valid
+        // slots, or zero: the list buffer is cleared at setup) so that both arrive in one round trip
+        int e[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int k = sub + u * LP;
+            e[u] = k < v.nbcap ? nbs[k] : 0;
+        }
         atom = v.lperm[s];
         xi = xq[s];
         const int n = v.nbn[s];
@@ -374,13 +383,6 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
                 x0_d = v.x0[d * N + s];
                 r0_d = v.r0[d * N + s];
             }
-        }
-        const int* __restrict__ nbs = v.nb + (size_t)s * v.nbcap;
-        int e[kBatch];
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-            const int k = sub + u * LP;
-            e[u] = k < n ? nbs[k] : 0;
         }
         float4 pj[kBatch];
 #pragma unroll
