@@ -109,3 +109,23 @@ def test_adam_reduces_loss():
         S.adam_step(p, g, state, lr=1e-2)
     l1, _ = S.loss_grad(p, G, u)
     assert l1 < 0.1 * l0
+
+
+def test_adam_matches_torch_optimizer():
+    """Three Adam steps of the oracle against torch.optim.Adam (same betas, eps, lr) on a fixed
+    gradient sequence; the first step is also the closed form -lr * g / (|g| + eps)."""
+    rng = np.random.default_rng(3)
+    w0 = rng.normal(size=(4, 5))
+    grads = [rng.normal(size=(4, 5)) for _ in range(3)]
+    p = {"W": w0.copy()}
+    state = {}
+    t = torch.tensor(w0.copy(), dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([t], lr=1e-3, betas=(0.9, 0.999), eps=1e-8)
+    for k, g in enumerate(grads):
+        S.adam_step(p, {"W": g}, state, lr=1e-3)
+        opt.zero_grad()
+        t.grad = torch.tensor(g, dtype=torch.float64)
+        opt.step()
+        assert np.allclose(p["W"], t.detach().numpy(), rtol=0, atol=1e-15)
+        if k == 0:
+            assert np.allclose(p["W"], w0 - 1e-3 * grads[0] / (np.abs(grads[0]) + 1e-8), atol=1e-15)
