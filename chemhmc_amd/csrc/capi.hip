@@ -59,6 +59,7 @@ struct chmc_ctx {
     bool list_ready = false;
     int lpar = 0;                // list parity: which FP32 coordinate buffer / rebuild flag is current
     bool list_valid = false;     // the lists and their rebuild flags account for every position change
+    bool saved_lists = false;    // the last save kept a valid list buffer with the positions
     std::vector<void*> lallocs;
     std::vector<Chunk> htiles;
 };
@@ -718,19 +719,21 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, lalloc(c, &lcell_chain, (size_t)lcells));
     HIPCHK(c, lalloc(c, &tiles, (size_t)v.ntiles));
     HIPCHK(c, lalloc(c, &v.lcount, (size_t)lcells));
-    HIPCHK(c, lalloc(c, &v.loffs, (size_t)lcells + 1));
+    // the per-slot list state twice (current / trajectory-start buffers)
+    HIPCHK(c, lalloc(c, &v.loffs, 2 * ((size_t)lcells + 1)));
     HIPCHK(c, lalloc(c, &v.lmem, (size_t)dense));
-    HIPCHK(c, lalloc(c, &v.lperm, N));
-    HIPCHK(c, lalloc(c, &v.linv, N));
-    HIPCHK(c, lalloc(c, &v.nb, N * nbcap));
-    HIPCHK(c, lalloc(c, &v.nbn, N));
+    HIPCHK(c, lalloc(c, &v.lperm, 2 * N));
+    HIPCHK(c, lalloc(c, &v.linv, 2 * N));
+    HIPCHK(c, lalloc(c, &v.nb, 2 * N * nbcap));
+    HIPCHK(c, lalloc(c, &v.nbn, 2 * N));
+    HIPCHK(c, lalloc(c, &v.lbuf, 2 * (size_t)c->nchains));
     HIPCHK(c, lalloc(c, &v.lflag, 4));
     HIPCHK(c, lalloc(c, &v.lreq, 2 * ((size_t)c->nchains + 1)));
     HIPCHK(c, lalloc(c, &v.lscan_tmp, (size_t)lcells));
-    HIPCHK(c, lalloc(c, &v.x0, 3 * N));
-    HIPCHK(c, lalloc(c, &v.r0, 3 * N));
+    HIPCHK(c, lalloc(c, &v.x0, 2 * 3 * N));
+    HIPCHK(c, lalloc(c, &v.r0, 2 * 3 * N));
     HIPCHK(c, lalloc(c, &v.rel64, 3 * N));
-    HIPCHK(c, lalloc(c, &v.xsl, 2 * N));
+    HIPCHK(c, lalloc(c, &v.xsl, 2 * 2 * N));
     HIPCHK(c, lalloc(c, &v.tile_part, 4 * (size_t)v.ntiles));
     v.lcell_chain = lcell_chain;
     v.tiles = tiles;
@@ -740,29 +743,34 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, hipMemset(v.lcount, 0, sizeof(int) * lcells));
     HIPCHK(c, hipMemset(v.lflag, 0, sizeof(int) * 4));
     HIPCHK(c, hipMemset(v.lreq, 0, sizeof(int) * 2 * (c->nchains + 1)));
+    HIPCHK(c, hipMemset(v.lbuf, 0, sizeof(int) * 2 * c->nchains));
     // loffs of a chain's first cell is its atom offset; the end entry is Ntot
     std::vector<int> lo0(lcells + 1, 0);
     for (const ChainDev& cd : c->hchains)
         for (int i = 0; i < cd.lncells; i++) lo0[cd.lcell_off + i] = cd.atom_off;
     lo0[lcells] = c->Ntot;
-    HIPCHK(c, hipMemcpy(v.loffs, lo0.data(), sizeof(int) * lo0.size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(v.nbn, 0, sizeof(int) * N));
-    HIPCHK(c, hipMemset(v.nb, 0, sizeof(int) * N * nbcap));    // every entry a valid slot (see k_lsweep)
+    for (int b = 0; b < 2; b++)
+        HIPCHK(c, hipMemcpy(v.loffs + (size_t)b * (lcells + 1), lo0.data(), sizeof(int) * lo0.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(v.nbn, 0, sizeof(int) * 2 * N));
+    HIPCHK(c, hipMemset(v.nb, 0, sizeof(int) * 2 * N * nbcap));    // every entry a valid slot (see k_lsweep)
     // valid atom ids in every slot before the first rebuild (a slot is rewritten by each rebuild)
     std::vector<int> iota(N);
-    std::vector<float4> xs0(2 * N);
+    std::vector<float4> xs0(N);
     for (size_t i = 0; i < N; i++) {
         iota[i] = (int)i;
         int ii = (int)i;
         float w;
         std::memcpy(&w, &ii, sizeof(float));
-        xs0[i] = xs0[N + i] = make_float4(0.f, 0.f, 0.f, w);
+        xs0[i] = make_float4(0.f, 0.f, 0.f, w);
     }
-    HIPCHK(c, hipMemcpy(v.lperm, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(v.linv, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(v.xsl, xs0.data(), sizeof(float4) * 2 * N, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(v.x0, 0, sizeof(double) * 3 * N));
-    HIPCHK(c, hipMemset(v.r0, 0, sizeof(double) * 3 * N));
+    for (int b = 0; b < 2; b++) {
+        HIPCHK(c, hipMemcpy(v.lperm + b * N, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(v.linv + b * N, iota.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+        for (int q = 0; q < 2; q++)
+            HIPCHK(c, hipMemcpy(v.xsl + (2 * b + q) * N, xs0.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
+    }
+    HIPCHK(c, hipMemset(v.x0, 0, sizeof(double) * 2 * 3 * N));
+    HIPCHK(c, hipMemset(v.r0, 0, sizeof(double) * 2 * 3 * N));
     HIPCHK(c, hipDeviceSynchronize());
     c->skin = skin;
     c->lpar = 0;
@@ -943,7 +951,9 @@ extern "C" int chmc_save_positions(chmc_ctx* c)
     CHECK_CTX(c);
     int rc = flush_drift(c);
     if (rc) return rc;
-    HIPCHK(c, launch_save_restore(c->v, 0, -1, c->stream));
+    // with valid lists the save also keeps their buffer, so a restore needs no rebuild
+    c->saved_lists = list_md(c) && c->list_ready && c->list_valid;
+    HIPCHK(c, launch_save_restore(c->v, 0, c->saved_lists ? c->lpar : -1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
 }
@@ -951,11 +961,12 @@ extern "C" int chmc_save_positions(chmc_ctx* c)
 extern "C" int chmc_restore_positions(chmc_ctx* c)
 {
     CHECK_CTX(c);
-    c->list_valid = false;             // positions change outside the lists
     int rc = flush_drift(c);
     if (rc) return rc;
     if ((rc = settle_bins(c))) return rc;
-    HIPCHK(c, launch_save_restore(c->v, 1, -1, c->stream));
+    const bool lists = c->saved_lists && list_md(c) && c->list_ready;
+    c->list_valid = lists;             // the saved list buffer is valid for the restored positions
+    HIPCHK(c, launch_save_restore(c->v, 1, lists ? c->lpar : -1, c->stream));
     c->bin_state = 2;
     c->force_kind = 0;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1032,7 +1043,15 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
 {
     int rc;
     if (list_md(c) && !c->md_surr) {
-        // heat bath + K_i; neighbour-list MD (fresh rebuild); FP64 V_f, W_f, K_f over the lists
+        // heat bath + K_i; neighbour-list MD; FP64 V_f, W_f, K_f over the lists
+        if (!c->list_valid) {
+            // positions moved outside the lists (a surrogate trajectory): rebuild now and make the
+            // rebuilt buffer the start buffer of this trajectory
+            if ((rc = list_invalidate(c))) return rc;
+            HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));
+            HIPCHK(c, launch_save_restore(c->v, 0, c->lpar, c->stream));
+            c->list_valid = true;
+        }
         HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, false, c->stream));
         if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false))) return rc;
         if (steps == 0 && !c->list_valid && (rc = list_invalidate(c))) return rc;
@@ -1196,7 +1215,8 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
         st[k].Wi = red[(size_t)k * RED_NSLOT + RED_W];
     }
     HIPCHK(c, hipMemcpyAsync(c->v.state, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, launch_save_restore(c->v, 0, -1, c->stream));
+    const bool lists = list_md(c) && c->list_ready && c->list_valid;
+    HIPCHK(c, launch_save_restore(c->v, 0, lists ? c->lpar : -1, c->stream));
 
     // ---- trajectories, in windows when the surrogate drives the MD (acceptance-based fallback) ----
     std::vector<TrajGraph> graphs;
@@ -1635,7 +1655,10 @@ extern "C" int chmc_list_stats(chmc_ctx* c, int chain, long long* entries, long 
     if ((rc = read_red(c, RED_NINT, nintv.data()))) return rc;
     const ChainDev& cd = c->hchains[chain];
     std::vector<int> n(cd.natoms);
-    HIPCHK(c, hipMemcpy(n.data(), c->v.nbn + cd.atom_off, sizeof(int) * cd.natoms, hipMemcpyDeviceToHost));
+    int cur = 0;
+    HIPCHK(c, hipMemcpy(&cur, c->v.lbuf + chain, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(n.data(), c->v.nbn + (size_t)cur * c->Ntot + cd.atom_off, sizeof(int) * cd.natoms,
+                        hipMemcpyDeviceToHost));
     int f[4];
     HIPCHK(c, hipMemcpy(f, c->v.lflag, sizeof(f), hipMemcpyDeviceToHost));
     long long tot = 0;

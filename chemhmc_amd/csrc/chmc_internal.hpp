@@ -34,6 +34,10 @@
 //                                              rebuild before its next sweep; chains rebuild
 //                                              independently)
 //   lflag         : int [4]                   ([2]: capacity overflow, [3]: chain rebuilds run)
+//   The per-slot list state (loffs, lperm, linv, x0, r0, xsl, nb, nbn) exists twice: a rebuild writes
+//   the buffer that is not the trajectory's start buffer, so a rejected trajectory goes back to the
+//   start buffer (valid for the restored positions) instead of rebuilding.  lbuf[k] / lbuf[nchains + k]
+//   are chain k's current and start buffers.
 //   lscan_tmp     : int [Cl]                  (global exclusive scan of the counts, rebuild scratch)
 //   tiles         : 32-slot ranges that never straddle chains (sweep / energy workgroups)
 #pragma once
@@ -99,12 +103,36 @@ struct DevView {
     const int* lcell_chain;
     const Chunk* tiles;
     int *lcount, *loffs, *lmem, *lperm, *linv, *nb, *nbn, *lflag, *lreq, *lscan_tmp;
+    int* lbuf;            // [2][nchains]: current list buffer, buffer of the trajectory's start
     double *x0, *r0, *rel64;
     float4* xsl;
     double* tile_part;    // [ntiles * 4]: energy, virial, interacting pairs, kinetic energy
 };
 
 enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8, LJ_DRIFT = 16 };
+
+// one of the two neighbour-list buffers
+struct ListBuf {
+    int *nb, *nbn, *lperm, *linv, *loffs;
+    double *x0, *r0;      // [3][Ntot] by slot
+    float4* xsl;          // [2 parities][Ntot] by slot
+};
+__device__ __forceinline__ ListBuf list_buf(const DevView& v, int b)
+{
+    const size_t N = (size_t)v.Ntot;
+    ListBuf L;
+    L.nb = v.nb + (size_t)b * N * v.nbcap;
+    L.nbn = v.nbn + (size_t)b * N;
+    L.lperm = v.lperm + (size_t)b * N;
+    L.linv = v.linv + (size_t)b * N;
+    L.loffs = v.loffs + (size_t)b * ((size_t)v.Cl + 1);
+    L.x0 = v.x0 + (size_t)b * 3 * N;
+    L.r0 = v.r0 + (size_t)b * 3 * N;
+    L.xsl = v.xsl + (size_t)b * 2 * N;
+    return L;
+}
+__device__ __forceinline__ int* list_cur(const DevView& v, int chain) { return v.lbuf + chain; }
+__device__ __forceinline__ int* list_start(const DevView& v, int chain) { return v.lbuf + v.nchains + chain; }
 
 // current / next binning buffers (the parity of the completed-binning counter selects them)
 __device__ __forceinline__ int bin_parity(const DevView& v) { return v.par; }
@@ -130,7 +158,7 @@ hipError_t launch_kinetic(const DevView& v, hipStream_t s);
 hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, hipStream_t s);
 hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustride, int therm, int ufirst,
                              HmcRecord* rec, int rec_base, int surr, int what, hipStream_t s);
-// list_q >= 0: neighbour-list MD at that list parity; the chains restored by mode 2 request a rebuild
+// list_q >= 0: neighbour-list MD at that list parity; saves and restores also switch the list buffers
 hipError_t launch_save_restore(const DevView& v, int mode, int list_q, hipStream_t s);
 
 // neighbour-list MD: the three rebuild kernels (each a no-op unless lflag[q] is set), the FP32

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q)
         const int k = v.lcell_chain[c];
         if (*req(v, q, k) == 0) continue;
         const ChainDev& C = v.chains[k];
-        v.loffs[c] = C.atom_off + v.lscan_tmp[c] - v.lscan_tmp[C.lcell_off];
+        list_buf(v, *list_start(v, k) ^ 1).loffs[c] = C.atom_off + v.lscan_tmp[c] - v.lscan_tmp[C.lcell_off];
         v.lcount[c] = 0;
     }
     if (tid == 0) v.lflag[3] += any;        // chain rebuilds run (statistics)
@@ -128,12 +128,17 @@ __global__ __launch_bounds__(64) void k_lsort(DevView v, int q)
 {
     if (*req_any(v, q) == 0) return;
     const int c = blockIdx.x;
-    const ChainDev& C = v.chains[v.lcell_chain[c]];
-    if (*req(v, q, v.lcell_chain[c]) == 0) return;
+    const int ch = v.lcell_chain[c];
+    const ChainDev& C = v.chains[ch];
+    if (*req(v, q, ch) == 0) return;
     __shared__ int sid[kLMaxI];
+    // a rebuild writes the buffer that is not the trajectory's start buffer, and makes it current
+    const int tb = *list_start(v, ch) ^ 1;
+    const ListBuf L = list_buf(v, tb);
+    if (threadIdx.x == 0) *list_cur(v, ch) = tb;
     // member counts are clamped to the dense capacity: after an overflow (flagged by k_lcount) the
     // missing atoms are simply absent and the host rejects the results
-    const int ibeg = v.loffs[c], ni = min(v.loffs[c + 1] - ibeg, C.lcap);
+    const int ibeg = L.loffs[c], ni = min(L.loffs[c + 1] - ibeg, C.lcap);
     if (ni == 0) return;
     const int lc = c - C.lcell_off;
     const int nx = C.lnbin[0], ny = C.lnbin[1];
@@ -153,13 +158,13 @@ __global__ __launch_bounds__(64) void k_lsort(DevView v, int q)
         for (int d = 0; d < 3; d++) {
             const double xv = v.x[d * N + a];
             const double rv = xv - o3[d];
-            v.x0[d * N + slot] = xv;
-            v.r0[d * N + slot] = rv;
+            L.x0[d * N + slot] = xv;
+            L.r0[d * N + slot] = rv;
             f3[d] = (float)rv;
         }
-        v.xsl[(size_t)q * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
-        v.lperm[slot] = a;
-        v.linv[a] = slot;
+        L.xsl[(size_t)q * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
+        L.lperm[slot] = a;
+        L.linv[a] = slot;
     }
 }
 
@@ -178,14 +183,16 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
     __shared__ float seg_sh[27][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = blockIdx.x;
-    const ChainDev& C = v.chains[v.lcell_chain[c]];
+    const int ch = v.lcell_chain[c];
+    const ChainDev& C = v.chains[ch];
+    const ListBuf L = list_buf(v, *list_start(v, ch) ^ 1);
     const int nx = C.lnbin[0], ny = C.lnbin[1], nz = C.lnbin[2];
     const int lc = c - C.lcell_off;
     const int bz = lc / (nx * ny), by = (lc - bz * nx * ny) / nx, bx = lc - nx * (by + bz * ny);
-    const int ibeg = v.loffs[c], ni = min(v.loffs[c + 1] - ibeg, C.lcap);
+    const int ibeg = L.loffs[c], ni = min(L.loffs[c + 1] - ibeg, C.lcap);
     if (ni == 0) return;
     const int N = v.Ntot;
-    const float4* __restrict__ xq = v.xsl + (size_t)q * N;
+    const float4* __restrict__ xq = L.xsl + (size_t)q * N;
     const int nbcap = v.nbcap;
 
     if (wave == 0) {
@@ -201,8 +208,8 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
                 seg_sh[lane][d] = (float)dd[d] * C.lsize_f[d];
             }
             const int jc = C.lcell_off + j3[0] + nx * (j3[1] + j3[2] * ny);
-            seg_beg[lane] = v.loffs[jc];
-            cnt = min(v.loffs[jc + 1] - v.loffs[jc], C.lcap);
+            seg_beg[lane] = L.loffs[jc];
+            cnt = min(L.loffs[jc + 1] - L.loffs[jc], C.lcap);
         }
         int incl = cnt;
 #pragma unroll
@@ -260,7 +267,7 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
                 const unsigned long long m = __ballot(hit) & pattern;
                 if (hit) {
                     const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
-                    if (pos < nbcap) v.nb[(size_t)slot_i * nbcap + pos] = (__float_as_int(p.w) << 5) | (int)scode[j];
+                    if (pos < nbcap) L.nb[(size_t)slot_i * nbcap + pos] = (__float_as_int(p.w) << 5) | (int)scode[j];
                 }
                 count += __popcll(m);
             }
@@ -270,7 +277,7 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
     __syncthreads();
     for (int r = tid; r < ni; r += NT) {
         const int n = icount[r];
-        v.nbn[ibeg + r] = min(n, nbcap);
+        L.nbn[ibeg + r] = min(n, nbcap);
         if (n > nbcap) atomicOr(&v.lflag[2], 2);
     }
 }
@@ -291,8 +298,8 @@ hipError_t launch_list_rebuild(const DevView& v, int q, hipStream_t s)
 // ---- sweep ---------------------------------------------------------------------------------------
 // drift of one atom by dt (FP64, wrap into [0, L)), its FP32 list coordinates of the next parity and
 // the rebuild request when it is beyond skin / 2 from its rebuild position
-__device__ __forceinline__ void list_drift_atom(const DevView& v, const ChainDev& C, int atom, int slot, int q,
-                                                double dt, const double pn[3])
+__device__ __forceinline__ void list_drift_atom(const DevView& v, const ListBuf& L, const ChainDev& C, int atom,
+                                                int slot, int q, double dt, const double pn[3])
 {
     const int N = v.Ntot;
     const double coeff_x = C.beta / C.mass;       // compute_coeff_position, identical_particles.cpp:646-648
@@ -304,12 +311,12 @@ __device__ __forceinline__ void list_drift_atom(const DevView& v, const ChainDev
         xv += dt * coeff_x * pn[d];
         xv -= C.L[d] * floor(xv / C.L[d]);
         v.x[d * N + atom] = xv;
-        double e = xv - v.x0[d * N + slot];
+        double e = xv - L.x0[d * N + slot];
         e -= C.L[d] * rint(e / C.L[d]);
         disp2 += e * e;
-        f3[d] = (float)(v.r0[d * N + slot] + e);
+        f3[d] = (float)(L.r0[d * N + slot] + e);
     }
-    v.xsl[(size_t)(q ^ 1) * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(atom));
+    L.xsl[(size_t)(q ^ 1) * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(atom));
     if (disp2 > C.half_skin2) request(v, q ^ 1, v.atom_chain[atom]);
 }
 
@@ -332,6 +339,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
     __shared__ float4 sh[27];
     const Chunk tl = v.tiles[blockIdx.x];
     const ChainDev& C = v.chains[tl.chain];
+    const ListBuf L = list_buf(v, *list_cur(v, tl.chain));
     load_shifts(sh, C);
     if (threadIdx.x == 0) {                               // this evaluation's rebuild is done
         if (tl.begin == C.atom_off) *req(v, q, tl.chain) = 0;
@@ -343,7 +351,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
     const int s = tl.begin + t;
     const bool val = s < tl.end;
     const int d = sub < 3 ? sub : 0;                      // the per-dimension tail runs on lanes 0, 1, 2
-    const float4* __restrict__ xq = v.xsl + (size_t)q * N;
+    const float4* __restrict__ xq = L.xsl + (size_t)q * N;
     const float rc2 = C.rc2_f, sig2 = C.sigma2_f;
     float ax = 0.f, ay = 0.f, az = 0.f;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -363,7 +371,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         az = __builtin_fmaf(gq, dz, az);
     };
     if (val) {
-        const int* __restrict__ nbs = v.nb + (size_t)s * v.nbcap;
+        const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
         // the list entries are read before the count is known (entries past the count are older valid
         // slots, or zero: the list buffer is cleared at setup) so that both arrive in one round trip
         int e[kBatch];
@@ -372,16 +380,16 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
             const int k = sub + u * LP;
             e[u] = k < v.nbcap ? nbs[k] : 0;
         }
-        atom = v.lperm[s];
+        atom = L.lperm[s];
         xi = xq[s];
-        const int n = v.nbn[s];
+        const int n = L.nbn[s];
         // the tail's operands are fetched now, alongside the list, not after the butterfly
         if (sub < 3) {
             p_d = v.p[d * N + atom];
             if constexpr ((MODE & LJ_DRIFT) != 0) {
                 x_d = v.x[d * N + atom];
-                x0_d = v.x0[d * N + s];
-                r0_d = v.r0[d * N + s];
+                x0_d = L.x0[d * N + s];
+                r0_d = L.r0[d * N + s];
             }
         }
         float4 pj[kBatch];
@@ -428,7 +436,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         const float ry = __shfl(rel, base + 1, 64), rz = __shfl(rel, base + 2, 64);
         const double ey = __shfl(e, base + 1, 64), ez = __shfl(e, base + 2, 64);
         if (val && sub == 0) {
-            v.xsl[(size_t)(q ^ 1) * N + s] = make_float4(rel, ry, rz, __int_as_float(atom));
+            L.xsl[(size_t)(q ^ 1) * N + s] = make_float4(rel, ry, rz, __int_as_float(atom));
             if (e * e + ey * ey + ez * ez > C.half_skin2) request(v, q ^ 1, tl.chain);
         }
     }
@@ -438,9 +446,10 @@ __global__ __launch_bounds__(256) void k_ldrift(DevView v, double dt, int q)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= v.Ntot) return;
-    const int atom = v.lperm[s];
-    const int ch = v.atom_chain[atom];
+    const int ch = v.atom_chain[s];          // a chain's slots span the same range as its atoms
     const ChainDev& C = v.chains[ch];
+    const ListBuf L = list_buf(v, *list_cur(v, ch));
+    const int atom = L.lperm[s];
     if (s == 0) *req_any(v, q) = 0;
     if (s == C.atom_off) {
         // a pending request of parity q carries over to q ^ 1 (nothing consumed it)
@@ -450,7 +459,7 @@ __global__ __launch_bounds__(256) void k_ldrift(DevView v, double dt, int q)
     }
     const int N = v.Ntot;
     const double pn[3] = { v.p[atom], v.p[N + atom], v.p[2 * N + atom] };
-    list_drift_atom(v, C, atom, s, q, dt, pn);
+    list_drift_atom(v, L, C, atom, s, q, dt, pn);
 }
 
 // ---- FP64 energy ---------------------------------------------------------------------------------
@@ -458,14 +467,16 @@ __global__ __launch_bounds__(256) void k_lrel64(DevView v)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= v.Ntot) return;
-    const int atom = v.lperm[s];
-    const ChainDev& C = v.chains[v.atom_chain[atom]];
+    const int ch = v.atom_chain[s];
+    const ChainDev& C = v.chains[ch];
+    const ListBuf L = list_buf(v, *list_cur(v, ch));
+    const int atom = L.lperm[s];
     const int N = v.Ntot;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double e = v.x[d * N + atom] - v.x0[d * N + s];
+        double e = v.x[d * N + atom] - L.x0[d * N + s];
         e -= C.L[d] * rint(e / C.L[d]);
-        v.rel64[d * N + s] = v.r0[d * N + s] + e;
+        v.rel64[d * N + s] = L.r0[d * N + s] + e;
     }
 }
 
@@ -484,6 +495,7 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
     __shared__ double wpart[4];
     const Chunk tl = v.tiles[blockIdx.x];
     const ChainDev& C = v.chains[tl.chain];
+    const ListBuf L = list_buf(v, *list_cur(v, tl.chain));
     if (threadIdx.x == 0) {
         if (tl.begin == C.atom_off) *req(v, q, tl.chain) = 0;
         if (blockIdx.x == 0) *req_any(v, q) = 0;
@@ -495,8 +507,8 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
     double au = 0.0, aw = 0.0, an = 0.0, ak = 0.0;
     if (val) {
         const double xi = v.rel64[s], yi = v.rel64[N + s], zi = v.rel64[2 * N + s];
-        const int n = v.nbn[s];
-        const int* __restrict__ nbs = v.nb + (size_t)s * v.nbcap;
+        const int n = L.nbn[s];
+        const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
         for (int k = sub; k < n; k += 8) {
             const int e = nbs[k];
             const int j = e >> 5, code = e & 31;
@@ -513,7 +525,7 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
             }
         }
         if (sub == 0) {
-            const int atom = v.lperm[s];
+            const int atom = L.lperm[s];
             const double px = v.p[atom], py = v.p[N + atom], pz = v.p[2 * N + atom];
             ak = (px * px + py * py + pz * pz) / (2 * C.mass);
         }

@@ -167,28 +167,42 @@ __global__ __launch_bounds__(256) void k_metropolis(DevView v, const double* __r
 
 // mode 0: save (x_saved = x), 1: restore (x = x_saved), 2: per chain by the Metropolis decision, which
 // also closes the trajectory on the device counter (read by the next heat bath and Metropolis)
-// list_q >= 0 (neighbour-list MD at list parity list_q): a rejected chain's restored positions get a
-// list rebuild before its next sweep
+// list_q >= 0 (neighbour-list MD at list parity list_q): a save makes the current list buffer the start
+// buffer; a restore makes the start buffer current again and recomputes its FP32 coordinates of parity
+// list_q for the restored positions (they are within skin / 2 of the start buffer's rebuild positions,
+// which it was valid for when they were saved, so no rebuild is needed)
 __global__ __launch_bounds__(256) void k_save_restore(DevView v, int mode, int list_q)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (mode == 2 && a == 0) *v.traj = *v.traj + 1;
     if (a >= v.Ntot) return;
+    const int ch = v.atom_chain[a];
+    const ChainDev& C = v.chains[ch];
     int save = mode == 0;
-    if (mode == 2) {
-        const int ch = v.atom_chain[a];
-        save = v.state[ch].accepted;
-        if (!save && list_q >= 0 && a == v.chains[ch].atom_off) {
-            atomicOr(v.lreq + list_q * (v.nchains + 1) + 1 + ch, 1);
-            atomicOr(v.lreq + list_q * (v.nchains + 1), 1);
-        }
-    }
+    if (mode == 2) save = v.state[ch].accepted;
     const int N = v.Ntot;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         if (save) v.x_saved[d * N + a] = v.x[d * N + a];
         else v.x[d * N + a] = v.x_saved[d * N + a];
     }
+    if (list_q < 0) return;
+    if (save) {
+        if (a == C.atom_off) *list_start(v, ch) = *list_cur(v, ch);
+        return;
+    }
+    const int b = *list_start(v, ch);
+    if (a == C.atom_off) *list_cur(v, ch) = b;
+    const ListBuf L = list_buf(v, b);
+    const int slot = L.linv[a];
+    float f3[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double e = v.x_saved[d * N + a] - L.x0[d * N + slot];
+        e -= C.L[d] * rint(e / C.L[d]);
+        f3[d] = (float)(L.r0[d * N + slot] + e);
+    }
+    L.xsl[(size_t)list_q * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
 }
 
 // Standalone kick with the stored force: p -= (dt*beta) f  (identical_particles.cpp:686-688)

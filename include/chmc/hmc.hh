@@ -89,6 +89,10 @@ public:
     void get_p(config<coordinate>& p) { check(chmc_get_momenta(ctx_, 0, reinterpret_cast<double*>(p.data())), ctx_, "get_momenta"); }
     void pull_x() { get_x(h_x); }                      // deep_copy(h_x, x)
     void push_x() { set_x(h_x); }                      // deep_copy(x, h_x)
+    // the same copies kept on the device (HMC.cpp:72,103,107,111 without the host round trip); they
+    // also keep the neighbour lists of the saved positions, so a restore needs no rebuild
+    void save_x() { check(chmc_save_positions(ctx_), ctx_, "save_positions"); }
+    void restore_x() { check(chmc_restore_positions(ctx_), ctx_, "restore_positions"); }
 
     void hb(uint64_t seed, uint64_t traj) { check(chmc_heatbath(ctx_, seed, traj), ctx_, "hb"); }
     double compute_potential() { double v; check(chmc_compute_potential(ctx_, &v), ctx_, "compute_potential"); return v; }
@@ -249,6 +253,7 @@ public:
         identical_particles* P = integrator->particles;
         double Vi = P->compute_potential();
         const double beta = P->get_beta();
+        P->save_x();
         P->pull_x();
         const int first = params.istart + 1, last = params.Ntrajectories + params.istart + 1;
         for (int i = first; i < last; i++) {
@@ -268,8 +273,8 @@ public:
                 acc = r < exp_mdh;
                 if (acc) acceptance++;
             }
-            if (acc) { Vi = Vf; Ki = Kf; P->pull_x(); }
-            else P->push_x();
+            if (acc) { Vi = Vf; Ki = Kf; P->save_x(); P->pull_x(); }
+            else P->restore_x();
             history.push_back(traj_record{ i, Ki, Vi, Kf, Vf, dh, acc });
             if (verbose) std::printf("traj %d dh= %.12g accepted= %d V= %.12g\n", i, dh, (int)acc, Vi);
         }
