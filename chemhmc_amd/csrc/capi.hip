@@ -58,6 +58,7 @@ struct chmc_ctx {
     double skin = 0.3;
     bool list_ready = false;
     int lpar = 0;                // list parity: which FP32 coordinate buffer / rebuild flag is current
+    int cpar = 0;                // list count buffer of the next rebuild launch (alternates per launch)
     bool list_valid = false;     // the lists and their rebuild flags account for every position change
     bool saved_lists = false;    // the last save kept a valid list buffer with the positions
     std::vector<void*> lallocs;
@@ -710,15 +711,22 @@ static int list_setup(chmc_ctx* c, double skin)
     const int nbcap = ((int)std::ceil(nbmax) + 31) / 32 * 32;
     const size_t N = (size_t)c->Ntot;
     if ((double)N * nbcap > 4.0e9) return fail(c, CHMC_E_ARG, "neighbour lists too large (reduce the skin)");
+    if (N >= (size_t)1 << (31 - kImageBits)) return fail(c, CHMC_E_ARG, "too many atoms for the list entry format");
     DevView& v = c->v;
     v.Cl = (int)lcells;
     v.ntiles = (int)c->htiles.size();
     v.nbcap = nbcap;
+    v.lscan_in_build = 1;
+    v.lcap_max = 1;
+    for (const ChainDev& cd : c->hchains) {
+        if (cd.lncells > kScanCells) v.lscan_in_build = 0;
+        v.lcap_max = std::max(v.lcap_max, cd.lcap);
+    }
     int* lcell_chain = nullptr;
     Chunk* tiles = nullptr;
     HIPCHK(c, lalloc(c, &lcell_chain, (size_t)lcells));
     HIPCHK(c, lalloc(c, &tiles, (size_t)v.ntiles));
-    HIPCHK(c, lalloc(c, &v.lcount, (size_t)lcells));
+    HIPCHK(c, lalloc(c, &v.lcount, 2 * (size_t)lcells));
     // the per-slot list state twice (current / trajectory-start buffers)
     HIPCHK(c, lalloc(c, &v.loffs, 2 * ((size_t)lcells + 1)));
     HIPCHK(c, lalloc(c, &v.lmem, (size_t)dense));
@@ -727,6 +735,7 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, lalloc(c, &v.nb, 2 * N * nbcap));
     HIPCHK(c, lalloc(c, &v.nbn, 2 * N));
     HIPCHK(c, lalloc(c, &v.lbuf, 2 * (size_t)c->nchains));
+    HIPCHK(c, lalloc(c, &v.tcur, (size_t)v.ntiles));
     HIPCHK(c, lalloc(c, &v.lflag, 4));
     HIPCHK(c, lalloc(c, &v.lreq, 2 * ((size_t)c->nchains + 1)));
     HIPCHK(c, lalloc(c, &v.lscan_tmp, (size_t)lcells));
@@ -740,10 +749,11 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, hipMemcpy(lcell_chain, hlc.data(), sizeof(int) * hlc.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(tiles, c->htiles.data(), sizeof(Chunk) * c->htiles.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_chains, c->hchains.data(), sizeof(ChainDev) * c->nchains, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(v.lcount, 0, sizeof(int) * lcells));
+    HIPCHK(c, hipMemset(v.lcount, 0, sizeof(int) * 2 * lcells));
     HIPCHK(c, hipMemset(v.lflag, 0, sizeof(int) * 4));
     HIPCHK(c, hipMemset(v.lreq, 0, sizeof(int) * 2 * (c->nchains + 1)));
     HIPCHK(c, hipMemset(v.lbuf, 0, sizeof(int) * 2 * c->nchains));
+    HIPCHK(c, hipMemset(v.tcur, 0, sizeof(int) * v.ntiles));
     // loffs of a chain's first cell is its atom offset; the end entry is Ntot
     std::vector<int> lo0(lcells + 1, 0);
     for (const ChainDev& cd : c->hchains)
@@ -774,6 +784,7 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, hipDeviceSynchronize());
     c->skin = skin;
     c->lpar = 0;
+    c->cpar = 0;
     c->list_valid = false;
     if (v.lanes == 0) v.lanes = 8;
     c->list_ready = true;
@@ -785,6 +796,15 @@ static int list_ensure(chmc_ctx* c)
     return c->list_ready ? CHMC_OK : list_setup(c, c->skin);
 }
 
+// the rebuild kernels of the current list parity (a no-op for chains without a request); every launch
+// alternates the count buffer
+static hipError_t list_rebuild(chmc_ctx* c)
+{
+    const hipError_t e = launch_list_rebuild(c->v, c->lpar, c->cpar, c->stream);
+    c->cpar ^= 1;
+    return e;
+}
+
 // request a rebuild before the next list sweep (positions changed outside the list path)
 static int list_invalidate(chmc_ctx* c)
 {
@@ -794,7 +814,7 @@ static int list_invalidate(chmc_ctx* c)
 
 static int list_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift, double drift_dt)
 {
-    HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));
+    HIPCHK(c, list_rebuild(c));
     const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0) | (fuse_drift ? LJ_DRIFT : 0);
     HIPCHK(c, launch_list_sweep(c->v, mode, c->lpar, dt, fuse_drift ? drift_dt : 0.0, c->stream));
     if (fuse_drift) c->lpar ^= 1;
@@ -815,7 +835,7 @@ static int list_drift(chmc_ctx* c, double dt)
 // FP64 energy / virial / kinetic per tile at the current positions (lists brought up to date first)
 static int list_energy(chmc_ctx* c, bool sums)
 {
-    HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));
+    HIPCHK(c, list_rebuild(c));
     HIPCHK(c, launch_list_energy(c->v, c->lpar, c->stream));
     if (sums) HIPCHK(c, launch_reduce(c->v, SUM_TILES, c->stream));
     return CHMC_OK;
@@ -1048,7 +1068,7 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
             // positions moved outside the lists (a surrogate trajectory): rebuild now and make the
             // rebuilt buffer the start buffer of this trajectory
             if ((rc = list_invalidate(c))) return rc;
-            HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));
+            HIPCHK(c, list_rebuild(c));
             HIPCHK(c, launch_save_restore(c->v, 0, c->lpar, c->stream));
             c->list_valid = true;
         }
@@ -1082,10 +1102,11 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
     return CHMC_OK;
 }
 
-// a captured trajectory sequence: valid for replays that start at binning parity `par` in MD mode `surr`
+// a captured trajectory sequence: valid for replays that start at binning parity `par`, list parity
+// `lpar` and list count buffer `cpar` in MD mode `surr`
 struct TrajGraph {
     hipGraphExec_t exec;
-    int par, lpar, per_graph;
+    int par, lpar, cpar, per_graph;
     bool surr;
 };
 
@@ -1105,19 +1126,19 @@ static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count
     int done = 0;
     const TrajGraph* g = nullptr;
     for (const TrajGraph& k : graphs)
-        if (k.par == c->v.par && k.lpar == c->lpar && k.surr == c->md_surr) g = &k;
+        if (k.par == c->v.par && k.lpar == c->lpar && k.cpar == c->cpar && k.surr == c->md_surr) g = &k;
     if (!g) {
-        // one trajectory per graph when it performs an even number of placements (the binning parity
-        // baked into the captured launches then repeats), two otherwise.  The host state after the
-        // capture is that of per_graph trajectories, which every replay reproduces.
+        // one trajectory per graph when the parities baked into the captured launches (binning, list,
+        // list count buffer) end where they started, two otherwise.  The host state after the capture
+        // is that of per_graph trajectories, which every replay reproduces.
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        const int par0 = c->v.par, lpar0 = c->lpar;
+        const int par0 = c->v.par, lpar0 = c->lpar, cpar0 = c->cpar;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         rc = enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base);
         int per_graph = 1;
-        if (!rc && (c->v.par != par0 || c->lpar != lpar0)) {
+        if (!rc && (c->v.par != par0 || c->lpar != lpar0 || c->cpar != cpar0)) {
             rc = enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base);
             per_graph = 2;
         }
@@ -1127,13 +1148,14 @@ static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count
         hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         HIPCHK(c, ei);
-        graphs.push_back(TrajGraph{ exec, par0, lpar0, per_graph, c->md_surr });
+        graphs.push_back(TrajGraph{ exec, par0, lpar0, cpar0, per_graph, c->md_surr });
         g = &graphs.back();
         if (count < per_graph) {
             // nothing of the capture ran; the host state is restored by running the leftover directly
             // from the capture's start parities
             c->v.par = par0;
             c->lpar = lpar0;
+            c->cpar = cpar0;
         } else {
             HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
             done = per_graph;
@@ -1711,11 +1733,11 @@ extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* m
     if ((rc = list_ensure(c))) return rc;
     // rebuilds: `reps` forced rebuilds back to back (request flags set by a fill kernel inside the span)
     if ((rc = list_invalidate(c))) return rc;
-    HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));            // warm-up
+    HIPCHK(c, list_rebuild(c));            // warm-up
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     for (int r = 0; r < reps; r++) {
         if ((rc = list_invalidate(c))) return rc;
-        HIPCHK(c, launch_list_rebuild(c->v, c->lpar, c->stream));
+        HIPCHK(c, list_rebuild(c));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     // the trajectory's MD step alone: force, kick and drift with dt = 0 (positions and momenta

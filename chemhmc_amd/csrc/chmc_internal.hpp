@@ -20,15 +20,18 @@
 // per-cell counts equals chain atom offset + the chain's own exclusive scan.
 //
 // Neighbour-list MD (FP32 forces between rebuilds; cells of side >= rc + skin on a second grid):
-//   lcount        : int [Cl]                  (zero between rebuilds)
+//   lcount        : int [2][Cl]               (member counts; the host alternates the buffer with every
+//                                              rebuild launch, which clears the other one)
 //   loffs         : int [Cl + 1]              (exclusive scan: first compact slot of each list cell)
-//   lmem          : int [sum_chains lncells * lcap]  (members of each cell in arrival order)
+//   lmem          : float4 [sum_chains lncells * lcap]  (members of each cell in arrival order: FP32
+//                                              position relative to the cell origin, atom id in w)
 //   lperm / linv  : int [Ntot]                (compact slot -> atom / atom -> slot, atom-id order
 //                                              inside a cell)
 //   x0, r0        : double SoA [3][Ntot] by slot (position at the rebuild; minus its cell origin)
 //   xsl           : float4 [2][Ntot] by slot  (current position relative to the rebuild cell origin,
 //                                              FP32, atom id in w; double-buffered across drifts)
-//   nb            : int [Ntot][nbcap]         (j slot << 5 | image code 0..26 of the 27-cell stencil)
+//   nb            : int [Ntot][nbcap]         (j slot << 6 | image code: 2 bits per dimension, the
+//                                              neighbour cell offset + 1, bits 0-1 x, 2-3 y, 4-5 z)
 //   nbn           : int [Ntot]
 //   lreq          : int [2][1 + nchains]      (parity q: [0] any chain, [1 + k] chain k needs a
 //                                              rebuild before its next sweep; chains rebuild
@@ -37,7 +40,8 @@
 //   The per-slot list state (loffs, lperm, linv, x0, r0, xsl, nb, nbn) exists twice: a rebuild writes
 //   the buffer that is not the trajectory's start buffer, so a rejected trajectory goes back to the
 //   start buffer (valid for the restored positions) instead of rebuilding.  lbuf[k] / lbuf[nchains + k]
-//   are chain k's current and start buffers.
+//   are chain k's current and start buffers; tcur[t] repeats the current buffer of tile t's chain, so
+//   that a sweep workgroup finds its buffer with one load that depends on nothing but its index.
 //   lscan_tmp     : int [Cl]                  (global exclusive scan of the counts, rebuild scratch)
 //   tiles         : 32-slot ranges that never straddle chains (sweep / energy workgroups)
 #pragma once
@@ -80,6 +84,7 @@ struct Chunk { int chain, begin, end, pad_; };
 constexpr int kMaxJ = 768;         // j-list entries staged in LDS per pass
 constexpr int kRedChunk = 256;     // atoms per deterministic reduction chunk
 constexpr int kTile = 32;          // slots per neighbour-list sweep workgroup (8 lanes per slot)
+constexpr int kScanCells = 3072;   // list cells of a chain whose counts one build workgroup scans in LDS
 
 // ---- launch wrappers (each defined in the TU that owns the kernel) ----------------------
 struct DevView {
@@ -102,8 +107,12 @@ struct DevView {
     int Cl, ntiles, nbcap, lanes;   // lanes: list-sweep lanes per slot (4, 8, 16)
     const int* lcell_chain;
     const Chunk* tiles;
-    int *lcount, *loffs, *lmem, *lperm, *linv, *nb, *nbn, *lflag, *lreq, *lscan_tmp;
+    int *lcount, *loffs, *lperm, *linv, *nb, *nbn, *lflag, *lreq, *lscan_tmp;
+    float4* lmem;
+    int lscan_in_build;   // every chain has at most kScanCells list cells: the build scans the counts
+    int lcap_max;         // largest list-cell member capacity over the chains (at most 512)
     int* lbuf;            // [2][nchains]: current list buffer, buffer of the trajectory's start
+    int* tcur;            // [ntiles]: current list buffer of each tile's chain (kept equal to lbuf)
     double *x0, *r0, *rel64;
     float4* xsl;
     double* tile_part;    // [ntiles * 4]: energy, virial, interacting pairs, kinetic energy
@@ -133,6 +142,21 @@ __device__ __forceinline__ ListBuf list_buf(const DevView& v, int b)
 }
 __device__ __forceinline__ int* list_cur(const DevView& v, int chain) { return v.lbuf + chain; }
 __device__ __forceinline__ int* list_start(const DevView& v, int chain) { return v.lbuf + v.nchains + chain; }
+// a thread of chain C that owns slot s marks its tile's current buffer when s begins a tile
+__device__ __forceinline__ void tile_cur_mark(const DevView& v, const ChainDev& C, int s, int b)
+{
+    const int r = s - C.atom_off;
+    if (r % kTile == 0) v.tcur[C.tile_first + r / kTile] = b;
+}
+
+// image code of a list entry: neighbour cell offset + 1 in two bits per dimension
+__device__ __forceinline__ int image_code(int dx, int dy, int dz) { return (dx + 1) | ((dy + 1) << 2) | ((dz + 1) << 4); }
+// the offset of one dimension times the cell side, exactly (-side, 0 or +side: f * side - side is exact)
+__device__ __forceinline__ float image_shift(int code, int dim, float side)
+{
+    return __builtin_fmaf((float)((code >> (2 * dim)) & 3), side, -side);
+}
+constexpr int kImageBits = 6;
 
 // current / next binning buffers (the parity of the completed-binning counter selects them)
 __device__ __forceinline__ int bin_parity(const DevView& v) { return v.par; }
@@ -161,10 +185,11 @@ hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustri
 // list_q >= 0: neighbour-list MD at that list parity; saves and restores also switch the list buffers
 hipError_t launch_save_restore(const DevView& v, int mode, int list_q, hipStream_t s);
 
-// neighbour-list MD: the three rebuild kernels (each a no-op unless lflag[q] is set), the FP32
+// neighbour-list MD: the rebuild (count, [scan,] build; each a no-op unless a chain's request flag of
+// parity q is set; cb = the count buffer, alternated by the host at every launch), the FP32
 // force sweep (LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT; with LJ_DRIFT the parity advances to q ^ 1),
 // a standalone drift (q -> q ^ 1) and the FP64 energy / virial / kinetic sums into tile_part
-hipError_t launch_list_rebuild(const DevView& v, int q, hipStream_t s);
+hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s);
 hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, hipStream_t s);
 hipError_t launch_list_drift(const DevView& v, int q, double dt, hipStream_t s);
 hipError_t launch_list_energy(const DevView& v, int q, hipStream_t s);

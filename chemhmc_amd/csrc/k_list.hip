@@ -6,16 +6,17 @@
 // a force evaluation between rebuilds is one sweep over ~(4/3) pi (rc + skin)^3 rho candidates per atom
 // instead of 27 cells' worth.
 //
-// Rebuild (four kernels that skip every chain whose request flag of the current list parity is clear,
-// so that they can sit in a captured trajectory and cost a launch when idle; chains rebuild
-// independently, so a chain of a batch evolves exactly as it does alone):
+// Rebuild (kernels that skip every chain whose request flag of the current list parity is clear, so
+// that they can sit in a captured trajectory and cost a launch when idle; chains rebuild independently,
+// so a chain of a batch evolves exactly as it does alone):
 //   k_lcount  atoms -> list cells (floor(x / lsize), clamped), arrival order into a dense member array
-//   k_lscan   one workgroup: exclusive scan of the cell counts -> first compact slot per cell
-//   k_lsort   one wavefront per list cell: members sorted by atom id give the compact slots; the cell
-//             writes its atoms' slot data (x0, r0 = x0 - cell origin, FP32 xsl)
-//   k_lbuild  one workgroup per list cell: the 27 neighbour cells' FP32 coordinates staged in LDS (image
-//             shift dd * lsize folded in, the way k_lj stages its j-list) and each own atom's list of
-//             the entries within rc + skin, in slot order
+//             that carries each member's FP32 cell-relative position
+//   k_lscan   only for chains of more than kScanCells cells: one workgroup scans the cell counts
+//   k_lbuild  one workgroup per list cell: (small chains: the counts scanned in LDS) the cell's members
+//             ranked by atom id give the compact slots and their slot data (x0, r0, FP32 xsl); the 27
+//             neighbour cells' members, ranked the same way, are staged in LDS in slot order with the
+//             image shift dd * lsize folded in (the way k_lj stages its j-list), and each own atom's
+//             list of the entries within rc + skin is appended in slot order
 // Sweep: 32 slots x LP lanes per workgroup (LP = 4, 8 or 16, tuned); each lane walks every LP-th list
 //   entry, gathers the neighbour's FP32 coordinates, and the partial sums meet in a butterfly.  Three
 //   lanes of an atom then kick p in FP64, optionally drift x in FP64 (identical_particles.cpp:651-673),
@@ -49,11 +50,17 @@ __device__ __forceinline__ void request(const DevView& v, int q, int chain)
     atomicOr(req_any(v, q), 1);
 }
 
-__global__ __launch_bounds__(256) void k_lcount(DevView v, int q)
+// Counts: two buffers lcount[cb][Cl]; the host alternates cb with every rebuild launch.  A launch
+// counts into buffer cb and clears buffer cb ^ 1 (the previous launch's counts), idle or not, so buffer
+// cb is zero whenever a launch starts and no later kernel has to clear the counts the build reads.
+// Each member is stored with its FP32 position relative to its cell origin (x - b * lsize rounded once,
+// the slot coordinates of the rebuild) and its atom id.
+__global__ __launch_bounds__(256) void k_lcount(DevView v, int q, int cb)
 {
-    if (*req_any(v, q) == 0) return;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= v.Ntot) return;
+    int* __restrict__ prev = v.lcount + (size_t)(cb ^ 1) * v.Cl;
+    for (int c = a; c < v.Cl; c += gridDim.x * blockDim.x) prev[c] = 0;
+    if (*req_any(v, q) == 0 || a >= v.Ntot) return;
     const int ch = v.atom_chain[a];
     if (*req(v, q, ch) == 0) return;
     const ChainDev& C = v.chains[ch];
@@ -61,29 +68,28 @@ __global__ __launch_bounds__(256) void k_lcount(DevView v, int q)
     const double x3[3] = { v.x[a], v.x[N + a], v.x[2 * N + a] };
     int b[3];
     const int c = list_cell(C, x3, b);
-    const int pos = atomicAdd(&v.lcount[c], 1);
-    if (pos < C.lcap) v.lmem[C.ldense_off + (c - C.lcell_off) * C.lcap + pos] = a;
+    float f3[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) f3[d] = (float)(x3[d] - (double)b[d] * C.lsize[d]);
+    const int pos = atomicAdd(&v.lcount[(size_t)cb * v.Cl + c], 1);
+    if (pos < C.lcap) v.lmem[C.ldense_off + (size_t)(c - C.lcell_off) * C.lcap + pos] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
     else atomicOr(&v.lflag[2], 1);
 }
 
-// global exclusive scan of the counts (zero in chains that did not count), then each rebuilding chain's
-// cells get atom_off + the scan relative to the chain's first cell; other chains keep their offsets
-__global__ __launch_bounds__(1024) void k_lscan(DevView v, int q)
+// chains with more list cells than kScanCells: one workgroup scans all counts (zero in chains that did
+// not count) and each rebuilding chain's cells get atom_off + the scan relative to the chain's first
+// cell, in the list offsets of the buffer being rebuilt
+__global__ __launch_bounds__(1024) void k_lscan(DevView v, int q, int cb)
 {
     if (*req_any(v, q) == 0) return;
-    __shared__ int any;
-    if (threadIdx.x == 0) any = 0;
-    __syncthreads();
-    for (int k = threadIdx.x; k < v.nchains; k += blockDim.x)
-        if (*req(v, q, k)) atomicAdd(&any, 1);
-    __syncthreads();
     __shared__ int wsum[16];
+    const int* __restrict__ cnt = v.lcount + (size_t)cb * v.Cl;
     const int nt = blockDim.x, tid = threadIdx.x;
     const int n = v.Cl;
     const int per = (n + nt - 1) / nt;
     const int beg = min(tid * per, n), end = min(beg + per, n);
     int s = 0;
-    for (int c = beg; c < end; c++) s += v.lcount[c];
+    for (int c = beg; c < end; c++) s += cnt[c];
     const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
     int incl = s;
 #pragma unroll
@@ -106,7 +112,7 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q)
     int run = incl - s + (wid > 0 ? wsum[wid - 1] : 0);
     for (int c = beg; c < end; c++) {
         v.lscan_tmp[c] = run;
-        run += v.lcount[c];
+        run += cnt[c];
     }
     __syncthreads();
     for (int c = beg; c < end; c++) {
@@ -114,140 +120,174 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q)
         if (*req(v, q, k) == 0) continue;
         const ChainDev& C = v.chains[k];
         list_buf(v, *list_start(v, k) ^ 1).loffs[c] = C.atom_off + v.lscan_tmp[c] - v.lscan_tmp[C.lcell_off];
-        v.lcount[c] = 0;
     }
-    if (tid == 0) v.lflag[3] += any;        // chain rebuilds run (statistics)
 }
 
-constexpr int kLMaxJ = 768;       // staged neighbour entries per pass
-constexpr int kLMaxI = 512;       // members of one list cell (lcap never exceeds this)
+constexpr int kLMaxJ = 768;       // staged neighbour members per pass (a cell never spans two passes)
 
-// one wavefront per list cell: members ranked by atom id give the compact slots; the cell writes its
-// atoms' slot data (x0, r0 = x0 - cell origin, FP32 coordinates of this parity, lperm, linv)
-__global__ __launch_bounds__(64) void k_lsort(DevView v, int q)
+// exclusive scan of n <= kScanCells values in LDS by one 256-thread workgroup
+__device__ __forceinline__ void block_scan_256(int* a, int n, int* wsum)
 {
-    if (*req_any(v, q) == 0) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = (n + 255) / 256;
+    const int beg = min(tid * per, n), end = min(beg + per, n);
+    int s = 0;
+    for (int i = beg; i < end; i++) s += a[i];
+    int incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int run = incl - s;
+    for (int w = 0; w < wave; w++) run += wsum[w];
+    for (int i = beg; i < end; i++) {
+        const int t = a[i];
+        a[i] = run;
+        run += t;
+    }
+    __syncthreads();
+}
+
+// One workgroup per list cell of a rebuilding chain; it writes the buffer that is not the trajectory's
+// start buffer and makes it current.
+//   offsets: SCAN = the chain's counts scanned in LDS (chains of at most kScanCells cells), else the
+//            offsets k_lscan wrote;
+//   own cell: members ranked by atom id give the compact slots; the cell writes its atoms' slot data
+//            (x0, r0 = x0 - cell origin, FP32 coordinates of parity q, lperm, linv, tile marks);
+//   lists:   the 27 neighbour cells' members, ranked the same way, are staged in LDS in slot order with
+//            the image shift dd * lsize folded in (the way k_lj stages its j-list), and each own atom
+//            appends the entries within rc + skin in staging order, so the list order is fixed by the
+//            slot order and not by the arrival order of the count.
+template <bool SCAN>
+__global__ __launch_bounds__(256) void k_lbuild(DevView v, int q, int cb)
+{
     const int c = blockIdx.x;
     const int ch = v.lcell_chain[c];
-    const ChainDev& C = v.chains[ch];
-    if (*req(v, q, ch) == 0) return;
-    __shared__ int sid[kLMaxI];
-    // a rebuild writes the buffer that is not the trajectory's start buffer, and makes it current
-    const int tb = *list_start(v, ch) ^ 1;
-    const ListBuf L = list_buf(v, tb);
-    if (threadIdx.x == 0) *list_cur(v, ch) = tb;
-    // member counts are clamped to the dense capacity: after an overflow (flagged by k_lcount) the
-    // missing atoms are simply absent and the host rejects the results
-    const int ibeg = L.loffs[c], ni = min(L.loffs[c + 1] - ibeg, C.lcap);
-    if (ni == 0) return;
-    const int lc = c - C.lcell_off;
-    const int nx = C.lnbin[0], ny = C.lnbin[1];
-    const int bz = lc / (nx * ny), by = (lc - bz * nx * ny) / nx, bx = lc - nx * (by + bz * ny);
-    const int ibase = C.ldense_off + lc * C.lcap;
-    for (int m = threadIdx.x; m < ni; m += 64) sid[m] = v.lmem[ibase + m];
-    __syncthreads();
-    const int N = v.Ntot;
-    const double o3[3] = { (double)bx * C.lsize[0], (double)by * C.lsize[1], (double)bz * C.lsize[2] };
-    for (int m = threadIdx.x; m < ni; m += 64) {
-        const int a = sid[m];
-        int r = 0;
-        for (int k = 0; k < ni; k++) r += sid[k] < a ? 1 : 0;
-        const int slot = ibeg + r;
-        float f3[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const double xv = v.x[d * N + a];
-            const double rv = xv - o3[d];
-            L.x0[d * N + slot] = xv;
-            L.r0[d * N + slot] = rv;
-            f3[d] = (float)rv;
-        }
-        L.xsl[(size_t)q * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
-        L.lperm[slot] = a;
-        L.linv[a] = slot;
-    }
-}
-
-// one workgroup per list cell: the 27 neighbour cells' sorted FP32 coordinates are staged in LDS with
-// the image shift folded in (the way k_lj stages its j-list) and each own atom appends the entries
-// within rc + skin in staging order, so the list order is fixed by the slot order
-__global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
-{
-    if (*req_any(v, q) == 0 || *req(v, q, v.lcell_chain[blockIdx.x]) == 0) return;
+    if (*req_any(v, q) == 0 || *req(v, q, ch) == 0) return;
     constexpr int IS = 4, JG = 16, NT = 256, W = 4;
-    __shared__ float4 sj[kLMaxJ];          // staged j: coordinates relative to the i-cell origin, slot in w
+    __shared__ float4 sj[kLMaxJ];          // staged j in slot order: coordinates relative to the i-cell origin, slot in w
+    __shared__ int sid[kLMaxJ];            // staged members' atom ids in arrival order (ranking)
     __shared__ unsigned char scode[kLMaxJ];
-    __shared__ float4 si[kLMaxI];          // own atoms by slot: FP32 coordinates, slot in w
-    __shared__ int icount[kLMaxI];
-    __shared__ int seg_beg[27], seg_pref[28];
+    // own atoms by slot (FP32 coordinates, slot in w) and their entry counts: lcap_max of each, in the
+    // dynamic LDS, so that the workgroup's LDS stays small enough for 8 or more resident per CU
+    extern __shared__ float4 dyn_lds[];
+    float4* si = dyn_lds;
+    int* icount = reinterpret_cast<int*>(dyn_lds + v.lcap_max);
+    __shared__ int seg_cell[27], seg_beg[27], seg_cnt[27], seg_pref[28], seg_code[27], wsum[4];
     __shared__ float seg_sh[27][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int c = blockIdx.x;
-    const int ch = v.lcell_chain[c];
     const ChainDev& C = v.chains[ch];
-    const ListBuf L = list_buf(v, *list_start(v, ch) ^ 1);
+    const int tb = *list_start(v, ch) ^ 1;
+    const ListBuf L = list_buf(v, tb);
     const int nx = C.lnbin[0], ny = C.lnbin[1], nz = C.lnbin[2];
     const int lc = c - C.lcell_off;
     const int bz = lc / (nx * ny), by = (lc - bz * nx * ny) / nx, bx = lc - nx * (by + bz * ny);
-    const int ibeg = L.loffs[c], ni = min(L.loffs[c + 1] - ibeg, C.lcap);
-    if (ni == 0) return;
+    const int* __restrict__ cnt = v.lcount + (size_t)cb * v.Cl + C.lcell_off;
+    const float4* __restrict__ mem = v.lmem + C.ldense_off;
     const int N = v.Ntot;
-    const float4* __restrict__ xq = L.xsl + (size_t)q * N;
-    const int nbcap = v.nbcap;
-
-    if (wave == 0) {
-        int cnt = 0;
-        if (lane < 27) {
-            const int dd[3] = { lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1 };
-            const int b3[3] = { bx, by, bz }, n3[3] = { nx, ny, nz };
-            int j3[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                j3[d] = b3[d] + dd[d];
-                if (j3[d] < 0 || j3[d] >= n3[d]) j3[d] = (b3[d] + n3[d] + dd[d]) % n3[d];
-                seg_sh[lane][d] = (float)dd[d] * C.lsize_f[d];
-            }
-            const int jc = C.lcell_off + j3[0] + nx * (j3[1] + j3[2] * ny);
-            seg_beg[lane] = L.loffs[jc];
-            cnt = min(L.loffs[jc + 1] - L.loffs[jc], C.lcap);
-        }
-        int incl = cnt;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const int t = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += t;
-        }
-        if (lane < 27) seg_pref[lane + 1] = incl;
-        if (lane == 0) seg_pref[0] = 0;
+    if (tid == 0) {
+        *list_cur(v, ch) = tb;
+        if (lc == 0) atomicAdd(&v.lflag[3], 1);        // chain rebuilds run (statistics)
     }
-    for (int r = tid; r < ni; r += NT) {
-        const float4 p = xq[ibeg + r];
-        si[r] = make_float4(p.x, p.y, p.z, __int_as_float(ibeg + r));
-        icount[r] = 0;
+    if (wave == 0 && lane < 27) {
+        const int dd[3] = { lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1 };
+        const int b3[3] = { bx, by, bz }, n3[3] = { nx, ny, nz };
+        int j3[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            j3[d] = b3[d] + dd[d];
+            if (j3[d] < 0 || j3[d] >= n3[d]) j3[d] = (b3[d] + n3[d] + dd[d]) % n3[d];
+            seg_sh[lane][d] = (float)dd[d] * C.lsize_f[d];
+        }
+        const int jl = j3[0] + nx * (j3[1] + j3[2] * ny);
+        seg_cell[lane] = jl;
+        seg_code[lane] = image_code(dd[0], dd[1], dd[2]);
+        if constexpr (!SCAN) {
+            seg_beg[lane] = L.loffs[C.lcell_off + jl];
+            seg_cnt[lane] = min(L.loffs[C.lcell_off + jl + 1] - L.loffs[C.lcell_off + jl], C.lcap);
+        } else {
+            seg_cnt[lane] = min(cnt[jl], C.lcap);
+        }
+    }
+    if constexpr (SCAN) {
+        int* scan = reinterpret_cast<int*>(sj);      // sj is free until the first pass
+        const int n = C.lncells;
+        for (int i = tid; i < n; i += NT) scan[i] = cnt[i];
+        __syncthreads();
+        block_scan_256(scan, n, wsum);
+        if (tid < 27) seg_beg[tid] = C.atom_off + scan[seg_cell[tid]];
     }
     __syncthreads();
-    const int nj = seg_pref[27];
+    if (tid == 0) {
+        int s = 0;
+        for (int k = 0; k < 27; k++) { seg_pref[k] = s; s += seg_cnt[k]; }
+        seg_pref[27] = s;
+    }
+    const int ni = seg_cnt[13], ibeg = seg_beg[13];     // segment 13: the cell itself
+    if (ni == 0) return;
+
+    // own cell: slots by atom id, slot data, FP32 coordinates of the i-atoms
+    const double o3[3] = { (double)bx * C.lsize[0], (double)by * C.lsize[1], (double)bz * C.lsize[2] };
+    for (int m = tid; m < ni; m += NT) sid[m] = __float_as_int(mem[(size_t)lc * C.lcap + m].w);
+    __syncthreads();
+    for (int m = tid; m < ni; m += NT) {
+        const float4 me = mem[(size_t)lc * C.lcap + m];
+        const int a = __float_as_int(me.w);
+        int r = 0;
+        for (int k = 0; k < ni; k++) r += sid[k] < a ? 1 : 0;
+        const int slot = ibeg + r;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double xv = v.x[d * N + a];
+            L.x0[d * N + slot] = xv;
+            L.r0[d * N + slot] = xv - o3[d];
+        }
+        L.xsl[(size_t)q * N + slot] = me;
+        L.lperm[slot] = a;
+        L.linv[a] = slot;
+        tile_cur_mark(v, C, slot, tb);
+        si[r] = make_float4(me.x, me.y, me.z, __int_as_float(slot));
+        icount[r] = 0;
+    }
+
     const float rl2 = C.rl2_f;
     const int ii = lane % IS, g = lane / IS;
     unsigned long long pattern = 0;
 #pragma unroll
     for (int k = 0; k < JG; k++) pattern |= 1ull << (k * IS + ii);
     const int nchunk = (ni + IS - 1) / IS;
-    for (int jlo = 0; jlo < nj; jlo += kLMaxJ) {
-        const int jhi = min(nj, jlo + kLMaxJ);
-        if (jlo > 0) __syncthreads();
-        for (int t = jlo + tid; t < jhi; t += NT) {
-            int k = 0;
+    const int nbcap = v.nbcap;
+    for (int k0 = 0; k0 < 27;) {
+        __syncthreads();                       // sid, sj free again; seg_pref written
+        int k1 = k0 + 1;
+        while (k1 < 27 && seg_pref[k1 + 1] - seg_pref[k0] <= kLMaxJ) k1++;
+        const int jlo = seg_pref[k0], jn = seg_pref[k1] - jlo;
+        auto seg_of = [&](int t) {             // segment of staged entry t (binary search over the prefix)
+            int k = k0;
 #pragma unroll
             for (int step = 16; step > 0; step >>= 1)
-                if (k + step < 27 && seg_pref[k + step] <= t) k += step;
-            const int slot = seg_beg[k] + (t - seg_pref[k]);
-            const float4 p = xq[slot];
-            sj[t - jlo] = make_float4(p.x + seg_sh[k][0], p.y + seg_sh[k][1], p.z + seg_sh[k][2], __int_as_float(slot));
-            scode[t - jlo] = (unsigned char)k;
+                if (k + step < k1 && seg_pref[k + step] <= jlo + t) k += step;
+            return k;
+        };
+        for (int t = tid; t < jn; t += NT) {
+            const int k = seg_of(t);
+            sid[t] = __float_as_int(mem[(size_t)seg_cell[k] * C.lcap + (jlo + t - seg_pref[k])].w);
         }
         __syncthreads();
-        const int jn = jhi - jlo;
+        for (int t = tid; t < jn; t += NT) {
+            const int k = seg_of(t);
+            const int s0 = seg_pref[k] - jlo, m = t - s0, nk = seg_cnt[k];
+            const int a = sid[t];
+            int r = 0;
+            for (int u = 0; u < nk; u++) r += sid[s0 + u] < a ? 1 : 0;
+            const float4 me = mem[(size_t)seg_cell[k] * C.lcap + m];
+            sj[s0 + r] = make_float4(me.x + seg_sh[k][0], me.y + seg_sh[k][1], me.z + seg_sh[k][2], __int_as_float(seg_beg[k] + r));
+            scode[s0 + r] = (unsigned char)seg_code[k];
+        }
+        __syncthreads();
         for (int chunk = wave; chunk < nchunk; chunk += W) {
             const int i = chunk * IS + ii;
             const bool ival = i < ni;
@@ -264,15 +304,16 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
                     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                     hit = r2 < rl2 && r2 > 0.f;
                 }
-                const unsigned long long m = __ballot(hit) & pattern;
+                const unsigned long long msk = __ballot(hit) & pattern;
                 if (hit) {
-                    const int pos = count + __popcll(m & ((1ull << lane) - 1ull));
-                    if (pos < nbcap) L.nb[(size_t)slot_i * nbcap + pos] = (__float_as_int(p.w) << 5) | (int)scode[j];
+                    const int pos = count + __popcll(msk & ((1ull << lane) - 1ull));
+                    if (pos < nbcap) L.nb[(size_t)slot_i * nbcap + pos] = (__float_as_int(p.w) << kImageBits) | (int)scode[j];
                 }
-                count += __popcll(m);
+                count += __popcll(msk);
             }
             if (ival && g == 0) icount[i] = count;
         }
+        k0 = k1;
     }
     __syncthreads();
     for (int r = tid; r < ni; r += NT) {
@@ -282,16 +323,19 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q)
     }
 }
 
-hipError_t launch_list_rebuild(const DevView& v, int q, hipStream_t s)
+hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lcount, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, q);
+    hipLaunchKernelGGL(k_lcount, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, q, cb);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_lscan, dim3(1), dim3(1024), 0, s, v, q);
+    const size_t dyn = (size_t)v.lcap_max * (sizeof(float4) + sizeof(int));
+    if (v.lscan_in_build) {
+        hipLaunchKernelGGL(k_lbuild<true>, dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_lscan, dim3(1), dim3(1024), 0, s, v, q, cb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_lsort, dim3(v.Cl), dim3(64), 0, s, v, q);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_lbuild, dim3(v.Cl), dim3(256), 0, s, v, q);
+    hipLaunchKernelGGL(k_lbuild<false>, dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
     return hipGetLastError();
 }
 
@@ -320,52 +364,41 @@ __device__ __forceinline__ void list_drift_atom(const DevView& v, const ListBuf&
     if (disp2 > C.half_skin2) request(v, q ^ 1, v.atom_chain[atom]);
 }
 
-__device__ __forceinline__ void load_shifts(float4* sh, const ChainDev& C)
-{
-    const int t = threadIdx.x;
-    if (t < 27)
-        sh[t] = make_float4((float)(t % 3 - 1) * C.lsize_f[0], (float)((t / 3) % 3 - 1) * C.lsize_f[1],
-                            (float)(t / 9 - 1) * C.lsize_f[2], 0.f);
-}
-
 // LP lanes per slot, kTile slots per workgroup.  Each lane first loads up to kBatch of its list entries
 // and gathers their coordinates, so that one memory latency covers most lists; the per-atom tail (kick,
-// drift, next-parity coordinates, displacement test) runs on three lanes, one per dimension.
+// drift, next-parity coordinates, displacement test) runs on three lanes, one per dimension.  The
+// workgroup's list buffer comes from tcur[blockIdx] and the image shifts are decoded from the entries in
+// registers, so nothing delays the first list load but the tile's own two scalar loads (no LDS table,
+// no barrier).
 constexpr int kBatch = 12;
 
 template <int MODE, int LP>
 __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q)
 {
-    __shared__ float4 sh[27];
     const Chunk tl = v.tiles[blockIdx.x];
+    const ListBuf L = list_buf(v, v.tcur[blockIdx.x]);
     const ChainDev& C = v.chains[tl.chain];
-    const ListBuf L = list_buf(v, *list_cur(v, tl.chain));
-    load_shifts(sh, C);
-    if (threadIdx.x == 0) {                               // this evaluation's rebuild is done
-        if (tl.begin == C.atom_off) *req(v, q, tl.chain) = 0;
-        if (blockIdx.x == 0) *req_any(v, q) = 0;
-    }
-    __syncthreads();
     const int N = v.Ntot;
     const int t = threadIdx.x / LP, sub = threadIdx.x % LP;
     const int s = tl.begin + t;
     const bool val = s < tl.end;
     const int d = sub < 3 ? sub : 0;                      // the per-dimension tail runs on lanes 0, 1, 2
     const float4* __restrict__ xq = L.xsl + (size_t)q * N;
-    const float rc2 = C.rc2_f, sig2 = C.sigma2_f;
     float ax = 0.f, ay = 0.f, az = 0.f;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
     int atom = 0;
     double p_d = 0.0, x_d = 0.0, x0_d = 0.0, r0_d = 0.0;
     auto pair = [&](const float4 pj, int code) {
-        const float4 shf = sh[code];
-        const float dx = xi.x - (pj.x + shf.x), dy = xi.y - (pj.y + shf.y), dz = xi.z - (pj.z + shf.z);
+        const float lx = C.lsize_f[0], ly = C.lsize_f[1], lz = C.lsize_f[2];
+        const float dx = xi.x - (pj.x + image_shift(code, 0, lx));
+        const float dy = xi.y - (pj.y + image_shift(code, 1, ly));
+        const float dz = xi.z - (pj.z + image_shift(code, 2, lz));
         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         const float inv = __builtin_amdgcn_rcpf(r2);
-        const float s2 = sig2 * inv;
+        const float s2 = C.sigma2_f * inv;
         const float s6 = s2 * s2 * s2;
         float gq = s6 * (0.5f - s6) * inv;            // sr6 (0.5 - sr6) / r^2, identical_particles.cpp:611-613
-        gq = (r2 < rc2 && r2 > 0.f) ? gq : 0.f;
+        gq = (r2 < C.rc2_f && r2 > 0.f) ? gq : 0.f;
         ax = __builtin_fmaf(gq, dx, ax);
         ay = __builtin_fmaf(gq, dy, ay);
         az = __builtin_fmaf(gq, dz, az);
@@ -376,10 +409,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         // slots, or zero: the list buffer is cleared at setup) so that both arrive in one round trip
         int e[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-            const int k = sub + u * LP;
-            e[u] = k < v.nbcap ? nbs[k] : 0;
-        }
+        for (int u = 0; u < kBatch; u++) e[u] = nbs[min(sub + u * LP, v.nbcap - 1)];
         atom = L.lperm[s];
         xi = xq[s];
         const int n = L.nbn[s];
@@ -394,13 +424,13 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         }
         float4 pj[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) pj[u] = xq[e[u] >> 5];
+        for (int u = 0; u < kBatch; u++) pj[u] = xq[e[u] >> kImageBits];
 #pragma unroll
         for (int u = 0; u < kBatch; u++)
-            if (sub + u * LP < n) pair(pj[u], e[u] & 31);
+            if (sub + u * LP < n) pair(pj[u], e[u] & 63);
         for (int k = sub + kBatch * LP; k < n; k += LP) {
             const int ee = nbs[k];
-            pair(xq[ee >> 5], ee & 31);
+            pair(xq[ee >> kImageBits], ee & 63);
         }
     }
 #pragma unroll
@@ -439,6 +469,10 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
             L.xsl[(size_t)(q ^ 1) * N + s] = make_float4(rel, ry, rz, __int_as_float(atom));
             if (e * e + ey * ey + ez * ez > C.half_skin2) request(v, q ^ 1, tl.chain);
         }
+    }
+    if (threadIdx.x == 0) {                               // this evaluation's rebuild is done
+        if (tl.begin == C.atom_off) *req(v, q, tl.chain) = 0;
+        if (blockIdx.x == 0) *req_any(v, q) = 0;
     }
 }
 
@@ -511,10 +545,10 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
         const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
         for (int k = sub; k < n; k += 8) {
             const int e = nbs[k];
-            const int j = e >> 5, code = e & 31;
-            const double dx = xi - (v.rel64[j] + (double)(code % 3 - 1) * C.lsize[0]);
-            const double dy = yi - (v.rel64[N + j] + (double)((code / 3) % 3 - 1) * C.lsize[1]);
-            const double dz = zi - (v.rel64[2 * N + j] + (double)(code / 9 - 1) * C.lsize[2]);
+            const int j = e >> kImageBits, code = e & 63;
+            const double dx = xi - (v.rel64[j] + (double)((code & 3) - 1) * C.lsize[0]);
+            const double dy = yi - (v.rel64[N + j] + (double)(((code >> 2) & 3) - 1) * C.lsize[1]);
+            const double dz = zi - (v.rel64[2 * N + j] + (double)(((code >> 4) & 3) - 1) * C.lsize[2]);
             const double r2 = dx * dx + dy * dy + dz * dz;
             if (sqrt(r2) < C.rc && r2 > 0.0) {                 // identical_particles.cpp:432-435
                 const double s2 = C.sigma2 / r2;

@@ -193,6 +193,7 @@ __global__ __launch_bounds__(256) void k_save_restore(DevView v, int mode, int l
     }
     const int b = *list_start(v, ch);
     if (a == C.atom_off) *list_cur(v, ch) = b;
+    tile_cur_mark(v, C, a, b);            // every tile of the chain is marked by one of its threads
     const ListBuf L = list_buf(v, b);
     const int slot = L.linv[a];
     float f3[3];

@@ -228,16 +228,17 @@ def test_integrator_fp32_config2(chmc, orc):
     assert np.max(dx) < 2e-4, np.max(dx)
 
 
-@pytest.mark.parametrize("N,rho,Lb", [(32768, 0.8442, None), (125, 0.58, None), (51, 0.8, None),
-                                      (100, None, (8.0, 6.0, 4.5))])
+@pytest.mark.parametrize("N,rho,Lb", [(32768, 0.8442, None), (131072, 0.8442, None), (125, 0.58, None),
+                                      (51, 0.8, None), (100, None, (8.0, 6.0, 4.5))])
 def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
     """Neighbour-list MD (skin lists, rebuilt on displacement) against the 27-cell sweep with a
     re-binning per evaluation and against the FP64 oracle: 10 OMF2 steps from the same start.  Both FP32
     paths differ from the oracle only by FP32 rounding (same bars as the FP32 integrator test); the
     small boxes put one or two list cells along a side, where the lists hold several images of a pair
     (the anisotropic box holds 100 atoms: its 300-atom lattice of the force tests, at rho = 1.39 with
-    0.64 sigma spacing, blows up under OMF2 dt = 0.01 in any arithmetic)."""
-    T = 0.722 if N == 32768 else 1.0
+    0.64 sigma spacing, blows up under OMF2 dt = 0.01 in any arithmetic).  N = 131072 has 19^3 list
+    cells, more than one build workgroup scans itself, so its rebuilds take the separate scan kernel."""
+    T = 0.722 if N >= 32768 else 1.0
     e, L = _eng(chmc, N, rho, Lb, T=T)
     ec, _ = _eng(chmc, N, rho, Lb, T=T, cell_md=True)
     e.heatbath(9, 1)
@@ -264,7 +265,16 @@ def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
     ec.compute_force()
     fl, fc = e.force(), ec.force()
     frms = np.sqrt(np.mean(fc ** 2))
-    assert np.max(np.abs(fl - fc)) < 5e-5 * frms
+    # the two FP32 paths round positions relative to different cell origins, so a pair within ~1e-6 of
+    # r_c can count on one path and not on the other: the truncated force then differs by |F(r_c)|
+    # (0.039 at r_c = 2.5) on both atoms of that pair.  Such atoms are rare; every other atom agrees to
+    # FP32 summation-order rounding.
+    fcut = abs(48.0 * RC ** -6 * (RC ** -6 - 0.5) / RC)
+    dev = np.max(np.abs(fl - fc), axis=1)
+    odd = np.flatnonzero(dev >= 5e-5 * frms)
+    print(f"force: max deviation {dev.max():.2e} (rms force {frms:.2f}); atoms beyond 5e-5 rms: {len(odd)}")
+    assert len(odd) <= max(2, N // 20000), odd
+    assert np.all(dev[odd] < 2 * fcut + 5e-5 * frms), dev[odd]
 
 
 @pytest.mark.parametrize("N,rho,nb", [(512, 0.8, 200), (4000, 0.8442, 500)])
