@@ -277,7 +277,7 @@ def main():
         e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
     # skin autotuner: wall time of a short run per skin on the equilibrated state
     skins = {}
-    for k, sk in enumerate((0.2, 0.3, 0.4, 0.5)):
+    for k, sk in enumerate((0.3, 0.4, 0.5, 0.6, 0.7, 0.8)):
         e.set_skin(sk)
         e.hmc(4, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=20 + k, use_graph=not args.no_graph)
         t0 = time.perf_counter()

@@ -692,7 +692,7 @@ static int list_setup(chmc_ctx* c, double skin)
         const double rho = cd.natoms / (cd.L[0] * cd.L[1] * cd.L[2]) * cd.sigma * cd.sigma * cd.sigma;
         const double dens = std::max(1.5, 1.5 * rho);
         cd.lcap = (int)std::ceil(dens * vol) + 16;
-        if (cd.lcap > 512)
+        if (cd.lcap > 256)                          // one build thread per member of a cell
             return fail(c, CHMC_E_GEOMETRY, "chain " + std::to_string(k) + ": list cells too large for the list sweep (use CHMC_CELL_MD)");
         const double rls = rl / cd.sigma;
         nbmax = std::max(nbmax, 4.0 / 3.0 * M_PI * rls * rls * rls * dens + 16.0);
@@ -730,6 +730,8 @@ static int list_setup(chmc_ctx* c, double skin)
     // the per-slot list state twice (current / trajectory-start buffers)
     HIPCHK(c, lalloc(c, &v.loffs, 2 * ((size_t)lcells + 1)));
     HIPCHK(c, lalloc(c, &v.lmem, (size_t)dense));
+    HIPCHK(c, lalloc(c, &v.lmemx, 3 * (size_t)dense));
+    v.ldense = (int)dense;
     HIPCHK(c, lalloc(c, &v.lperm, 2 * N));
     HIPCHK(c, lalloc(c, &v.linv, 2 * N));
     HIPCHK(c, lalloc(c, &v.nb, 2 * N * nbcap));

@@ -25,6 +25,7 @@
 //   loffs         : int [Cl + 1]              (exclusive scan: first compact slot of each list cell)
 //   lmem          : float4 [sum_chains lncells * lcap]  (members of each cell in arrival order: FP32
 //                                              position relative to the cell origin, atom id in w)
+//   lmemx         : double [3][same]          (the members' FP64 positions, for the rebuild reference)
 //   lperm / linv  : int [Ntot]                (compact slot -> atom / atom -> slot, atom-id order
 //                                              inside a cell)
 //   x0, r0        : double SoA [3][Ntot] by slot (position at the rebuild; minus its cell origin)
@@ -109,8 +110,10 @@ struct DevView {
     const Chunk* tiles;
     int *lcount, *loffs, *lperm, *linv, *nb, *nbn, *lflag, *lreq, *lscan_tmp;
     float4* lmem;
+    double* lmemx;        // [3][dense]
+    int ldense;           // member slots of all chains (sum of lncells * lcap)
     int lscan_in_build;   // every chain has at most kScanCells list cells: the build scans the counts
-    int lcap_max;         // largest list-cell member capacity over the chains (at most 512)
+    int lcap_max;         // largest list-cell member capacity over the chains (at most 256)
     int* lbuf;            // [2][nchains]: current list buffer, buffer of the trajectory's start
     int* tcur;            // [ntiles]: current list buffer of each tile's chain (kept equal to lbuf)
     double *x0, *r0, *rel64;

@@ -72,8 +72,14 @@ __global__ __launch_bounds__(256) void k_lcount(DevView v, int q, int cb)
 #pragma unroll
     for (int d = 0; d < 3; d++) f3[d] = (float)(x3[d] - (double)b[d] * C.lsize[d]);
     const int pos = atomicAdd(&v.lcount[(size_t)cb * v.Cl + c], 1);
-    if (pos < C.lcap) v.lmem[C.ldense_off + (size_t)(c - C.lcell_off) * C.lcap + pos] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
-    else atomicOr(&v.lflag[2], 1);
+    if (pos < C.lcap) {
+        const size_t m = C.ldense_off + (size_t)(c - C.lcell_off) * C.lcap + pos;
+        v.lmem[m] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
+#pragma unroll
+        for (int d = 0; d < 3; d++) v.lmemx[(size_t)d * v.ldense + m] = x3[d];
+    } else {
+        atomicOr(&v.lflag[2], 1);
+    }
 }
 
 // chains with more list cells than kScanCells: one workgroup scans all counts (zero in chains that did
@@ -123,7 +129,7 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q, int cb)
     }
 }
 
-constexpr int kLMaxJ = 768;       // staged neighbour members per pass (a cell never spans two passes)
+constexpr int kLMaxJ = 1024;      // staged neighbour members per pass (a cell never spans two passes)
 
 // exclusive scan of n <= kScanCells values in LDS by one 256-thread workgroup
 __device__ __forceinline__ void block_scan_256(int* a, int n, int* wsum)
@@ -155,27 +161,32 @@ __device__ __forceinline__ void block_scan_256(int* a, int n, int* wsum)
 // start buffer and makes it current.
 //   offsets: SCAN = the chain's counts scanned in LDS (chains of at most kScanCells cells), else the
 //            offsets k_lscan wrote;
-//   own cell: members ranked by atom id give the compact slots; the cell writes its atoms' slot data
-//            (x0, r0 = x0 - cell origin, FP32 coordinates of parity q, lperm, linv, tile marks);
-//   lists:   the 27 neighbour cells' members, ranked the same way, are staged in LDS in slot order with
-//            the image shift dd * lsize folded in (the way k_lj stages its j-list), and each own atom
-//            appends the entries within rc + skin in staging order, so the list order is fixed by the
-//            slot order and not by the arrival order of the count.
+//   staging: the 27 neighbour cells' members (FP32 cell-relative positions, loaded once into
+//            registers), ranked by atom id inside each cell, are staged in LDS in slot order with the
+//            image shift dd * lsize folded in (the way k_lj stages its j-list);
+//   own cell: the ranks of its members give their compact slots and slot data (x0 and r0 = x0 - cell
+//            origin from the FP64 positions the count stored with the members, FP32 coordinates of
+//            parity q, lperm, linv, tile marks);
+//   lists:   each own atom appends the entries within rc + skin in staging order, so the list order
+//            is fixed by the slot order and not by the arrival order of the count.
+// A neighbourhood of more than kLMaxJ members is staged in passes of whole cells; the own cell is
+// loaded with the first pass.
 template <bool SCAN>
-__global__ __launch_bounds__(256) void k_lbuild(DevView v, int q, int cb)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_lbuild(DevView v, int q, int cb)
 {
     const int c = blockIdx.x;
     const int ch = v.lcell_chain[c];
     if (*req_any(v, q) == 0 || *req(v, q, ch) == 0) return;
-    constexpr int IS = 4, JG = 16, NT = 256, W = 4;
+    constexpr int IS = 4, JG = 16, NT = 256, W = 4, PT = kLMaxJ / NT;
     __shared__ float4 sj[kLMaxJ];          // staged j in slot order: coordinates relative to the i-cell origin, slot in w
     __shared__ int sid[kLMaxJ];            // staged members' atom ids in arrival order (ranking)
     __shared__ unsigned char scode[kLMaxJ];
-    // own atoms by slot (FP32 coordinates, slot in w) and their entry counts: lcap_max of each, in the
-    // dynamic LDS, so that the workgroup's LDS stays small enough for 8 or more resident per CU
+    // own atoms by slot (FP32 coordinates, slot in w), their entry counts and their ids in arrival
+    // order: lcap_max of each, in the dynamic LDS, so that the workgroup's LDS stays small
     extern __shared__ float4 dyn_lds[];
     float4* si = dyn_lds;
     int* icount = reinterpret_cast<int*>(dyn_lds + v.lcap_max);
+    int* sido = icount + v.lcap_max;
     __shared__ int seg_cell[27], seg_beg[27], seg_cnt[27], seg_pref[28], seg_code[27], wsum[4];
     __shared__ float seg_sh[27][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -187,6 +198,8 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q, int cb)
     const int bz = lc / (nx * ny), by = (lc - bz * nx * ny) / nx, bx = lc - nx * (by + bz * ny);
     const int* __restrict__ cnt = v.lcount + (size_t)cb * v.Cl + C.lcell_off;
     const float4* __restrict__ mem = v.lmem + C.ldense_off;
+    const double* __restrict__ memx = v.lmemx + C.ldense_off;
+    const size_t xstride = (size_t)v.ldense;
     const int N = v.Ntot;
     if (tid == 0) {
         *list_cur(v, ch) = tb;
@@ -226,32 +239,9 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q, int cb)
         for (int k = 0; k < 27; k++) { seg_pref[k] = s; s += seg_cnt[k]; }
         seg_pref[27] = s;
     }
-    const int ni = seg_cnt[13], ibeg = seg_beg[13];     // segment 13: the cell itself
-    if (ni == 0) return;
-
-    // own cell: slots by atom id, slot data, FP32 coordinates of the i-atoms
-    const double o3[3] = { (double)bx * C.lsize[0], (double)by * C.lsize[1], (double)bz * C.lsize[2] };
-    for (int m = tid; m < ni; m += NT) sid[m] = __float_as_int(mem[(size_t)lc * C.lcap + m].w);
     __syncthreads();
-    for (int m = tid; m < ni; m += NT) {
-        const float4 me = mem[(size_t)lc * C.lcap + m];
-        const int a = __float_as_int(me.w);
-        int r = 0;
-        for (int k = 0; k < ni; k++) r += sid[k] < a ? 1 : 0;
-        const int slot = ibeg + r;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const double xv = v.x[d * N + a];
-            L.x0[d * N + slot] = xv;
-            L.r0[d * N + slot] = xv - o3[d];
-        }
-        L.xsl[(size_t)q * N + slot] = me;
-        L.lperm[slot] = a;
-        L.linv[a] = slot;
-        tile_cur_mark(v, C, slot, tb);
-        si[r] = make_float4(me.x, me.y, me.z, __int_as_float(slot));
-        icount[r] = 0;
-    }
+    const int ni = seg_cnt[13], ibeg = seg_beg[13];     // segment 13: the cell itself (ni <= lcap <= 256)
+    if (ni == 0) return;
 
     const float rl2 = C.rl2_f;
     const int ii = lane % IS, g = lane / IS;
@@ -261,31 +251,72 @@ __global__ __launch_bounds__(256) void k_lbuild(DevView v, int q, int cb)
     const int nchunk = (ni + IS - 1) / IS;
     const int nbcap = v.nbcap;
     for (int k0 = 0; k0 < 27;) {
-        __syncthreads();                       // sid, sj free again; seg_pref written
+        const bool first = k0 == 0;
+        if (!first) __syncthreads();           // sid, sj free again
         int k1 = k0 + 1;
         while (k1 < 27 && seg_pref[k1 + 1] - seg_pref[k0] <= kLMaxJ) k1++;
         const int jlo = seg_pref[k0], jn = seg_pref[k1] - jlo;
-        auto seg_of = [&](int t) {             // segment of staged entry t (binary search over the prefix)
-            int k = k0;
+        // with the first pass: the own cell's members and their FP64 positions (one per thread)
+        float4 meo = make_float4(0.f, 0.f, 0.f, 0.f);
+        double xo[3] = { 0.0, 0.0, 0.0 };
+        if (first && tid < ni) {
+            const size_t mi = (size_t)lc * C.lcap + tid;
+            meo = mem[mi];
 #pragma unroll
-            for (int step = 16; step > 0; step >>= 1)
-                if (k + step < k1 && seg_pref[k + step] <= jlo + t) k += step;
-            return k;
-        };
-        for (int t = tid; t < jn; t += NT) {
-            const int k = seg_of(t);
-            sid[t] = __float_as_int(mem[(size_t)seg_cell[k] * C.lcap + (jlo + t - seg_pref[k])].w);
+            for (int d = 0; d < 3; d++) xo[d] = memx[d * xstride + mi];
+            sido[tid] = __float_as_int(meo.w);
+        }
+        // members of this pass, loaded once (PT per thread), ids to LDS for the ranking
+        float4 me[PT];
+        int kk[PT];
+#pragma unroll
+        for (int u = 0; u < PT; u++) {
+            const int t = tid + NT * u;
+            kk[u] = k0;
+            me[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (t < jn) {
+                int k = k0;
+#pragma unroll
+                for (int step = 16; step > 0; step >>= 1)
+                    if (k + step < k1 && seg_pref[k + step] <= jlo + t) k += step;
+                kk[u] = k;
+                me[u] = mem[(size_t)seg_cell[k] * C.lcap + (jlo + t - seg_pref[k])];
+                sid[t] = __float_as_int(me[u].w);
+            }
         }
         __syncthreads();
-        for (int t = tid; t < jn; t += NT) {
-            const int k = seg_of(t);
-            const int s0 = seg_pref[k] - jlo, m = t - s0, nk = seg_cnt[k];
-            const int a = sid[t];
+        if (first && tid < ni) {
+            // own slot data: slots by atom id; x0 and r0 = x0 - cell origin in FP64
+            const int a = __float_as_int(meo.w);
             int r = 0;
-            for (int u = 0; u < nk; u++) r += sid[s0 + u] < a ? 1 : 0;
-            const float4 me = mem[(size_t)seg_cell[k] * C.lcap + m];
-            sj[s0 + r] = make_float4(me.x + seg_sh[k][0], me.y + seg_sh[k][1], me.z + seg_sh[k][2], __int_as_float(seg_beg[k] + r));
-            scode[s0 + r] = (unsigned char)seg_code[k];
+            for (int k = 0; k < ni; k++) r += sido[k] < a ? 1 : 0;
+            const int slot = ibeg + r;
+            const double o3[3] = { (double)bx * C.lsize[0], (double)by * C.lsize[1], (double)bz * C.lsize[2] };
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                L.x0[d * N + slot] = xo[d];
+                L.r0[d * N + slot] = xo[d] - o3[d];
+            }
+            L.xsl[(size_t)q * N + slot] = meo;
+            L.lperm[slot] = a;
+            L.linv[a] = slot;
+            tile_cur_mark(v, C, slot, tb);
+            si[r] = make_float4(meo.x, meo.y, meo.z, __int_as_float(slot));
+            icount[r] = 0;
+        }
+#pragma unroll
+        for (int u = 0; u < PT; u++) {
+            const int t = tid + NT * u;
+            if (t < jn) {
+                const int k = kk[u];
+                const int s0 = seg_pref[k] - jlo, nk = seg_cnt[k];
+                const int a = __float_as_int(me[u].w);
+                int r = 0;
+                for (int w = 0; w < nk; w++) r += sid[s0 + w] < a ? 1 : 0;
+                sj[s0 + r] = make_float4(me[u].x + seg_sh[k][0], me[u].y + seg_sh[k][1], me[u].z + seg_sh[k][2],
+                                         __int_as_float(seg_beg[k] + r));
+                scode[s0 + r] = (unsigned char)seg_code[k];
+            }
         }
         __syncthreads();
         for (int chunk = wave; chunk < nchunk; chunk += W) {
@@ -328,7 +359,7 @@ hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s)
     hipLaunchKernelGGL(k_lcount, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, q, cb);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const size_t dyn = (size_t)v.lcap_max * (sizeof(float4) + sizeof(int));
+    const size_t dyn = (size_t)v.lcap_max * (sizeof(float4) + 2 * sizeof(int));
     if (v.lscan_in_build) {
         hipLaunchKernelGGL(k_lbuild<true>, dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
         return hipGetLastError();
