@@ -61,6 +61,8 @@ struct chmc_ctx {
     int cpar = 0;                // list count buffer of the next rebuild launch (alternates per launch)
     bool list_valid = false;     // the lists and their rebuild flags account for every position change
     bool saved_lists = false;    // the last save kept a valid list buffer with the positions
+    bool lstate = false;         // list MD owns the state: xl, pl, xl_saved (slot order) are current and
+                                 // x, p, x_saved (atom order) are not; false at every API return
     std::vector<void*> lallocs;
     std::vector<Chunk> htiles;
 };
@@ -666,6 +668,7 @@ static hipError_t lalloc(chmc_ctx* c, T** p, size_t n)
 // rc + skin), 32-slot tiles, and the list buffers
 static int list_setup(chmc_ctx* c, double skin)
 {
+    if (c->lstate) return fail(c, CHMC_E_STATE, "internal: list buffers replaced while list MD owns the state");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (void* p : c->lallocs) hipFree(p);
     c->lallocs.clear();
@@ -731,6 +734,10 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, lalloc(c, &v.loffs, 2 * ((size_t)lcells + 1)));
     HIPCHK(c, lalloc(c, &v.lmem, (size_t)dense));
     HIPCHK(c, lalloc(c, &v.lmemx, 3 * (size_t)dense));
+    HIPCHK(c, lalloc(c, &v.lmemp, 3 * (size_t)dense));
+    HIPCHK(c, lalloc(c, &v.xl, 3 * N));
+    HIPCHK(c, lalloc(c, &v.pl, 3 * N));
+    HIPCHK(c, lalloc(c, &v.xl_saved, 3 * N));
     v.ldense = (int)dense;
     HIPCHK(c, lalloc(c, &v.lperm, 2 * N));
     HIPCHK(c, lalloc(c, &v.linv, 2 * N));
@@ -796,6 +803,27 @@ static int list_setup(chmc_ctx* c, double skin)
 static int list_ensure(chmc_ctx* c)
 {
     return c->list_ready ? CHMC_OK : list_setup(c, c->skin);
+}
+
+// list MD takes the state over: x, p in the current buffer's slot order (xl, pl); h_x is taken over by
+// the next save
+static int list_import(chmc_ctx* c)
+{
+    if (c->lstate) return CHMC_OK;
+    int rc = list_ensure(c);
+    if (rc) return rc;
+    HIPCHK(c, launch_list_import(c->v, c->stream));
+    c->lstate = true;
+    return CHMC_OK;
+}
+
+// hand the state back in atom order; saved: also h_x (a device HMC loop's last save)
+static int list_export(chmc_ctx* c, bool saved)
+{
+    if (!c->lstate) return CHMC_OK;
+    HIPCHK(c, launch_list_export(c->v, saved, c->stream));
+    c->lstate = false;
+    return CHMC_OK;
 }
 
 // the rebuild kernels of the current list parity (a no-op for chains without a request); every launch
@@ -905,10 +933,10 @@ static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double l
     std::vector<MdOp> ops;
     if (build_ops(scheme, dt, steps, lambda, ops) != 0) return fail(c, CHMC_E_ARG, "unknown integrator scheme");
     if (list_md(c) && !c->md_surr) {
-        // neighbour-list MD from a fresh rebuild of the starting positions
+        // neighbour-list MD on the slot-ordered state
         int rc = flush_drift(c);
         if (rc) return rc;
-        if ((rc = list_ensure(c))) return rc;
+        if (!c->lstate) return fail(c, CHMC_E_STATE, "internal: list MD without the slot-ordered state");
         if (!c->list_valid && (rc = list_invalidate(c))) return rc;
         c->list_valid = true;
         for (size_t k = 0; k < ops.size(); k++) {
@@ -951,8 +979,10 @@ extern "C" int chmc_integrate(chmc_ctx* c, int scheme, double dt, int steps, dou
     if (steps < 1 || !scheme_ok(scheme)) return fail(c, CHMC_E_ARG, "integrate: steps must be >= 1 and the scheme known");
     int rc = flush_drift(c);
     if (rc) return rc;
+    if (list_md(c) && (rc = list_import(c))) return rc;
     rc = integrate_seq(c, scheme, dt, steps, lambda, true);
     if (rc) return rc;
+    if ((rc = list_export(c, false))) return rc;
     rc = flush_drift(c);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -963,7 +993,7 @@ extern "C" int chmc_integrate(chmc_ctx* c, int scheme, double dt, int steps, dou
 extern "C" int chmc_heatbath(chmc_ctx* c, uint64_t seed, uint64_t traj)
 {
     CHECK_CTX(c);
-    HIPCHK(c, launch_heatbath(c->v, seed, (long long)traj, false, c->stream));
+    HIPCHK(c, launch_heatbath(c->v, seed, (long long)traj, false, false, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
 }
@@ -1071,23 +1101,23 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
             // rebuilt buffer the start buffer of this trajectory
             if ((rc = list_invalidate(c))) return rc;
             HIPCHK(c, list_rebuild(c));
-            HIPCHK(c, launch_save_restore(c->v, 0, c->lpar, c->stream));
+            HIPCHK(c, launch_save_restore_slots(c->v, 0, c->lpar, c->stream));
             c->list_valid = true;
         }
-        HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, false, c->stream));
+        HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, false, true, c->stream));
         if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false))) return rc;
         if (steps == 0 && !c->list_valid && (rc = list_invalidate(c))) return rc;
         c->list_valid = true;
         if ((rc = list_energy(c, false))) return rc;
         HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, 0, SUM_TILES,
                                     c->stream));
-        HIPCHK(c, launch_save_restore(c->v, 2, c->lpar, c->stream));
+        HIPCHK(c, launch_save_restore_slots(c->v, 2, c->lpar, c->stream));
         c->force_kind = 0;
         c->bin_state = 2;
         return CHMC_OK;
     }
     // heat bath + K_i + the count phase of the binning of the (accepted or restored) positions
-    HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, true, c->stream));
+    HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, true, false, c->stream));
     c->bin_state = 1;
     // a zero-length binomial draw leaves x and p unchanged (the reference's size_t loop would not end, Q18)
     if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false))) return rc;
@@ -1223,8 +1253,9 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     HIPCHK(c, hipMemcpyAsync(c->d_uniforms, u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice, c->stream));
 
     // ---- V_i, W_i of the start configuration (HMC.cpp:64), h_x = x (HMC.cpp:72) ----
-    if (list_md(c) && !hp->surrogate) {
-        if ((rc = list_ensure(c)) || (!c->list_valid && (rc = list_invalidate(c))) || (rc = list_energy(c, true))) return rc;
+    const bool slot_run = list_md(c) && !hp->surrogate;      // the run starts on the slot-ordered state
+    if (slot_run) {
+        if ((rc = list_import(c)) || (!c->list_valid && (rc = list_invalidate(c))) || (rc = list_energy(c, true))) return rc;
         c->list_valid = true;
     } else if ((rc = do_energy(c, true))) {
         return rc;
@@ -1239,8 +1270,12 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
         st[k].Wi = red[(size_t)k * RED_NSLOT + RED_W];
     }
     HIPCHK(c, hipMemcpyAsync(c->v.state, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice, c->stream));
-    const bool lists = list_md(c) && c->list_ready && c->list_valid;
-    HIPCHK(c, launch_save_restore(c->v, 0, lists ? c->lpar : -1, c->stream));
+    if (slot_run) {
+        HIPCHK(c, launch_save_restore_slots(c->v, 0, c->lpar, c->stream));
+    } else {
+        const bool lists = list_md(c) && c->list_ready && c->list_valid;
+        HIPCHK(c, launch_save_restore(c->v, 0, lists ? c->lpar : -1, c->stream));
+    }
 
     // ---- trajectories, in windows when the surrogate drives the MD (acceptance-based fallback) ----
     std::vector<TrajGraph> graphs;
@@ -1249,6 +1284,17 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     std::vector<HmcRecord> wrec;
     for (int t0 = 0; t0 < ntraj && !rc; t0 += window) {
         const int count = std::min(window, ntraj - t0);
+        if (list_md(c)) {
+            // surrogate MD runs on atom order (x, p, x_saved); list MD on the slot order, from a rebuild
+            // and a save of the positions the surrogate trajectories left
+            if (c->md_surr) {
+                rc = list_export(c, true);
+            } else if (!c->lstate) {
+                c->list_valid = false;
+                rc = list_import(c);
+            }
+            if (rc) break;
+        }
         rc = run_segment(c, *hp, t0, count, ufirst, rec_base, rsteps, graphs);
         if (rc || !c->md_surr) continue;
         if ((rc = surr_check_flag(c))) break;
@@ -1264,6 +1310,8 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     for (TrajGraph& g : graphs) hipGraphExecDestroy(g.exec);
     c->md_surr = false;
     if (rc) return rc;
+    if ((rc = list_export(c, true))) return rc;
+    c->saved_lists = false;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (list_md(c) && (rc = list_check_overflow(c))) return rc;
     c->traj_done += ntraj;
@@ -1674,7 +1722,7 @@ extern "C" int chmc_list_stats(chmc_ctx* c, int chain, long long* entries, long 
     if (!c || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
     int rc = flush_drift(c);
     if (rc) return rc;
-    if ((rc = list_ensure(c)) || (rc = list_invalidate(c)) || (rc = list_energy(c, true))) return rc;
+    if ((rc = list_import(c)) || (rc = list_invalidate(c)) || (rc = list_energy(c, true)) || (rc = list_export(c, false))) return rc;
     std::vector<double> nintv(c->nchains);
     if ((rc = read_red(c, RED_NINT, nintv.data()))) return rc;
     const ChainDev& cd = c->hchains[chain];
@@ -1732,7 +1780,7 @@ extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* m
     if (fp64_md(c)) return fail(c, CHMC_E_STATE, "time_list: the list sweep is the FP32 path");
     int rc = flush_drift(c);
     if (rc) return rc;
-    if ((rc = list_ensure(c))) return rc;
+    if ((rc = list_import(c))) return rc;
     // rebuilds: `reps` forced rebuilds back to back (request flags set by a fill kernel inside the span)
     if ((rc = list_invalidate(c))) return rc;
     HIPCHK(c, list_rebuild(c));            // warm-up
@@ -1749,6 +1797,7 @@ extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* m
         c->lpar ^= 1;
     }
     HIPCHK(c, hipEventRecord(c->ev2, c->stream));
+    if ((rc = list_export(c, false))) return rc;
     HIPCHK(c, hipEventSynchronize(c->ev2));
     float a = 0.f, b = 0.f;
     HIPCHK(c, hipEventElapsedTime(&a, c->ev0, c->ev1));

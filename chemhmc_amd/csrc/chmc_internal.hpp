@@ -25,7 +25,13 @@
 //   loffs         : int [Cl + 1]              (exclusive scan: first compact slot of each list cell)
 //   lmem          : float4 [sum_chains lncells * lcap]  (members of each cell in arrival order: FP32
 //                                              position relative to the cell origin, atom id in w)
-//   lmemx         : double [3][same]          (the members' FP64 positions, for the rebuild reference)
+//   lmemx, lmemp  : double [3][same]          (the members' FP64 positions and momenta)
+//   xl, pl        : double SoA [3][Ntot] by slot  (positions and momenta in the current buffer's slot
+//                                              order while list MD owns the state: the sweeps read
+//                                              and write them coalesced; imported from / exported to
+//                                              x and p (atom order) at the boundaries of an MD run)
+//   xl_saved      : double SoA [3][Ntot] by slot  (h_x of the device HMC loop, in the slot order of
+//                                              the trajectory's start buffer)
 //   lperm / linv  : int [Ntot]                (compact slot -> atom / atom -> slot, atom-id order
 //                                              inside a cell)
 //   x0, r0        : double SoA [3][Ntot] by slot (position at the rebuild; minus its cell origin)
@@ -110,7 +116,8 @@ struct DevView {
     const Chunk* tiles;
     int *lcount, *loffs, *lperm, *linv, *nb, *nbn, *lflag, *lreq, *lscan_tmp;
     float4* lmem;
-    double* lmemx;        // [3][dense]
+    double *lmemx, *lmemp; // [3][dense]
+    double *xl, *pl, *xl_saved;
     int ldense;           // member slots of all chains (sum of lncells * lcap)
     int lscan_in_build;   // every chain has at most kScanCells list cells: the build scans the counts
     int lcap_max;         // largest list-cell member capacity over the chains (at most 256)
@@ -182,11 +189,17 @@ hipError_t launch_kick(const DevView& v, double dt, bool fp64, hipStream_t s);
 // tiling: i-slots per wave (4, 8, 16) x waves per workgroup (1, 2, 4), encoded islots * 10 + waves
 hipError_t launch_lj(const DevView& v, int mode, bool fp64, int tiling, double kick_dt, double drift_dt, hipStream_t s);
 hipError_t launch_kinetic(const DevView& v, hipStream_t s);
-hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, hipStream_t s);
+// slots: write the momenta in the current list buffer's slot order (pl) instead of atom order (p)
+hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, bool slots, hipStream_t s);
 hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustride, int therm, int ufirst,
                              HmcRecord* rec, int rec_base, int surr, int what, hipStream_t s);
 // list_q >= 0: neighbour-list MD at that list parity; saves and restores also switch the list buffers
 hipError_t launch_save_restore(const DevView& v, int mode, int list_q, hipStream_t s);
+// the same on the slot-ordered state (xl, xl_saved) of list MD, at list parity list_q
+hipError_t launch_save_restore_slots(const DevView& v, int mode, int list_q, hipStream_t s);
+// x, p (and x_saved from xl_saved) <-> xl, pl in the current list buffer's slot order
+hipError_t launch_list_import(const DevView& v, hipStream_t s);
+hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s);
 
 // neighbour-list MD: the rebuild (count, [scan,] build; each a no-op unless a chain's request flag of
 // parity q is set; cb = the count buffer, alternated by the host at every launch), the FP32

@@ -54,18 +54,20 @@ __device__ __forceinline__ void request(const DevView& v, int q, int chain)
 // counts into buffer cb and clears buffer cb ^ 1 (the previous launch's counts), idle or not, so buffer
 // cb is zero whenever a launch starts and no later kernel has to clear the counts the build reads.
 // Each member is stored with its FP32 position relative to its cell origin (x - b * lsize rounded once,
-// the slot coordinates of the rebuild) and its atom id.
+// the slot coordinates of the rebuild) and its atom id, and its FP64 position and momentum.  The count
+// runs over the slots of the current buffer (the slot-ordered state xl, pl).
 __global__ __launch_bounds__(256) void k_lcount(DevView v, int q, int cb)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     int* __restrict__ prev = v.lcount + (size_t)(cb ^ 1) * v.Cl;
-    for (int c = a; c < v.Cl; c += gridDim.x * blockDim.x) prev[c] = 0;
-    if (*req_any(v, q) == 0 || a >= v.Ntot) return;
-    const int ch = v.atom_chain[a];
+    for (int c = s; c < v.Cl; c += gridDim.x * blockDim.x) prev[c] = 0;
+    if (*req_any(v, q) == 0 || s >= v.Ntot) return;
+    const int ch = v.atom_chain[s];               // a chain's slots span the same range as its atoms
     if (*req(v, q, ch) == 0) return;
     const ChainDev& C = v.chains[ch];
     const int N = v.Ntot;
-    const double x3[3] = { v.x[a], v.x[N + a], v.x[2 * N + a] };
+    const int a = list_buf(v, *list_cur(v, ch)).lperm[s];
+    const double x3[3] = { v.xl[s], v.xl[N + s], v.xl[2 * N + s] };
     int b[3];
     const int c = list_cell(C, x3, b);
     float f3[3];
@@ -76,7 +78,10 @@ __global__ __launch_bounds__(256) void k_lcount(DevView v, int q, int cb)
         const size_t m = C.ldense_off + (size_t)(c - C.lcell_off) * C.lcap + pos;
         v.lmem[m] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
 #pragma unroll
-        for (int d = 0; d < 3; d++) v.lmemx[(size_t)d * v.ldense + m] = x3[d];
+        for (int d = 0; d < 3; d++) {
+            v.lmemx[(size_t)d * v.ldense + m] = x3[d];
+            v.lmemp[(size_t)d * v.ldense + m] = v.pl[d * N + s];
+        }
     } else {
         atomicOr(&v.lflag[2], 1);
     }
@@ -199,6 +204,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int* __restrict__ cnt = v.lcount + (size_t)cb * v.Cl + C.lcell_off;
     const float4* __restrict__ mem = v.lmem + C.ldense_off;
     const double* __restrict__ memx = v.lmemx + C.ldense_off;
+    const double* __restrict__ memp = v.lmemp + C.ldense_off;
     const size_t xstride = (size_t)v.ldense;
     const int N = v.Ntot;
     if (tid == 0) {
@@ -258,12 +264,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         const int jlo = seg_pref[k0], jn = seg_pref[k1] - jlo;
         // with the first pass: the own cell's members and their FP64 positions (one per thread)
         float4 meo = make_float4(0.f, 0.f, 0.f, 0.f);
-        double xo[3] = { 0.0, 0.0, 0.0 };
+        double xo[3] = { 0.0, 0.0, 0.0 }, po[3] = { 0.0, 0.0, 0.0 };
         if (first && tid < ni) {
             const size_t mi = (size_t)lc * C.lcap + tid;
             meo = mem[mi];
 #pragma unroll
-            for (int d = 0; d < 3; d++) xo[d] = memx[d * xstride + mi];
+            for (int d = 0; d < 3; d++) {
+                xo[d] = memx[d * xstride + mi];
+                po[d] = memp[d * xstride + mi];
+            }
             sido[tid] = __float_as_int(meo.w);
         }
         // members of this pass, loaded once (PT per thread), ids to LDS for the ranking
@@ -296,6 +305,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             for (int d = 0; d < 3; d++) {
                 L.x0[d * N + slot] = xo[d];
                 L.r0[d * N + slot] = xo[d] - o3[d];
+                v.xl[d * N + slot] = xo[d];
+                v.pl[d * N + slot] = po[d];
             }
             L.xsl[(size_t)q * N + slot] = meo;
             L.lperm[slot] = a;
@@ -373,7 +384,7 @@ hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s)
 // ---- sweep ---------------------------------------------------------------------------------------
 // drift of one atom by dt (FP64, wrap into [0, L)), its FP32 list coordinates of the next parity and
 // the rebuild request when it is beyond skin / 2 from its rebuild position
-__device__ __forceinline__ void list_drift_atom(const DevView& v, const ListBuf& L, const ChainDev& C, int atom,
+__device__ __forceinline__ void list_drift_slot(const DevView& v, const ListBuf& L, const ChainDev& C, int ch,
                                                 int slot, int q, double dt, const double pn[3])
 {
     const int N = v.Ntot;
@@ -382,17 +393,17 @@ __device__ __forceinline__ void list_drift_atom(const DevView& v, const ListBuf&
     double disp2 = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double xv = v.x[d * N + atom];
+        double xv = v.xl[d * N + slot];
         xv += dt * coeff_x * pn[d];
         xv -= C.L[d] * floor(xv / C.L[d]);
-        v.x[d * N + atom] = xv;
+        v.xl[d * N + slot] = xv;
         double e = xv - L.x0[d * N + slot];
         e -= C.L[d] * rint(e / C.L[d]);
         disp2 += e * e;
         f3[d] = (float)(L.r0[d * N + slot] + e);
     }
-    L.xsl[(size_t)(q ^ 1) * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(atom));
-    if (disp2 > C.half_skin2) request(v, q ^ 1, v.atom_chain[atom]);
+    L.xsl[(size_t)(q ^ 1) * N + slot] = make_float4(f3[0], f3[1], f3[2], 0.f);
+    if (disp2 > C.half_skin2) request(v, q ^ 1, ch);
 }
 
 // LP lanes per slot, kTile slots per workgroup.  Each lane first loads up to kBatch of its list entries
@@ -441,14 +452,14 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         int e[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; u++) e[u] = nbs[min(sub + u * LP, v.nbcap - 1)];
-        atom = L.lperm[s];
+        if constexpr ((MODE & LJ_STOREF) != 0) atom = L.lperm[s];     // the stored force is in atom order
         xi = xq[s];
         const int n = L.nbn[s];
         // the tail's operands are fetched now, alongside the list, not after the butterfly
         if (sub < 3) {
-            p_d = v.p[d * N + atom];
+            p_d = v.pl[d * N + s];
             if constexpr ((MODE & LJ_DRIFT) != 0) {
-                x_d = v.x[d * N + atom];
+                x_d = v.xl[d * N + s];
                 x0_d = L.x0[d * N + s];
                 r0_d = L.r0[d * N + s];
             }
@@ -479,14 +490,14 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         if constexpr ((MODE & LJ_KICK) != 0) {
             const double kdt = kick_dt * C.beta;       // dt * coeff_p, coeff_p = beta (:642-644)
             pn = pn - kdt * (double)fd;
-            v.p[d * N + atom] = pn;
+            v.pl[d * N + s] = pn;
         }
         if constexpr ((MODE & LJ_DRIFT) != 0) {
             const double coeff_x = C.beta / C.mass;    // compute_coeff_position, identical_particles.cpp:646-648
             double xv = x_d;
             xv += drift_dt * coeff_x * pn;
             xv -= C.L[d] * floor(xv / C.L[d]);
-            v.x[d * N + atom] = xv;
+            v.xl[d * N + s] = xv;
             e = xv - x0_d;
             e -= C.L[d] * rint(e / C.L[d]);
             rel = (float)(r0_d + e);
@@ -497,7 +508,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         const float ry = __shfl(rel, base + 1, 64), rz = __shfl(rel, base + 2, 64);
         const double ey = __shfl(e, base + 1, 64), ez = __shfl(e, base + 2, 64);
         if (val && sub == 0) {
-            L.xsl[(size_t)(q ^ 1) * N + s] = make_float4(rel, ry, rz, __int_as_float(atom));
+            L.xsl[(size_t)(q ^ 1) * N + s] = make_float4(rel, ry, rz, 0.f);
             if (e * e + ey * ey + ez * ez > C.half_skin2) request(v, q ^ 1, tl.chain);
         }
     }
@@ -514,7 +525,6 @@ __global__ __launch_bounds__(256) void k_ldrift(DevView v, double dt, int q)
     const int ch = v.atom_chain[s];          // a chain's slots span the same range as its atoms
     const ChainDev& C = v.chains[ch];
     const ListBuf L = list_buf(v, *list_cur(v, ch));
-    const int atom = L.lperm[s];
     if (s == 0) *req_any(v, q) = 0;
     if (s == C.atom_off) {
         // a pending request of parity q carries over to q ^ 1 (nothing consumed it)
@@ -523,8 +533,8 @@ __global__ __launch_bounds__(256) void k_ldrift(DevView v, double dt, int q)
         if (pend) request(v, q ^ 1, ch);
     }
     const int N = v.Ntot;
-    const double pn[3] = { v.p[atom], v.p[N + atom], v.p[2 * N + atom] };
-    list_drift_atom(v, L, C, atom, s, q, dt, pn);
+    const double pn[3] = { v.pl[s], v.pl[N + s], v.pl[2 * N + s] };
+    list_drift_slot(v, L, C, ch, s, q, dt, pn);
 }
 
 // ---- FP64 energy ---------------------------------------------------------------------------------
@@ -535,11 +545,10 @@ __global__ __launch_bounds__(256) void k_lrel64(DevView v)
     const int ch = v.atom_chain[s];
     const ChainDev& C = v.chains[ch];
     const ListBuf L = list_buf(v, *list_cur(v, ch));
-    const int atom = L.lperm[s];
     const int N = v.Ntot;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        double e = v.x[d * N + atom] - L.x0[d * N + s];
+        double e = v.xl[d * N + s] - L.x0[d * N + s];
         e -= C.L[d] * rint(e / C.L[d]);
         v.rel64[d * N + s] = L.r0[d * N + s] + e;
     }
@@ -590,8 +599,7 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
             }
         }
         if (sub == 0) {
-            const int atom = L.lperm[s];
-            const double px = v.p[atom], py = v.p[N + atom], pz = v.p[2 * N + atom];
+            const double px = v.pl[s], py = v.pl[N + s], pz = v.pl[2 * N + s];
             ak = (px * px + py * py + pz * pz) / (2 * C.mass);
         }
     }
@@ -603,6 +611,52 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
         double* tp = &v.tile_part[4 * blockIdx.x];
         tp[0] = au; tp[1] = aw; tp[2] = an; tp[3] = ak;
     }
+}
+
+// ---- slot-ordered state <-> atom order ------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_limport(DevView v)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.Ntot) return;
+    const int a = list_buf(v, *list_cur(v, v.atom_chain[s])).lperm[s];
+    const int N = v.Ntot;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        v.xl[d * N + s] = v.x[d * N + a];
+        v.pl[d * N + s] = v.p[d * N + a];
+    }
+}
+
+// saved: also h_x (xl_saved, in the start buffer's slot order) into x_saved
+__global__ __launch_bounds__(256) void k_lexport(DevView v, int saved)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.Ntot) return;
+    const int ch = v.atom_chain[s];
+    const int a = list_buf(v, *list_cur(v, ch)).lperm[s];
+    const int N = v.Ntot;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        v.x[d * N + a] = v.xl[d * N + s];
+        v.p[d * N + a] = v.pl[d * N + s];
+    }
+    if (saved) {
+        const int as = list_buf(v, *list_start(v, ch)).lperm[s];
+#pragma unroll
+        for (int d = 0; d < 3; d++) v.x_saved[d * N + as] = v.xl_saved[d * N + s];
+    }
+}
+
+hipError_t launch_list_import(const DevView& v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_limport, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lexport, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, saved ? 1 : 0);
+    return hipGetLastError();
 }
 
 // ---- launchers -----------------------------------------------------------------------------------

@@ -46,8 +46,9 @@ __global__ __launch_bounds__(256) void k_kinetic(DevView v)
 }
 
 // heat bath p ~ N(0, m/beta) (identical_particles.cpp:287-296), K_i partials of the fresh momenta and,
-// optionally, the count phase of the binning of the unchanged positions
-__global__ __launch_bounds__(256) void k_heatbath(DevView v, uint64_t seed, long long traj_explicit, int count)
+// optionally, the count phase of the binning of the unchanged positions.  With slots the chunk runs
+// over list slots: the draw of the slot's atom (the same Philox key) goes to pl in slot order.
+__global__ __launch_bounds__(256) void k_heatbath(DevView v, uint64_t seed, long long traj_explicit, int count, int slots)
 {
     const Chunk ck = v.chunks[blockIdx.x];
     const int ch = ck.chain;
@@ -57,16 +58,19 @@ __global__ __launch_bounds__(256) void k_heatbath(DevView v, uint64_t seed, long
     const double twopi = 6.283185307179586;
     const int N = v.Ntot;
     double s = 0.0;
-    for (int a = ck.begin + (int)threadIdx.x; a < ck.end; a += 256) {
+    const int* lperm = slots ? list_buf(v, *list_cur(v, ch)).lperm : nullptr;
+    double* pout = slots ? v.pl : v.p;
+    for (int i = ck.begin + (int)threadIdx.x; i < ck.end; i += 256) {
+        const int a = slots ? lperm[i] : i;       // i: the atom, or the slot of atom a
         const u64x4 r = philox4x64_10((uint64_t)(a - C.atom_off), traj, (uint64_t)ch, 0, seed, kKeyHeatbath);
         const double r0 = sqrt(-2.0 * log(u01_oo(r.v[0])));
         const double a0 = twopi * u01_co(r.v[1]);
         const double r1 = sqrt(-2.0 * log(u01_oo(r.v[2])));
         const double a1 = twopi * u01_co(r.v[3]);
         const double px = (r0 * cos(a0)) * sd, py = (r0 * sin(a0)) * sd, pz = (r1 * cos(a1)) * sd;
-        v.p[a] = px;
-        v.p[N + a] = py;
-        v.p[2 * N + a] = pz;
+        pout[i] = px;
+        pout[N + i] = py;
+        pout[2 * N + i] = pz;
         s += (px * px + py * py + pz * pz) / (2 * C.mass);
         if (count) {
             const double x3[3] = { v.x[a], v.x[N + a], v.x[2 * N + a] };
@@ -206,6 +210,41 @@ __global__ __launch_bounds__(256) void k_save_restore(DevView v, int mode, int l
     L.xsl[(size_t)list_q * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
 }
 
+// The same on the slot-ordered state of list MD: xl_saved <- xl on save (the current list buffer becomes
+// the start buffer), xl <- xl_saved on restore (the start buffer becomes current again, and its FP32
+// coordinates of parity list_q are recomputed for the restored positions)
+__global__ __launch_bounds__(256) void k_save_restore_slots(DevView v, int mode, int list_q)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == 2 && s == 0) *v.traj = *v.traj + 1;
+    if (s >= v.Ntot) return;
+    const int ch = v.atom_chain[s];               // a chain's slots span the same range as its atoms
+    const ChainDev& C = v.chains[ch];
+    int save = mode == 0;
+    if (mode == 2) save = v.state[ch].accepted;
+    const int N = v.Ntot;
+    if (save) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) v.xl_saved[d * N + s] = v.xl[d * N + s];
+        if (s == C.atom_off) *list_start(v, ch) = *list_cur(v, ch);
+        return;
+    }
+    const int b = *list_start(v, ch);
+    if (s == C.atom_off) *list_cur(v, ch) = b;
+    tile_cur_mark(v, C, s, b);
+    const ListBuf L = list_buf(v, b);
+    float f3[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double xv = v.xl_saved[d * N + s];
+        v.xl[d * N + s] = xv;
+        double e = xv - L.x0[d * N + s];
+        e -= C.L[d] * rint(e / C.L[d]);
+        f3[d] = (float)(L.r0[d * N + s] + e);
+    }
+    L.xsl[(size_t)list_q * N + s] = make_float4(f3[0], f3[1], f3[2], 0.f);
+}
+
 // Standalone kick with the stored force: p -= (dt*beta) f  (identical_particles.cpp:686-688)
 __global__ __launch_bounds__(256) void k_kick(DevView v, double dt, int fp64)
 {
@@ -221,9 +260,9 @@ __global__ __launch_bounds__(256) void k_kick(DevView v, double dt, int fp64)
     }
 }
 
-hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, hipStream_t s)
+hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, bool slots, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_heatbath, dim3(v.nchunks), dim3(256), 0, s, v, seed, traj_or_neg, count ? 1 : 0);
+    hipLaunchKernelGGL(k_heatbath, dim3(v.nchunks), dim3(256), 0, s, v, seed, traj_or_neg, count ? 1 : 0, slots ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -250,6 +289,12 @@ hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustri
 hipError_t launch_save_restore(const DevView& v, int mode, int list_q, hipStream_t s)
 {
     hipLaunchKernelGGL(k_save_restore, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, mode, list_q);
+    return hipGetLastError();
+}
+
+hipError_t launch_save_restore_slots(const DevView& v, int mode, int list_q, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_save_restore_slots, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, mode, list_q);
     return hipGetLastError();
 }
 
