@@ -689,7 +689,15 @@ static int list_setup(chmc_ctx* c, double skin)
             cd.lsize_f[d] = (float)cd.lsize[d];
             vol *= cd.lsize[d] / cd.sigma;
         }
-        cd.half_skin2 = 0.25 * sk * sk;
+        // displacements are measured from the FP32 rebuild reference (within lsize * 2^-24 per
+        // coordinate of the FP64 one): the trigger sits that much below skin / 2, so it is never late
+        double lmax = 0.0;
+        for (int d = 0; d < 3; d++) {
+            lmax = std::max(lmax, cd.lsize[d]);
+            if (cd.lnbin[d] > 1023) return fail(c, CHMC_E_GEOMETRY, "chain " + std::to_string(k) + ": more than 1023 list cells along a side");
+        }
+        const double half = std::max(0.0, 0.5 * sk - 4.0e-7 * lmax);
+        cd.half_skin2 = half * half;
         cd.rl2_f = (float)(rl * rl * (1.0 + 1e-5));   // a margin over the FP32 rounding of r^2 at the rebuild
         // capacities from a local density bound of max(1.5 / sigma^3, 1.5 x the chain's mean density)
         const double rho = cd.natoms / (cd.L[0] * cd.L[1] * cd.L[2]) * cd.sigma * cd.sigma * cd.sigma;
@@ -748,8 +756,7 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, lalloc(c, &v.lflag, 4));
     HIPCHK(c, lalloc(c, &v.lreq, 2 * ((size_t)c->nchains + 1)));
     HIPCHK(c, lalloc(c, &v.lscan_tmp, (size_t)lcells));
-    HIPCHK(c, lalloc(c, &v.x0, 2 * 3 * N));
-    HIPCHK(c, lalloc(c, &v.r0, 2 * 3 * N));
+    HIPCHK(c, lalloc(c, &v.lref, 2 * N));
     HIPCHK(c, lalloc(c, &v.rel64, 3 * N));
     HIPCHK(c, lalloc(c, &v.xsl, 2 * 2 * N));
     HIPCHK(c, lalloc(c, &v.tile_part, 4 * (size_t)v.ntiles));
@@ -788,8 +795,7 @@ static int list_setup(chmc_ctx* c, double skin)
         for (int q = 0; q < 2; q++)
             HIPCHK(c, hipMemcpy(v.xsl + (2 * b + q) * N, xs0.data(), sizeof(float4) * N, hipMemcpyHostToDevice));
     }
-    HIPCHK(c, hipMemset(v.x0, 0, sizeof(double) * 2 * 3 * N));
-    HIPCHK(c, hipMemset(v.r0, 0, sizeof(double) * 2 * 3 * N));
+    HIPCHK(c, hipMemset(v.lref, 0, sizeof(float4) * 2 * N));
     HIPCHK(c, hipDeviceSynchronize());
     c->skin = skin;
     c->lpar = 0;

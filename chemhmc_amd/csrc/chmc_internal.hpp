@@ -34,7 +34,9 @@
 //                                              the trajectory's start buffer)
 //   lperm / linv  : int [Ntot]                (compact slot -> atom / atom -> slot, atom-id order
 //                                              inside a cell)
-//   x0, r0        : double SoA [3][Ntot] by slot (position at the rebuild; minus its cell origin)
+//   lref          : float4 [Ntot] by slot     (rebuild reference: the FP32 position relative to the
+//                                              rebuild cell origin, r0, and the cell's grid coordinates
+//                                              bx | by << 10 | bz << 20 in w)
 //   xsl           : float4 [2][Ntot] by slot  (current position relative to the rebuild cell origin,
 //                                              FP32, atom id in w; double-buffered across drifts)
 //   nb            : int [Ntot][nbcap]         (j slot << 6 | image code: 2 bits per dimension, the
@@ -44,7 +46,7 @@
 //                                              rebuild before its next sweep; chains rebuild
 //                                              independently)
 //   lflag         : int [4]                   ([2]: capacity overflow, [3]: chain rebuilds run)
-//   The per-slot list state (loffs, lperm, linv, x0, r0, xsl, nb, nbn) exists twice: a rebuild writes
+//   The per-slot list state (loffs, lperm, linv, lref, xsl, nb, nbn) exists twice: a rebuild writes
 //   the buffer that is not the trajectory's start buffer, so a rejected trajectory goes back to the
 //   start buffer (valid for the restored positions) instead of rebuilding.  lbuf[k] / lbuf[nchains + k]
 //   are chain k's current and start buffers; tcur[t] repeats the current buffer of tile t's chain, so
@@ -123,7 +125,8 @@ struct DevView {
     int lcap_max;         // largest list-cell member capacity over the chains (at most 256)
     int* lbuf;            // [2][nchains]: current list buffer, buffer of the trajectory's start
     int* tcur;            // [ntiles]: current list buffer of each tile's chain (kept equal to lbuf)
-    double *x0, *r0, *rel64;
+    float4* lref;
+    double* rel64;
     float4* xsl;
     double* tile_part;    // [ntiles * 4]: energy, virial, interacting pairs, kinetic energy
 };
@@ -133,7 +136,7 @@ enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8, LJ_DRIFT 
 // one of the two neighbour-list buffers
 struct ListBuf {
     int *nb, *nbn, *lperm, *linv, *loffs;
-    double *x0, *r0;      // [3][Ntot] by slot
+    float4* lref;         // [Ntot] by slot
     float4* xsl;          // [2 parities][Ntot] by slot
 };
 __device__ __forceinline__ ListBuf list_buf(const DevView& v, int b)
@@ -145,8 +148,7 @@ __device__ __forceinline__ ListBuf list_buf(const DevView& v, int b)
     L.lperm = v.lperm + (size_t)b * N;
     L.linv = v.linv + (size_t)b * N;
     L.loffs = v.loffs + (size_t)b * ((size_t)v.Cl + 1);
-    L.x0 = v.x0 + (size_t)b * 3 * N;
-    L.r0 = v.r0 + (size_t)b * 3 * N;
+    L.lref = v.lref + (size_t)b * N;
     L.xsl = v.xsl + (size_t)b * 2 * N;
     return L;
 }
@@ -167,6 +169,22 @@ __device__ __forceinline__ float image_shift(int code, int dim, float side)
     return __builtin_fmaf((float)((code >> (2 * dim)) & 3), side, -side);
 }
 constexpr int kImageBits = 6;
+
+// FP64 position of coordinate x (dimension d) relative to the rebuild cell origin of reference ref,
+// taken through the periodic image nearest the rebuild position r0 (an atom moves far less than L / 2
+// between rebuilds, so the image is the one of the same box or of a neighbour box)
+__device__ __forceinline__ double list_rel(const ChainDev& C, const float4 ref, int d, double x)
+{
+    const int cell = __float_as_int(ref.w);
+    const double o = (double)((cell >> (10 * d)) & 1023) * C.lsize[d];
+    const float r0 = d == 0 ? ref.x : (d == 1 ? ref.y : ref.z);
+    double r = x - o;
+    const double diff = r - (double)r0;
+    if (diff > 0.5 * C.L[d]) r -= C.L[d];
+    else if (diff < -0.5 * C.L[d]) r += C.L[d];
+    return r;
+}
+__device__ __forceinline__ float ref_coord(const float4 ref, int d) { return d == 0 ? ref.x : (d == 1 ? ref.y : ref.z); }
 
 // current / next binning buffers (the parity of the completed-binning counter selects them)
 __device__ __forceinline__ int bin_parity(const DevView& v) { return v.par; }

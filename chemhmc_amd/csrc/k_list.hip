@@ -295,19 +295,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         }
         __syncthreads();
         if (first && tid < ni) {
-            // own slot data: slots by atom id; x0 and r0 = x0 - cell origin in FP64
+            // own slot data: slots by atom id; the rebuild reference (FP32 cell-relative position, cell)
             const int a = __float_as_int(meo.w);
             int r = 0;
             for (int k = 0; k < ni; k++) r += sido[k] < a ? 1 : 0;
             const int slot = ibeg + r;
-            const double o3[3] = { (double)bx * C.lsize[0], (double)by * C.lsize[1], (double)bz * C.lsize[2] };
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                L.x0[d * N + slot] = xo[d];
-                L.r0[d * N + slot] = xo[d] - o3[d];
                 v.xl[d * N + slot] = xo[d];
                 v.pl[d * N + slot] = po[d];
             }
+            L.lref[slot] = make_float4(meo.x, meo.y, meo.z, __int_as_float(bx | (by << 10) | (bz << 20)));
             L.xsl[(size_t)q * N + slot] = meo;
             L.lperm[slot] = a;
             L.linv[a] = slot;
@@ -391,16 +389,17 @@ __device__ __forceinline__ void list_drift_slot(const DevView& v, const ListBuf&
     const double coeff_x = C.beta / C.mass;       // compute_coeff_position, identical_particles.cpp:646-648
     float f3[3];
     double disp2 = 0.0;
+    const float4 ref = L.lref[slot];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         double xv = v.xl[d * N + slot];
         xv += dt * coeff_x * pn[d];
         xv -= C.L[d] * floor(xv / C.L[d]);
         v.xl[d * N + slot] = xv;
-        double e = xv - L.x0[d * N + slot];
-        e -= C.L[d] * rint(e / C.L[d]);
+        const double r = list_rel(C, ref, d, xv);
+        const double e = r - (double)ref_coord(ref, d);
         disp2 += e * e;
-        f3[d] = (float)(L.r0[d * N + slot] + e);
+        f3[d] = (float)r;
     }
     L.xsl[(size_t)(q ^ 1) * N + slot] = make_float4(f3[0], f3[1], f3[2], 0.f);
     if (disp2 > C.half_skin2) request(v, q ^ 1, ch);
@@ -429,7 +428,8 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
     float ax = 0.f, ay = 0.f, az = 0.f;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
     int atom = 0;
-    double p_d = 0.0, x_d = 0.0, x0_d = 0.0, r0_d = 0.0;
+    double p_d = 0.0, x_d = 0.0;
+    float4 ref = make_float4(0.f, 0.f, 0.f, 0.f);
     auto pair = [&](const float4 pj, int code) {
         const float lx = C.lsize_f[0], ly = C.lsize_f[1], lz = C.lsize_f[2];
         const float dx = xi.x - (pj.x + image_shift(code, 0, lx));
@@ -460,8 +460,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
             p_d = v.pl[d * N + s];
             if constexpr ((MODE & LJ_DRIFT) != 0) {
                 x_d = v.xl[d * N + s];
-                x0_d = L.x0[d * N + s];
-                r0_d = L.r0[d * N + s];
+                ref = L.lref[s];
             }
         }
         float4 pj[kBatch];
@@ -498,9 +497,9 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
             xv += drift_dt * coeff_x * pn;
             xv -= C.L[d] * floor(xv / C.L[d]);
             v.xl[d * N + s] = xv;
-            e = xv - x0_d;
-            e -= C.L[d] * rint(e / C.L[d]);
-            rel = (float)(r0_d + e);
+            const double r = list_rel(C, ref, d, xv);
+            e = r - (double)ref_coord(ref, d);
+            rel = (float)r;
         }
     }
     if constexpr ((MODE & LJ_DRIFT) != 0) {
@@ -546,12 +545,9 @@ __global__ __launch_bounds__(256) void k_lrel64(DevView v)
     const ChainDev& C = v.chains[ch];
     const ListBuf L = list_buf(v, *list_cur(v, ch));
     const int N = v.Ntot;
+    const float4 ref = L.lref[s];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        double e = v.xl[d * N + s] - L.x0[d * N + s];
-        e -= C.L[d] * rint(e / C.L[d]);
-        v.rel64[d * N + s] = L.r0[d * N + s] + e;
-    }
+    for (int d = 0; d < 3; d++) v.rel64[d * N + s] = list_rel(C, ref, d, v.xl[d * N + s]);
 }
 
 __device__ __forceinline__ double tile_sum_256(double s, double* part)

@@ -201,12 +201,9 @@ __global__ __launch_bounds__(256) void k_save_restore(DevView v, int mode, int l
     const ListBuf L = list_buf(v, b);
     const int slot = L.linv[a];
     float f3[3];
+    const float4 ref = L.lref[slot];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        double e = v.x_saved[d * N + a] - L.x0[d * N + slot];
-        e -= C.L[d] * rint(e / C.L[d]);
-        f3[d] = (float)(L.r0[d * N + slot] + e);
-    }
+    for (int d = 0; d < 3; d++) f3[d] = (float)list_rel(C, ref, d, v.x_saved[d * N + a]);
     L.xsl[(size_t)list_q * N + slot] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
 }
 
@@ -234,13 +231,12 @@ __global__ __launch_bounds__(256) void k_save_restore_slots(DevView v, int mode,
     tile_cur_mark(v, C, s, b);
     const ListBuf L = list_buf(v, b);
     float f3[3];
+    const float4 ref = L.lref[s];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const double xv = v.xl_saved[d * N + s];
         v.xl[d * N + s] = xv;
-        double e = xv - L.x0[d * N + s];
-        e -= C.L[d] * rint(e / C.L[d]);
-        f3[d] = (float)(L.r0[d * N + s] + e);
+        f3[d] = (float)list_rel(C, ref, d, xv);
     }
     L.xsl[(size_t)list_q * N + s] = make_float4(f3[0], f3[1], f3[2], 0.f);
 }
