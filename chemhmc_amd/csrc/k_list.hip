@@ -59,21 +59,48 @@ __device__ __forceinline__ void request(const DevView& v, int q, int chain)
 __global__ __launch_bounds__(256) void k_lcount(DevView v, int q, int cb)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     int* __restrict__ prev = v.lcount + (size_t)(cb ^ 1) * v.Cl;
     for (int c = s; c < v.Cl; c += gridDim.x * blockDim.x) prev[c] = 0;
-    if (*req_any(v, q) == 0 || s >= v.Ntot) return;
-    const int ch = v.atom_chain[s];               // a chain's slots span the same range as its atoms
-    if (*req(v, q, ch) == 0) return;
+    if (*req_any(v, q) == 0) return;
+    // lanes of slots in requesting chains take part; whole waves stay together for the ballots below
+    const int ch = s < v.Ntot ? v.atom_chain[s] : 0;     // a chain's slots span the same range as its atoms
+    const bool part = s < v.Ntot && *req(v, q, ch) != 0;
+    if (__ballot(part) == 0) return;
     const ChainDev& C = v.chains[ch];
     const int N = v.Ntot;
-    const int a = list_buf(v, *list_cur(v, ch)).lperm[s];
-    const double x3[3] = { v.xl[s], v.xl[N + s], v.xl[2 * N + s] };
-    int b[3];
-    const int c = list_cell(C, x3, b);
-    float f3[3];
+    int a = 0, c = -1;
+    double x3[3] = { 0.0, 0.0, 0.0 };
+    float f3[3] = { 0.f, 0.f, 0.f };
+    if (part) {
+        a = list_buf(v, *list_cur(v, ch)).lperm[s];
 #pragma unroll
-    for (int d = 0; d < 3; d++) f3[d] = (float)(x3[d] - (double)b[d] * C.lsize[d]);
-    const int pos = atomicAdd(&v.lcount[(size_t)cb * v.Cl + c], 1);
+        for (int d = 0; d < 3; d++) x3[d] = v.xl[d * N + s];
+        int b[3];
+        c = list_cell(C, x3, b);
+#pragma unroll
+        for (int d = 0; d < 3; d++) f3[d] = (float)(x3[d] - (double)b[d] * C.lsize[d]);
+    }
+    // one atomic per distinct cell of the wave: consecutive slots (the previous rebuild's cell order)
+    // mostly fall into a few new cells.  Lanes of a group take consecutive member positions.
+    unsigned long long left = __ballot(part);
+    int lead = 0, rank = 0, gsize = 0;
+    while (left) {
+        const int l = __ffsll((long long)left) - 1;
+        const int cl = __shfl(c, l, 64);
+        const unsigned long long m = __ballot(part && c == cl) & left;
+        if (part && c == cl) {
+            lead = l;
+            rank = __popcll(m & ((1ull << lane) - 1ull));
+        }
+        if (lane == l) gsize = __popcll(m);
+        left &= ~m;
+    }
+    int base = 0;
+    if (part && lane == lead) base = atomicAdd(&v.lcount[(size_t)cb * v.Cl + c], gsize);
+    base = __shfl(base, lead, 64);
+    if (!part) return;
+    const int pos = base + rank;
     if (pos < C.lcap) {
         const size_t m = C.ldense_off + (size_t)(c - C.lcell_off) * C.lcap + pos;
         v.lmem[m] = make_float4(f3[0], f3[1], f3[2], __int_as_float(a));
