@@ -161,6 +161,16 @@ __device__ __forceinline__ void tile_cur_mark(const DevView& v, const ChainDev& 
     if (r % kTile == 0) v.tcur[C.tile_first + r / kTile] = b;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8), and each XCD has
+// its own L2: work item xcd_block(b, n) gives XCD x the contiguous range of items
+// [x * (n / 8) + min(x, n % 8), ...), so that neighbouring cells or tiles share one L2.  A bijection of
+// [0, n) for every n.
+__device__ __forceinline__ int xcd_block(int b, int n)
+{
+    const int x = b & 7, per = n >> 3, rem = n & 7;
+    return x * per + min(x, rem) + (b >> 3);
+}
+
 // image code of a list entry: neighbour cell offset + 1 in two bits per dimension
 __device__ __forceinline__ int image_code(int dx, int dy, int dz) { return (dx + 1) | ((dy + 1) << 2) | ((dz + 1) << 4); }
 // the offset of one dimension times the cell side, exactly (-side, 0 or +side: f * side - side is exact)

@@ -206,7 +206,7 @@ __device__ __forceinline__ void block_scan_256(int* a, int n, int* wsum)
 template <bool SCAN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_lbuild(DevView v, int q, int cb)
 {
-    const int c = blockIdx.x;
+    const int c = xcd_block(blockIdx.x, v.Cl);
     const int ch = v.lcell_chain[c];
     if (*req_any(v, q) == 0 || *req(v, q, ch) == 0) return;
     constexpr int IS = 4, JG = 16, NT = 256, W = 4, PT = kLMaxJ / NT;
@@ -361,21 +361,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             const float4 xi = si[ival ? i : ni - 1];
             const int slot_i = __float_as_int(xi.w);
             int count = icount[ival ? i : ni - 1];
+            int* __restrict__ row = L.nb + (size_t)slot_i * nbcap;
+            // branch-free test of every lane (j clamped to a staged entry); the hits of each i-lane
+            // group are appended in j order: position = count + hits of the group's lower lanes
             for (int j0 = 0; j0 < jn; j0 += JG) {
                 const int j = j0 + g;
-                bool hit = false;
-                float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (ival && j < jn) {
-                    p = sj[j];
-                    const float dx = xi.x - p.x, dy = xi.y - p.y, dz = xi.z - p.z;
-                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    hit = r2 < rl2 && r2 > 0.f;
-                }
+                const int jc = min(j, jn - 1);
+                const float4 p = sj[jc];
+                const float dx = xi.x - p.x, dy = xi.y - p.y, dz = xi.z - p.z;
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                const bool hit = ival && j < jn && r2 < rl2 && r2 > 0.f;
                 const unsigned long long msk = __ballot(hit) & pattern;
-                if (hit) {
-                    const int pos = count + __popcll(msk & ((1ull << lane) - 1ull));
-                    if (pos < nbcap) L.nb[(size_t)slot_i * nbcap + pos] = (__float_as_int(p.w) << kImageBits) | (int)scode[j];
-                }
+                const int pos = count + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
+                if (hit && pos < nbcap) row[pos] = (__float_as_int(p.w) << kImageBits) | (int)scode[jc];
                 count += __popcll(msk);
             }
             if (ival && g == 0) icount[i] = count;
@@ -443,8 +441,9 @@ constexpr int kBatch = 12;
 template <int MODE, int LP>
 __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q)
 {
-    const Chunk tl = v.tiles[blockIdx.x];
-    const ListBuf L = list_buf(v, v.tcur[blockIdx.x]);
+    const int tile = xcd_block(blockIdx.x, v.ntiles);
+    const Chunk tl = v.tiles[tile];
+    const ListBuf L = list_buf(v, v.tcur[tile]);
     const ChainDev& C = v.chains[tl.chain];
     const int N = v.Ntot;
     const int t = threadIdx.x / LP, sub = threadIdx.x % LP;
@@ -590,7 +589,8 @@ __device__ __forceinline__ double tile_sum_256(double s, double* part)
 __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
 {
     __shared__ double wpart[4];
-    const Chunk tl = v.tiles[blockIdx.x];
+    const int tile = xcd_block(blockIdx.x, v.ntiles);
+    const Chunk tl = v.tiles[tile];
     const ChainDev& C = v.chains[tl.chain];
     const ListBuf L = list_buf(v, *list_cur(v, tl.chain));
     if (threadIdx.x == 0) {
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
     an = tile_sum_256(an, wpart);
     ak = tile_sum_256(ak, wpart);
     if (threadIdx.x == 0) {
-        double* tp = &v.tile_part[4 * blockIdx.x];
+        double* tp = &v.tile_part[4 * tile];
         tp[0] = au; tp[1] = aw; tp[2] = an; tp[3] = ak;
     }
 }
