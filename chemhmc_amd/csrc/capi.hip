@@ -63,6 +63,9 @@ struct chmc_ctx {
     bool saved_lists = false;    // the last save kept a valid list buffer with the positions
     bool lstate = false;         // list MD owns the state: xl, pl, xl_saved (slot order) are current and
                                  // x, p, x_saved (atom order) are not; false at every API return
+    bool list_fresh = false;     // a sweep without drift (or the energy) ran since the last position
+                                 // change: no rebuild request can be pending at the current parity
+    bool rel64_fresh = false;    // rel64 holds the current positions (written by such a sweep)
     std::vector<void*> lallocs;
     std::vector<Chunk> htiles;
 };
@@ -820,6 +823,7 @@ static int list_import(chmc_ctx* c)
     if (rc) return rc;
     HIPCHK(c, launch_list_import(c->v, c->stream));
     c->lstate = true;
+    c->list_fresh = c->rel64_fresh = false;
     return CHMC_OK;
 }
 
@@ -844,16 +848,21 @@ static hipError_t list_rebuild(chmc_ctx* c)
 // request a rebuild before the next list sweep (positions changed outside the list path)
 static int list_invalidate(chmc_ctx* c)
 {
+    c->list_fresh = c->rel64_fresh = false;
     HIPCHK(c, launch_fill_int(c->v.lreq + (size_t)c->lpar * (c->nchains + 1), 1, c->nchains + 1, c->stream));
     return CHMC_OK;
 }
 
-static int list_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift, double drift_dt)
+// rel64 (without fuse_drift and storef): the sweep also writes the FP64 positions the energy reads
+static int list_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift, double drift_dt, bool rel64 = false)
 {
     HIPCHK(c, list_rebuild(c));
-    const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0) | (fuse_drift ? LJ_DRIFT : 0);
+    const bool r64 = rel64 && !fuse_drift && !storef;
+    const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0) | (fuse_drift ? LJ_DRIFT : 0) | (r64 ? LJ_REL64 : 0);
     HIPCHK(c, launch_list_sweep(c->v, mode, c->lpar, dt, fuse_drift ? drift_dt : 0.0, c->stream));
     if (fuse_drift) c->lpar ^= 1;
+    c->list_fresh = !fuse_drift;
+    c->rel64_fresh = r64;
     c->force_kind = (storef && !fuse_drift) ? 1 : 0;
     c->bin_state = 2;
     return CHMC_OK;
@@ -861,6 +870,7 @@ static int list_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift,
 
 static int list_drift(chmc_ctx* c, double dt)
 {
+    c->list_fresh = c->rel64_fresh = false;
     HIPCHK(c, launch_list_drift(c->v, c->lpar, dt, c->stream));
     c->lpar ^= 1;
     c->force_kind = 0;
@@ -871,8 +881,10 @@ static int list_drift(chmc_ctx* c, double dt)
 // FP64 energy / virial / kinetic per tile at the current positions (lists brought up to date first)
 static int list_energy(chmc_ctx* c, bool sums)
 {
-    HIPCHK(c, list_rebuild(c));
-    HIPCHK(c, launch_list_energy(c->v, c->lpar, c->stream));
+    // after a sweep without drift the lists already hold the current positions (its rebuild check ran)
+    if (!c->list_fresh) HIPCHK(c, list_rebuild(c));
+    HIPCHK(c, launch_list_energy(c->v, c->lpar, !c->rel64_fresh, c->stream));
+    c->list_fresh = c->rel64_fresh = true;
     if (sums) HIPCHK(c, launch_reduce(c->v, SUM_TILES, c->stream));
     return CHMC_OK;
 }
@@ -934,7 +946,8 @@ static int do_surr_force_kick(chmc_ctx* c, double dt, bool storef);
 static int surr_geometry(chmc_ctx* c);
 static int surr_check_flag(chmc_ctx* c);
 
-static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double lambda, bool storef)
+// rel64_last: a closing kick also writes the FP64 positions of the energy that follows
+static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double lambda, bool storef, bool rel64_last = false)
 {
     std::vector<MdOp> ops;
     if (build_ops(scheme, dt, steps, lambda, ops) != 0) return fail(c, CHMC_E_ARG, "unknown integrator scheme");
@@ -948,7 +961,8 @@ static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double l
         for (size_t k = 0; k < ops.size(); k++) {
             if (ops[k].kick) {
                 const bool next_x = k + 1 < ops.size() && !ops[k + 1].kick;
-                rc = list_force_kick(c, ops[k].dt, storef, next_x, next_x ? ops[k + 1].dt : 0.0);
+                rc = list_force_kick(c, ops[k].dt, storef, next_x, next_x ? ops[k + 1].dt : 0.0,
+                                     rel64_last && k + 1 == ops.size());
                 if (next_x) k++;
             } else {
                 rc = list_drift(c, ops[k].dt);
@@ -1111,13 +1125,14 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
             c->list_valid = true;
         }
         HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, false, true, c->stream));
-        if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false))) return rc;
+        if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false, true))) return rc;
         if (steps == 0 && !c->list_valid && (rc = list_invalidate(c))) return rc;
         c->list_valid = true;
         if ((rc = list_energy(c, false))) return rc;
         HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, 0, SUM_TILES,
                                     c->stream));
         HIPCHK(c, launch_save_restore_slots(c->v, 2, c->lpar, c->stream));
+        c->list_fresh = c->rel64_fresh = false;       // a rejected chain returned to its start positions
         c->force_kind = 0;
         c->bin_state = 2;
         return CHMC_OK;

@@ -131,7 +131,8 @@ struct DevView {
     double* tile_part;    // [ntiles * 4]: energy, virial, interacting pairs, kinetic energy
 };
 
-enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8, LJ_DRIFT = 16 };
+// LJ_REL64 (list sweep without drift): also the FP64 cell-relative positions of the energy (rel64)
+enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8, LJ_DRIFT = 16, LJ_REL64 = 32 };
 
 // one of the two neighbour-list buffers
 struct ListBuf {
@@ -236,7 +237,8 @@ hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s);
 hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s);
 hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, hipStream_t s);
 hipError_t launch_list_drift(const DevView& v, int q, double dt, hipStream_t s);
-hipError_t launch_list_energy(const DevView& v, int q, hipStream_t s);
+// rel64: compute the FP64 relative positions first (false when the last sweep wrote them)
+hipError_t launch_list_energy(const DevView& v, int q, bool rel64, hipStream_t s);
 hipError_t launch_rdf(const DevView& v, int chain, int natoms, int nbins, double size_bin,
                       unsigned long long* counts, hipStream_t s);
 hipError_t launch_open_potential(const double* xyz_soa, int n, int nexp, double a, double b,

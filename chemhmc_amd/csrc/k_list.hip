@@ -484,7 +484,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         // the tail's operands are fetched now, alongside the list, not after the butterfly
         if (sub < 3) {
             p_d = v.pl[d * N + s];
-            if constexpr ((MODE & LJ_DRIFT) != 0) {
+            if constexpr ((MODE & (LJ_DRIFT | LJ_REL64)) != 0) {
                 x_d = v.xl[d * N + s];
                 ref = L.lref[s];
             }
@@ -527,6 +527,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
             e = r - (double)ref_coord(ref, d);
             rel = (float)r;
         }
+        if constexpr ((MODE & LJ_REL64) != 0) v.rel64[d * N + s] = list_rel(C, ref, d, x_d);
     }
     if constexpr ((MODE & LJ_DRIFT) != 0) {
         const int base = threadIdx.x & ~(LP - 1) & 63;
@@ -705,6 +706,7 @@ hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, 
     case LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT:
         return sweep_lp<LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT>(v, lp, q, kick_dt, drift_dt, s);
     case LJ_FORCE | LJ_STOREF: return sweep_lp<LJ_FORCE | LJ_STOREF>(v, lp, q, kick_dt, drift_dt, s);
+    case LJ_FORCE | LJ_KICK | LJ_REL64: return sweep_lp<LJ_FORCE | LJ_KICK | LJ_REL64>(v, lp, q, kick_dt, drift_dt, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -715,11 +717,13 @@ hipError_t launch_list_drift(const DevView& v, int q, double dt, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_list_energy(const DevView& v, int q, hipStream_t s)
+hipError_t launch_list_energy(const DevView& v, int q, bool rel64, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lrel64, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (rel64) {
+        hipLaunchKernelGGL(k_lrel64, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_lenergy, dim3(v.ntiles), dim3(256), 0, s, v, q);
     return hipGetLastError();
 }
