@@ -9,7 +9,7 @@ the state resident in HBM and ends after the device has finished the K-th trajec
   value      atom-steps/s = N * 10 MD steps * K trajectories / seconds
   ess_per_s  ESS/s of U/N over the timed trajectories (Sokal-windowed tau_int)
   roofline   the dominant kernel, the FP32 neighbour-list sweep (force + kick + drift), HBM-bound by its
-             arithmetic intensity: algorithmic bytes per launch (4 per list entry + 184 per atom, list
+             arithmetic intensity: algorithmic bytes per launch (4 per list entry + 148 per atom, list
              entries counted on the device for the current configuration) / its average duration
              measured with HIP events on the context's stream (50 launches back to back)
   cpu_baseline  the FP64 C oracle (a port of the reference algorithm, cpu-opt binning) timed on this
@@ -32,9 +32,9 @@ N, RHO, T, RC = 32768, 0.8442, 0.722, 2.5
 DT, NSTEPS, LAM = 0.01, 10, 0.1932
 PEAK_FP32_TFLOPS = 157.3        # MI355X vector FP32 (vendor figure; packed FP32)
 PEAK_HBM_GBS = 8000.0
-# list sweep bytes per atom besides its list row: lperm 4, own FP32 coordinates 16, count 4, p 24 + 24,
-# x 24 + 24, x0 24, r0 24, next FP32 coordinates 16
-SWEEP_BYTES_PER_ATOM = 184.0
+# list sweep bytes per atom besides its list row: own FP32 coordinates 16, entry count 4, p 24 + 24 and
+# x 24 + 24 (read and written, slot order), rebuild reference 16, next FP32 coordinates 16
+SWEEP_BYTES_PER_ATOM = 148.0
 
 
 def _omp_threads(n):
@@ -98,6 +98,15 @@ def other_workload(args):
     t0 = time.perf_counter()
     e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
     t_warm = time.perf_counter() - t0
+    # skin autotuner, as for the headline workload (wall time of a few trajectories per skin)
+    skins = {}
+    for k, sk in enumerate((0.3, 0.4, 0.5, 0.6)):
+        e.set_skin(sk)
+        e.hmc(2, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=20 + k, use_graph=not args.no_graph)
+        t1 = time.perf_counter()
+        e.hmc(8, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=30 + k, use_graph=not args.no_graph)
+        skins[sk] = (time.perf_counter() - t1) / 8 * 1e3
+    e.set_skin(min(skins, key=skins.get))
     e.set_list_lanes(0)
     t0 = time.perf_counter()
     rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=not args.no_graph)
@@ -122,7 +131,7 @@ def other_workload(args):
         "config": {"workload": f"{args.workload}: {len(w.chains)} chain(s), {nat} atoms, "
                                f"{'LEAP' if w.scheme == chmc.LEAPFROG else 'OMF2'} {w.steps}x{w.dt}",
                    "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "skin": e.skin(),
-                   "warmup_seconds": t_warm},
+                   "ms_per_traj_by_skin": skins, "warmup_seconds": t_warm},
         "acceptance": float(nacc.sum()) / (steps * len(w.chains)), "ess_per_s_sum_over_chains": ess / el,
         "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
@@ -240,6 +249,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2000, help="timed HMC trajectories")
     ap.add_argument("--warmup", type=int, default=100, help="untimed trajectories (also thermalisation)")
     ap.add_argument("--tune", type=int, default=40, help="trajectories per skin in the skin autotuner")
+    ap.add_argument("--skin", type=float, default=None, help="fixed list skin (no skin autotuning)")
+    ap.add_argument("--lanes", type=int, default=0, choices=[0, 4, 8, 16],
+                    help="fixed list-sweep lanes per atom (0: autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true",
@@ -277,15 +289,15 @@ def main():
         e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
     # skin autotuner: wall time of a short run per skin on the equilibrated state
     skins = {}
-    for k, sk in enumerate((0.3, 0.4, 0.5, 0.6, 0.7, 0.8)):
+    for k, sk in enumerate((0.3, 0.4, 0.5, 0.6, 0.7, 0.8) if args.skin is None else ()):
         e.set_skin(sk)
         e.hmc(4, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=20 + k, use_graph=not args.no_graph)
         t0 = time.perf_counter()
         e.hmc(args.tune, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=30 + k, use_graph=not args.no_graph)
         skins[sk] = (time.perf_counter() - t0) / args.tune * 1e3
-    skin = min(skins, key=skins.get)
+    skin = min(skins, key=skins.get) if args.skin is None else args.skin
     e.set_skin(skin)
-    e.set_list_lanes(0)                         # lanes per atom of the list sweep (autotuned)
+    e.set_list_lanes(args.lanes)                # lanes per atom of the list sweep (0: autotuned)
     lanes = e.list_lanes()
     _, _, rebuilds0 = e.list_stats()
 
@@ -302,7 +314,7 @@ def main():
     # dominant kernel: the neighbour-list FP32 sweep (force + kick + drift).  Its arithmetic intensity
     # (~2.6 flop/B) is below the ridge (157.3 TF/s / 8 TB/s = 19.7 flop/B), so the roofline that bounds
     # it is HBM: algorithmic bytes per launch = 4 B per list entry + SWEEP_BYTES_PER_ATOM per atom
-    # (own coordinates, count, atom id, rebuild reference, p and x read and written, next coordinates);
+    # (own coordinates, entry count, rebuild reference, p and x read and written, next coordinates);
     # the neighbour coordinates it gathers are cache hits and not counted.  Flops: 8 per list entry
     # (candidate) + 13 per interacting pair.
     entries, nint, rebuilds1 = e.list_stats()
