@@ -760,7 +760,7 @@ static int list_setup(chmc_ctx* c, double skin)
     HIPCHK(c, lalloc(c, &v.lreq, 2 * ((size_t)c->nchains + 1)));
     HIPCHK(c, lalloc(c, &v.lscan_tmp, (size_t)lcells));
     HIPCHK(c, lalloc(c, &v.lref, 2 * N));
-    HIPCHK(c, lalloc(c, &v.rel64, 3 * N));
+    HIPCHK(c, lalloc(c, &v.rel64, 4 * N));
     HIPCHK(c, lalloc(c, &v.xsl, 2 * 2 * N));
     HIPCHK(c, lalloc(c, &v.tile_part, 4 * (size_t)v.ntiles));
     v.lcell_chain = lcell_chain;

@@ -126,7 +126,7 @@ struct DevView {
     int* lbuf;            // [2][nchains]: current list buffer, buffer of the trajectory's start
     int* tcur;            // [ntiles]: current list buffer of each tile's chain (kept equal to lbuf)
     float4* lref;
-    double* rel64;
+    double* rel64;        // [Ntot][4] by slot: FP64 position relative to the rebuild cell origin (x, y, z, -)
     float4* xsl;
     double* tile_part;    // [ntiles * 4]: energy, virial, interacting pairs, kinetic energy
 };

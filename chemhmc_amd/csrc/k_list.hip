@@ -527,7 +527,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
             e = r - (double)ref_coord(ref, d);
             rel = (float)r;
         }
-        if constexpr ((MODE & LJ_REL64) != 0) v.rel64[d * N + s] = list_rel(C, ref, d, x_d);
+        if constexpr ((MODE & LJ_REL64) != 0) v.rel64[4 * (size_t)s + d] = list_rel(C, ref, d, x_d);
     }
     if constexpr ((MODE & LJ_DRIFT) != 0) {
         const int base = threadIdx.x & ~(LP - 1) & 63;
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(256) void k_lrel64(DevView v)
     const int N = v.Ntot;
     const float4 ref = L.lref[s];
 #pragma unroll
-    for (int d = 0; d < 3; d++) v.rel64[d * N + s] = list_rel(C, ref, d, v.xl[d * N + s]);
+    for (int d = 0; d < 3; d++) v.rel64[4 * (size_t)s + d] = list_rel(C, ref, d, v.xl[d * N + s]);
 }
 
 __device__ __forceinline__ double tile_sum_256(double s, double* part)
@@ -604,17 +604,24 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
     const bool val = s < tl.end;
     double au = 0.0, aw = 0.0, an = 0.0, ak = 0.0;
     if (val) {
-        const double xi = v.rel64[s], yi = v.rel64[N + s], zi = v.rel64[2 * N + s];
+        const double2* __restrict__ r64 = reinterpret_cast<const double2*>(v.rel64);   // [Ntot][4]: x, y | z, -
+        const double2 ia = r64[2 * (size_t)s], ib = r64[2 * (size_t)s + 1];
+        const double xi = ia.x, yi = ia.y, zi = ib.x;
         const int n = L.nbn[s];
         const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
+        // sqrt(r2) < rc (the reference's test) decided without the square root away from the cut-off:
+        // the FP64 square root is within one ulp, far inside a 1e-12 relative band
+        const double lo2 = C.rc2 * (1.0 - 1e-12), hi2 = C.rc2 * (1.0 + 1e-12);
         for (int k = sub; k < n; k += 8) {
             const int e = nbs[k];
             const int j = e >> kImageBits, code = e & 63;
-            const double dx = xi - (v.rel64[j] + (double)((code & 3) - 1) * C.lsize[0]);
-            const double dy = yi - (v.rel64[N + j] + (double)(((code >> 2) & 3) - 1) * C.lsize[1]);
-            const double dz = zi - (v.rel64[2 * N + j] + (double)(((code >> 4) & 3) - 1) * C.lsize[2]);
+            const double2 ja = r64[2 * (size_t)j], jb = r64[2 * (size_t)j + 1];
+            const double dx = xi - (ja.x + (double)((code & 3) - 1) * C.lsize[0]);
+            const double dy = yi - (ja.y + (double)(((code >> 2) & 3) - 1) * C.lsize[1]);
+            const double dz = zi - (jb.x + (double)(((code >> 4) & 3) - 1) * C.lsize[2]);
             const double r2 = dx * dx + dy * dy + dz * dz;
-            if (sqrt(r2) < C.rc && r2 > 0.0) {                 // identical_particles.cpp:432-435
+            const bool inside = r2 < lo2 ? true : (r2 >= hi2 ? false : sqrt(r2) < C.rc);
+            if (inside && r2 > 0.0) {                          // identical_particles.cpp:432-435
                 const double s2 = C.sigma2 / r2;
                 const double s6 = s2 * s2 * s2;
                 au += s6 * (s6 - 1.0);
