@@ -436,11 +436,14 @@ __device__ __forceinline__ void list_drift_slot(const DevView& v, const ListBuf&
 // workgroup's list buffer comes from tcur[blockIdx] and the image shifts are decoded from the entries in
 // registers, so nothing delays the first list load but the tile's own two scalar loads (no LDS table,
 // no barrier).
-constexpr int kBatch = 12;
+// entries per lane in the first batch, by lanes per atom (measured at skins 0.5-0.7, N=32768: 14 at 8
+// lanes covers the typical 105 entries at skin 0.6 and beats 12 and 16)
+template <int LP> constexpr int batch_of() { return LP == 4 ? 16 : (LP == 8 ? 14 : 12); }
 
 template <int MODE, int LP>
 __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q)
 {
+    constexpr int kBatch = batch_of<LP>();
     const int tile = xcd_block(blockIdx.x, v.ntiles);
     const Chunk tl = v.tiles[tile];
     const ListBuf L = list_buf(v, v.tcur[tile]);
