@@ -209,10 +209,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int c = xcd_block(blockIdx.x, v.Cl);
     const int ch = v.lcell_chain[c];
     if (*req_any(v, q) == 0 || *req(v, q, ch) == 0) return;
-    constexpr int IS = 4, JG = 16, NT = 256, W = 4, PT = kLMaxJ / NT;
-    __shared__ float4 sj[kLMaxJ];          // staged j in slot order: coordinates relative to the i-cell origin, slot in w
+    constexpr int IS = 4, JG = 16, NT = 256, W = 4, PT = (kLMaxJ + NT - 1) / NT;
+    __shared__ float4 sj[kLMaxJ];          // staged j in slot order: coordinates relative to the i-cell origin, list entry in w
     __shared__ int sid[kLMaxJ];            // staged members' atom ids in arrival order (ranking)
-    __shared__ unsigned char scode[kLMaxJ];
     // own atoms by slot (FP32 coordinates, slot in w), their entry counts and their ids in arrival
     // order: lcap_max of each, in the dynamic LDS, so that the workgroup's LDS stays small
     extern __shared__ float4 dyn_lds[];
@@ -325,6 +324,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             // own slot data: slots by atom id; the rebuild reference (FP32 cell-relative position, cell)
             const int a = __float_as_int(meo.w);
             int r = 0;
+#pragma unroll 8
             for (int k = 0; k < ni; k++) r += sido[k] < a ? 1 : 0;
             const int slot = ibeg + r;
 #pragma unroll
@@ -348,10 +348,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 const int s0 = seg_pref[k] - jlo, nk = seg_cnt[k];
                 const int a = __float_as_int(me[u].w);
                 int r = 0;
+#pragma unroll 8
                 for (int w = 0; w < nk; w++) r += sid[s0 + w] < a ? 1 : 0;
+                // w holds the finished list entry: j slot << kImageBits | image code
                 sj[s0 + r] = make_float4(me[u].x + seg_sh[k][0], me[u].y + seg_sh[k][1], me[u].z + seg_sh[k][2],
-                                         __int_as_float(seg_beg[k] + r));
-                scode[s0 + r] = (unsigned char)seg_code[k];
+                                         __int_as_float(((seg_beg[k] + r) << kImageBits) | seg_code[k]));
             }
         }
         __syncthreads();
@@ -360,21 +361,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             const bool ival = i < ni;
             const float4 xi = si[ival ? i : ni - 1];
             const int slot_i = __float_as_int(xi.w);
-            int count = icount[ival ? i : ni - 1];
+            const int count0 = icount[ival ? i : ni - 1];
+            int count = count0;
             int* __restrict__ row = L.nb + (size_t)slot_i * nbcap;
             // branch-free test of every lane (j clamped to a staged entry); the hits of each i-lane
             // group are appended in j order: position = count + hits of the group's lower lanes
-            for (int j0 = 0; j0 < jn; j0 += JG) {
-                const int j = j0 + g;
-                const int jc = min(j, jn - 1);
-                const float4 p = sj[jc];
-                const float dx = xi.x - p.x, dy = xi.y - p.y, dz = xi.z - p.z;
-                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                const bool hit = ival && j < jn && r2 < rl2 && r2 > 0.f;
-                const unsigned long long msk = __ballot(hit) & pattern;
-                const int pos = count + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
-                if (hit && pos < nbcap) row[pos] = (__float_as_int(p.w) << kImageBits) | (int)scode[jc];
-                count += __popcll(msk);
+            constexpr int U = 2;                  // two LDS reads in flight per iteration
+            for (int j0 = 0; j0 < jn; j0 += JG * U) {
+                float4 p[U];
+                bool hit[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int j = j0 + u * JG + g;
+                    p[u] = sj[min(j, jn - 1)];
+                    const float dx = xi.x - p[u].x, dy = xi.y - p[u].y, dz = xi.z - p[u].z;
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    hit[u] = ival && j < jn && r2 < rl2 && r2 > 0.f;
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const unsigned long long msk = __ballot(hit[u]) & pattern;
+                    const int pos = count + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
+                    if (hit[u] && pos < nbcap) row[pos] = __float_as_int(p[u].w);
+                    count += __popcll(msk);
+                }
             }
             if (ival && g == 0) icount[i] = count;
         }
