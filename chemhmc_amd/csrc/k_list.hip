@@ -237,42 +237,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         *list_cur(v, ch) = tb;
         if (lc == 0) atomicAdd(&v.lflag[3], 1);        // chain rebuilds run (statistics)
     }
-    if (wave == 0 && lane < 27) {
-        const int dd[3] = { lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1 };
-        const int b3[3] = { bx, by, bz }, n3[3] = { nx, ny, nz };
-        int j3[3];
+    // the own cell's members and their FP64 positions (one per thread, in bounds for every thread
+    // below lcap): issued first, their latency overlaps the offsets
+    float4 meo = make_float4(0.f, 0.f, 0.f, 0.f);
+    double xo[3] = { 0.0, 0.0, 0.0 }, po[3] = { 0.0, 0.0, 0.0 };
+    if (tid < C.lcap) {
+        const size_t mi = (size_t)lc * C.lcap + tid;
+        meo = mem[mi];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            j3[d] = b3[d] + dd[d];
-            if (j3[d] < 0 || j3[d] >= n3[d]) j3[d] = (b3[d] + n3[d] + dd[d]) % n3[d];
-            seg_sh[lane][d] = (float)dd[d] * C.lsize_f[d];
-        }
-        const int jl = j3[0] + nx * (j3[1] + j3[2] * ny);
-        seg_cell[lane] = jl;
-        seg_code[lane] = image_code(dd[0], dd[1], dd[2]);
-        if constexpr (!SCAN) {
-            seg_beg[lane] = L.loffs[C.lcell_off + jl];
-            seg_cnt[lane] = min(L.loffs[C.lcell_off + jl + 1] - L.loffs[C.lcell_off + jl], C.lcap);
-        } else {
-            seg_cnt[lane] = min(cnt[jl], C.lcap);
+            xo[d] = memx[d * xstride + mi];
+            po[d] = memp[d * xstride + mi];
         }
     }
+    if (wave == 0) {
+        int nk = 0;
+        if (lane < 27) {
+            const int dd[3] = { lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1 };
+            const int b3[3] = { bx, by, bz }, n3[3] = { nx, ny, nz };
+            int j3[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                j3[d] = b3[d] + dd[d];
+                if (j3[d] < 0 || j3[d] >= n3[d]) j3[d] = (b3[d] + n3[d] + dd[d]) % n3[d];
+                seg_sh[lane][d] = (float)dd[d] * C.lsize_f[d];
+            }
+            const int jl = j3[0] + nx * (j3[1] + j3[2] * ny);
+            seg_cell[lane] = jl;
+            seg_code[lane] = image_code(dd[0], dd[1], dd[2]);
+            if constexpr (!SCAN) {
+                seg_beg[lane] = L.loffs[C.lcell_off + jl];
+                nk = min(L.loffs[C.lcell_off + jl + 1] - L.loffs[C.lcell_off + jl], C.lcap);
+            } else {
+                nk = min(cnt[jl], C.lcap);
+            }
+            seg_cnt[lane] = nk;
+        }
+        // staging offsets of the 27 segments: a scan over lanes 0..26
+        int incl = nk;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane < 27) seg_pref[lane + 1] = incl;
+        if (lane == 0) seg_pref[0] = 0;
+    }
+    int* scan = reinterpret_cast<int*>(sj);          // SCAN: sj is free until the first pass is staged
     if constexpr (SCAN) {
-        int* scan = reinterpret_cast<int*>(sj);      // sj is free until the first pass
-        const int n = C.lncells;
-        for (int i = tid; i < n; i += NT) scan[i] = cnt[i];
-        __syncthreads();
-        block_scan_256(scan, n, wsum);
-        if (tid < 27) seg_beg[tid] = C.atom_off + scan[seg_cell[tid]];
+        for (int i = tid; i < C.lncells; i += NT) scan[i] = cnt[i];
     }
     __syncthreads();
-    if (tid == 0) {
-        int s = 0;
-        for (int k = 0; k < 27; k++) { seg_pref[k] = s; s += seg_cnt[k]; }
-        seg_pref[27] = s;
-    }
-    __syncthreads();
-    const int ni = seg_cnt[13], ibeg = seg_beg[13];     // segment 13: the cell itself (ni <= lcap <= 256)
+    const int ni = seg_cnt[13];                      // segment 13: the cell itself (ni <= lcap <= 256)
     if (ni == 0) return;
 
     const float rl2 = C.rl2_f;
@@ -282,26 +298,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     for (int k = 0; k < JG; k++) pattern |= 1ull << (k * IS + ii);
     const int nchunk = (ni + IS - 1) / IS;
     const int nbcap = v.nbcap;
+    int ibeg = 0;
     for (int k0 = 0; k0 < 27;) {
         const bool first = k0 == 0;
         if (!first) __syncthreads();           // sid, sj free again
         int k1 = k0 + 1;
         while (k1 < 27 && seg_pref[k1 + 1] - seg_pref[k0] <= kLMaxJ) k1++;
         const int jlo = seg_pref[k0], jn = seg_pref[k1] - jlo;
-        // with the first pass: the own cell's members and their FP64 positions (one per thread)
-        float4 meo = make_float4(0.f, 0.f, 0.f, 0.f);
-        double xo[3] = { 0.0, 0.0, 0.0 }, po[3] = { 0.0, 0.0, 0.0 };
-        if (first && tid < ni) {
-            const size_t mi = (size_t)lc * C.lcap + tid;
-            meo = mem[mi];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                xo[d] = memx[d * xstride + mi];
-                po[d] = memp[d * xstride + mi];
-            }
-            sido[tid] = __float_as_int(meo.w);
-        }
-        // members of this pass, loaded once (PT per thread), ids to LDS for the ranking
+        // members of this pass, loaded once (PT per thread); with the first pass the loads are in
+        // flight during the scan of the counts, the ids go to LDS for the ranking after it
         float4 me[PT];
         int kk[PT];
 #pragma unroll
@@ -316,10 +321,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                     if (k + step < k1 && seg_pref[k + step] <= jlo + t) k += step;
                 kk[u] = k;
                 me[u] = mem[(size_t)seg_cell[k] * C.lcap + (jlo + t - seg_pref[k])];
-                sid[t] = __float_as_int(me[u].w);
             }
         }
-        __syncthreads();
+        if constexpr (SCAN) {
+            if (first) {
+                block_scan_256(scan, C.lncells, wsum);     // ends with a barrier: scan[] final
+                if (tid < 27) seg_beg[tid] = C.atom_off + scan[seg_cell[tid]];
+            }
+        }
+        if (first && tid < ni) sido[tid] = __float_as_int(meo.w);
+#pragma unroll
+        for (int u = 0; u < PT; u++)
+            if (tid + NT * u < jn) sid[tid + NT * u] = __float_as_int(me[u].w);
+        __syncthreads();                       // seg_beg, sid, sido; the scan area is free again
+        if (first) ibeg = seg_beg[13];
         if (first && tid < ni) {
             // own slot data: slots by atom id; the rebuild reference (FP32 cell-relative position, cell)
             const int a = __float_as_int(meo.w);
