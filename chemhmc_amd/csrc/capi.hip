@@ -722,7 +722,7 @@ static int list_setup(chmc_ctx* c, double skin)
         cd.ntiles = (int)c->htiles.size() - cd.tile_first;
     }
     if (lcells > (1LL << 28) || dense > (1LL << 30)) return fail(c, CHMC_E_ARG, "list grids too large");
-    const int nbcap = ((int)std::ceil(nbmax) + 31) / 32 * 32;
+    const int nbcap = std::max(((int)std::ceil(nbmax) + 31) / 32 * 32, (kListSpan + 31) / 32 * 32);
     const size_t N = (size_t)c->Ntot;
     if ((double)N * nbcap > 4.0e9) return fail(c, CHMC_E_ARG, "neighbour lists too large (reduce the skin)");
     if (N >= (size_t)1 << (31 - kImageBits)) return fail(c, CHMC_E_ARG, "too many atoms for the list entry format");
@@ -781,7 +781,8 @@ static int list_setup(chmc_ctx* c, double skin)
     for (int b = 0; b < 2; b++)
         HIPCHK(c, hipMemcpy(v.loffs + (size_t)b * (lcells + 1), lo0.data(), sizeof(int) * lo0.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(v.nbn, 0, sizeof(int) * 2 * N));
-    HIPCHK(c, hipMemset(v.nb, 0, sizeof(int) * 2 * N * nbcap));    // every entry a valid slot (see k_lsweep)
+    HIPCHK(c, launch_list_pad_init(v, c->stream));      // every entry a no-op entry of its row's slot
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     // valid atom ids in every slot before the first rebuild (a slot is rewritten by each rebuild)
     std::vector<int> iota(N);
     std::vector<float4> xs0(N);

@@ -93,6 +93,9 @@ struct Chunk { int chain, begin, end, pad_; };
 constexpr int kMaxJ = 768;         // j-list entries staged in LDS per pass
 constexpr int kRedChunk = 256;     // atoms per deterministic reduction chunk
 constexpr int kTile = 32;          // slots per neighbour-list sweep workgroup (8 lanes per slot)
+// leading entries of every list row that the sweep reads unconditionally: the build pads a row past its
+// count up to this span with entries that contribute nothing (the slot itself, no image shift)
+constexpr int kListSpan = 112;
 constexpr int kScanCells = 3072;   // list cells of a chain whose counts one build workgroup scans in LDS
 
 // ---- launch wrappers (each defined in the TU that owns the kernel) ----------------------
@@ -235,6 +238,7 @@ hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s);
 // force sweep (LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT; with LJ_DRIFT the parity advances to q ^ 1),
 // a standalone drift (q -> q ^ 1) and the FP64 energy / virial / kinetic sums into tile_part
 hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s);
+hipError_t launch_list_pad_init(const DevView& v, hipStream_t s);
 hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, hipStream_t s);
 hipError_t launch_list_drift(const DevView& v, int q, double dt, hipStream_t s);
 // rel64: compute the FP64 relative positions first (false when the last sweep wrote them)

@@ -411,6 +411,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         L.nbn[ibeg + r] = min(n, nbcap);
         if (n > nbcap) atomicOr(&v.lflag[2], 2);
     }
+    // rows padded to the sweep's unconditional span with the slot itself (r2 = 0: no contribution)
+    const int span = min(kListSpan, nbcap);
+    for (int r = wave; r < ni; r += W) {
+        const int slot = __float_as_int(si[r].w);
+        const int pad = (slot << kImageBits) | image_code(0, 0, 0);
+        int* __restrict__ row = L.nb + (size_t)slot * nbcap;
+        for (int k = icount[r] + lane; k < span; k += 64) row[k] = pad;
+    }
+}
+
+// every list row of both buffers filled with its slot's no-op entry (before the first rebuild)
+__global__ __launch_bounds__(256) void k_lpad_init(DevView v)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t per = (size_t)v.Ntot * v.nbcap;
+    if (i >= 2 * per) return;
+    const int slot = (int)((i % per) / v.nbcap);
+    v.nb[i] = (slot << kImageBits) | image_code(0, 0, 0);
+}
+
+hipError_t launch_list_pad_init(const DevView& v, hipStream_t s)
+{
+    const size_t n = 2 * (size_t)v.Ntot * v.nbcap;
+    hipLaunchKernelGGL(k_lpad_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v);
+    return hipGetLastError();
 }
 
 hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s)
@@ -463,12 +488,14 @@ __device__ __forceinline__ void list_drift_slot(const DevView& v, const ListBuf&
 // no barrier).
 // entries per lane in the first batch, by lanes per atom (measured at skins 0.5-0.7, N=32768: 14 at 8
 // lanes covers the typical 105 entries at skin 0.6 and beats 12 and 16)
-template <int LP> constexpr int batch_of() { return LP == 4 ? 16 : (LP == 8 ? 14 : 12); }
+template <int LP> constexpr int batch_of() { return LP == 4 ? 16 : (LP == 8 ? 14 : 7); }
 
 template <int MODE, int LP>
 __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q)
 {
     constexpr int kBatch = batch_of<LP>();
+    static_assert(kBatch * LP <= kListSpan, "the first batch reads padded entries only");
+    __shared__ float4 shift_tab[64];                     // image shift by 6-bit image code
     const int tile = xcd_block(blockIdx.x, v.ntiles);
     const Chunk tl = v.tiles[tile];
     const ListBuf L = list_buf(v, v.tcur[tile]);
@@ -485,10 +512,10 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
     double p_d = 0.0, x_d = 0.0;
     float4 ref = make_float4(0.f, 0.f, 0.f, 0.f);
     auto pair = [&](const float4 pj, int code) {
-        const float lx = C.lsize_f[0], ly = C.lsize_f[1], lz = C.lsize_f[2];
-        const float dx = xi.x - (pj.x + image_shift(code, 0, lx));
-        const float dy = xi.y - (pj.y + image_shift(code, 1, ly));
-        const float dz = xi.z - (pj.z + image_shift(code, 2, lz));
+        const float4 sh = shift_tab[code];
+        const float dx = xi.x - (pj.x + sh.x);
+        const float dy = xi.y - (pj.y + sh.y);
+        const float dz = xi.z - (pj.z + sh.z);
         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         const float inv = __builtin_amdgcn_rcpf(r2);
         const float s2 = C.sigma2_f * inv;
@@ -499,16 +526,23 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         ay = __builtin_fmaf(gq, dy, ay);
         az = __builtin_fmaf(gq, dz, az);
     };
+    if (threadIdx.x < 64) {
+        const int code = threadIdx.x;
+        shift_tab[code] = make_float4(image_shift(code, 0, C.lsize_f[0]), image_shift(code, 1, C.lsize_f[1]),
+                                      image_shift(code, 2, C.lsize_f[2]), 0.f);
+    }
+    int ent[kBatch];
+    float4 pj[kBatch];
+    int n = 0;
     if (val) {
         const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
-        // the list entries are read before the count is known (entries past the count are older valid
-        // slots, or zero: the list buffer is cleared at setup) so that both arrive in one round trip
-        int e[kBatch];
+        // the first kBatch * LP entries are read unconditionally, before the count is known: past the
+        // count the build pads them with no-op entries (nbcap >= kListSpan)
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) e[u] = nbs[min(sub + u * LP, v.nbcap - 1)];
+        for (int u = 0; u < kBatch; u++) ent[u] = nbs[sub + u * LP];
         if constexpr ((MODE & LJ_STOREF) != 0) atom = L.lperm[s];     // the stored force is in atom order
         xi = xq[s];
-        const int n = L.nbn[s];
+        n = L.nbn[s];
         // the tail's operands are fetched now, alongside the list, not after the butterfly
         if (sub < 3) {
             p_d = v.pl[d * N + s];
@@ -517,15 +551,17 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
                 ref = L.lref[s];
             }
         }
-        float4 pj[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) pj[u] = xq[e[u] >> kImageBits];
+        for (int u = 0; u < kBatch; u++) pj[u] = xq[(unsigned)ent[u] >> kImageBits];
+    }
+    __syncthreads();                                      // shift_tab
+    if (val) {
 #pragma unroll
-        for (int u = 0; u < kBatch; u++)
-            if (sub + u * LP < n) pair(pj[u], e[u] & 63);
+        for (int u = 0; u < kBatch; u++) pair(pj[u], ent[u] & 63);
+        const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
         for (int k = sub + kBatch * LP; k < n; k += LP) {
             const int ee = nbs[k];
-            pair(xq[ee >> kImageBits], ee & 63);
+            pair(xq[(unsigned)ee >> kImageBits], ee & 63);
         }
     }
 #pragma unroll
