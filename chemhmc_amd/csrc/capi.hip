@@ -724,7 +724,7 @@ static int list_setup(chmc_ctx* c, double skin)
     if (lcells > (1LL << 28) || dense > (1LL << 30)) return fail(c, CHMC_E_ARG, "list grids too large");
     const int nbcap = std::max(((int)std::ceil(nbmax) + 31) / 32 * 32, (kListSpan + 31) / 32 * 32);
     const size_t N = (size_t)c->Ntot;
-    if ((double)N * nbcap > 4.0e9) return fail(c, CHMC_E_ARG, "neighbour lists too large (reduce the skin)");
+    if ((double)N * nbcap >= 2147483647.0) return fail(c, CHMC_E_ARG, "neighbour lists too large (reduce the skin)");
     if (N >= (size_t)1 << (31 - kImageBits)) return fail(c, CHMC_E_ARG, "too many atoms for the list entry format");
     DevView& v = c->v;
     v.Cl = (int)lcells;

@@ -94,8 +94,11 @@ constexpr int kMaxJ = 768;         // j-list entries staged in LDS per pass
 constexpr int kRedChunk = 256;     // atoms per deterministic reduction chunk
 constexpr int kTile = 32;          // slots per neighbour-list sweep workgroup (8 lanes per slot)
 // leading entries of every list row that the sweep reads unconditionally: the build pads a row past its
-// count up to this span with entries that contribute nothing (the slot itself, no image shift)
+// count up to this span with entries that contribute nothing (the slot itself with the image code
+// kFarCode, which no neighbour has: the sweep's shift table places that image beyond any cut-off)
 constexpr int kListSpan = 112;
+constexpr int kFarCode = 63;
+constexpr float kFarShift = 1.0e18f;
 constexpr int kScanCells = 3072;   // list cells of a chain whose counts one build workgroup scans in LDS
 
 // ---- launch wrappers (each defined in the TU that owns the kernel) ----------------------

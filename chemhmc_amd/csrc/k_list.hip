@@ -162,6 +162,7 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q, int cb)
 }
 
 constexpr int kLMaxJ = 1024;      // staged neighbour members per pass (a cell never spans two passes)
+constexpr int kJPad = 32;         // far entries past a pass's members: one iteration of the pair test
 
 // exclusive scan of n <= kScanCells values in LDS by one 256-thread workgroup
 __device__ __forceinline__ void block_scan_256(int* a, int n, int* wsum)
@@ -210,7 +211,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int ch = v.lcell_chain[c];
     if (*req_any(v, q) == 0 || *req(v, q, ch) == 0) return;
     constexpr int IS = 4, JG = 16, NT = 256, W = 4, PT = (kLMaxJ + NT - 1) / NT;
-    __shared__ float4 sj[kLMaxJ];          // staged j in slot order: coordinates relative to the i-cell origin, list entry in w
+    // staged j in slot order: coordinates relative to the i-cell origin, list entry in w; padded past the
+    // pass's members to a whole iteration of the test loop with far-away entries
+    __shared__ float4 sj[kLMaxJ + kJPad];
     __shared__ int sid[kLMaxJ];            // staged members' atom ids in arrival order (ranking)
     // own atoms by slot (FP32 coordinates, slot in w), their entry counts and their ids in arrival
     // order: lcap_max of each, in the dynamic LDS, so that the workgroup's LDS stays small
@@ -370,6 +373,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                                          __int_as_float(((seg_beg[k] + r) << kImageBits) | seg_code[k]));
             }
         }
+        if (tid < kJPad) sj[jn + tid] = make_float4(kFarShift, kFarShift, kFarShift, 0.f);
         __syncthreads();
         for (int chunk = wave; chunk < nchunk; chunk += W) {
             const int i = chunk * IS + ii;
@@ -378,26 +382,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             const int slot_i = __float_as_int(xi.w);
             const int count0 = icount[ival ? i : ni - 1];
             int count = count0;
-            int* __restrict__ row = L.nb + (size_t)slot_i * nbcap;
-            // branch-free test of every lane (j clamped to a staged entry); the hits of each i-lane
+            // row offsets in 32 bits: Ntot * nbcap < 2^31 (checked at setup)
+            const int rowoff = slot_i * nbcap;
+            // branch-free test of every lane (the far padding fails every test); the hits of each i-lane
             // group are appended in j order: position = count + hits of the group's lower lanes
-            constexpr int U = 2;                  // two LDS reads in flight per iteration
+            constexpr int U = kJPad / JG;         // two LDS reads in flight per iteration
+            char* __restrict__ nbb = reinterpret_cast<char*>(L.nb);
             for (int j0 = 0; j0 < jn; j0 += JG * U) {
                 float4 p[U];
-                bool hit[U];
+                float r2[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const int j = j0 + u * JG + g;
-                    p[u] = sj[min(j, jn - 1)];
+                    p[u] = sj[j0 + u * JG + g];
                     const float dx = xi.x - p[u].x, dy = xi.y - p[u].y, dz = xi.z - p[u].z;
-                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    hit[u] = ival && j < jn && r2 < rl2 && r2 > 0.f;
+                    r2[u] = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
                 }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const unsigned long long msk = __ballot(hit[u]) & pattern;
+                    // the wave's hit mask straight from the compares (4: ordered less, 2: ordered greater);
+                    // the lanes of an i-lane group share i and so i < ni, and the position and count of
+                    // an i beyond ni are never used, so the mask needs no i < ni term
+                    const unsigned long long hits = __builtin_amdgcn_fcmpf(r2[u], rl2, 4) & __builtin_amdgcn_fcmpf(r2[u], 0.f, 2);
+                    const unsigned long long msk = hits & pattern;
                     const int pos = count + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
-                    if (hit[u] && pos < nbcap) row[pos] = __float_as_int(p[u].w);
+                    if (ival && r2[u] < rl2 && r2[u] > 0.f && pos < nbcap)
+                        *reinterpret_cast<int*>(nbb + (unsigned)((rowoff + pos) << 2)) = __float_as_int(p[u].w);
                     count += __popcll(msk);
                 }
             }
@@ -411,11 +420,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         L.nbn[ibeg + r] = min(n, nbcap);
         if (n > nbcap) atomicOr(&v.lflag[2], 2);
     }
-    // rows padded to the sweep's unconditional span with the slot itself (r2 = 0: no contribution)
+    // rows padded to the sweep's unconditional span with no-op entries (the far image of the slot)
     const int span = min(kListSpan, nbcap);
     for (int r = wave; r < ni; r += W) {
         const int slot = __float_as_int(si[r].w);
-        const int pad = (slot << kImageBits) | image_code(0, 0, 0);
+        const int pad = (slot << kImageBits) | kFarCode;
         int* __restrict__ row = L.nb + (size_t)slot * nbcap;
         for (int k = icount[r] + lane; k < span; k += 64) row[k] = pad;
     }
@@ -428,7 +437,7 @@ __global__ __launch_bounds__(256) void k_lpad_init(DevView v)
     const size_t per = (size_t)v.Ntot * v.nbcap;
     if (i >= 2 * per) return;
     const int slot = (int)((i % per) / v.nbcap);
-    v.nb[i] = (slot << kImageBits) | image_code(0, 0, 0);
+    v.nb[i] = (slot << kImageBits) | kFarCode;
 }
 
 hipError_t launch_list_pad_init(const DevView& v, hipStream_t s)
@@ -521,15 +530,17 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         const float s2 = C.sigma2_f * inv;
         const float s6 = s2 * s2 * s2;
         float gq = s6 * (0.5f - s6) * inv;            // sr6 (0.5 - sr6) / r^2, identical_particles.cpp:611-613
-        gq = (r2 < C.rc2_f && r2 > 0.f) ? gq : 0.f;
+        gq = r2 < C.rc2_f ? gq : 0.f;                 // list entries never hold the atom itself
         ax = __builtin_fmaf(gq, dx, ax);
         ay = __builtin_fmaf(gq, dy, ay);
         az = __builtin_fmaf(gq, dz, az);
     };
     if (threadIdx.x < 64) {
         const int code = threadIdx.x;
-        shift_tab[code] = make_float4(image_shift(code, 0, C.lsize_f[0]), image_shift(code, 1, C.lsize_f[1]),
-                                      image_shift(code, 2, C.lsize_f[2]), 0.f);
+        shift_tab[code] = code == kFarCode
+                              ? make_float4(kFarShift, kFarShift, kFarShift, 0.f)
+                              : make_float4(image_shift(code, 0, C.lsize_f[0]), image_shift(code, 1, C.lsize_f[1]),
+                                            image_shift(code, 2, C.lsize_f[2]), 0.f);
     }
     int ent[kBatch];
     float4 pj[kBatch];
