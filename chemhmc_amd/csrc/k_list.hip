@@ -665,9 +665,17 @@ __device__ __forceinline__ double tile_sum_256(double s, double* part)
 __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
 {
     __shared__ double wpart[4];
+    __shared__ double shift64[64][3];                   // image shift by 6-bit image code, exact in FP64
     const int tile = xcd_block(blockIdx.x, v.ntiles);
     const Chunk tl = v.tiles[tile];
     const ChainDev& C = v.chains[tl.chain];
+    if (threadIdx.x < 64) {
+        const int code = threadIdx.x;
+        shift64[code][0] = (double)((code & 3) - 1) * C.lsize[0];
+        shift64[code][1] = (double)(((code >> 2) & 3) - 1) * C.lsize[1];
+        shift64[code][2] = (double)(((code >> 4) & 3) - 1) * C.lsize[2];
+    }
+    __syncthreads();
     const ListBuf L = list_buf(v, *list_cur(v, tl.chain));
     if (threadIdx.x == 0) {
         if (tl.begin == C.atom_off) *req(v, q, tl.chain) = 0;
@@ -687,21 +695,42 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
         // sqrt(r2) < rc (the reference's test) decided without the square root away from the cut-off:
         // the FP64 square root is within one ulp, far inside a 1e-12 relative band
         const double lo2 = C.rc2 * (1.0 - 1e-12), hi2 = C.rc2 * (1.0 + 1e-12);
-        for (int k = sub; k < n; k += 8) {
-            const int e = nbs[k];
-            const int j = e >> kImageBits, code = e & 63;
-            const double2 ja = r64[2 * (size_t)j], jb = r64[2 * (size_t)j + 1];
-            const double dx = xi - (ja.x + (double)((code & 3) - 1) * C.lsize[0]);
-            const double dy = yi - (ja.y + (double)(((code >> 2) & 3) - 1) * C.lsize[1]);
-            const double dz = zi - (jb.x + (double)(((code >> 4) & 3) - 1) * C.lsize[2]);
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            const bool inside = r2 < lo2 ? true : (r2 >= hi2 ? false : sqrt(r2) < C.rc);
-            if (inside && r2 > 0.0) {                          // identical_particles.cpp:432-435
-                const double s2 = C.sigma2 / r2;
-                const double s6 = s2 * s2 * s2;
-                au += s6 * (s6 - 1.0);
-                aw += s6 * (s6 - 0.5);
-                an += 1.0;
+        // batches of kEB entries per lane: the batch's entries, then its neighbours' coordinates, are
+        // loaded together (two memory round trips per batch); each lane still adds its entries in
+        // increasing order
+        constexpr int kEB = 8;
+        for (int k0 = sub; k0 < n; k0 += 8 * kEB) {
+            int ent[kEB];
+#pragma unroll
+            for (int u = 0; u < kEB; u++) ent[u] = nbs[min(k0 + 8 * u, v.nbcap - 1)];
+            double2 ja[kEB];
+            double jz[kEB];
+#pragma unroll
+            for (int u = 0; u < kEB; u++) {
+                const size_t j = (unsigned)ent[u] >> kImageBits;
+                ja[u] = r64[2 * j];
+                jz[u] = v.rel64[4 * j + 2];
+            }
+#pragma unroll
+            for (int u = 0; u < kEB; u++) {
+                const bool live = k0 + 8 * u < n;
+                const double* sh = shift64[ent[u] & 63];
+                const double dx = xi - (ja[u].x + sh[0]);
+                const double dy = yi - (ja[u].y + sh[1]);
+                const double dz = zi - (jz[u] + sh[2]);
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                bool inside = r2 < lo2;
+                // the square root only where some lane of the wave is within the band (a uniform branch)
+                const bool band = live && !inside && r2 < hi2;
+                if (__builtin_amdgcn_ballot_w64(band) != 0ull)
+                    if (band) inside = sqrt(r2) < C.rc;
+                if (live && inside && r2 > 0.0) {              // identical_particles.cpp:432-435
+                    const double s2 = C.sigma2 / r2;
+                    const double s6 = s2 * s2 * s2;
+                    au += s6 * (s6 - 1.0);
+                    aw += s6 * (s6 - 0.5);
+                    an += 1.0;
+                }
             }
         }
         if (sub == 0) {
