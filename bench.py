@@ -35,6 +35,11 @@ PEAK_HBM_GBS = 8000.0
 # list sweep bytes per atom besides its list row: own FP32 coordinates 16, entry count 4, p 24 + 24 and
 # x 24 + 24 (read and written, slot order), rebuild reference 16, next FP32 coordinates 16
 SWEEP_BYTES_PER_ATOM = 148.0
+# HBM traffic of one list sweep (force + kick + drift) measured with rocprofv3 --pmc FETCH_SIZE and
+# --pmc WRITE_SIZE (medians over 2850 dispatches, KiB; a plain copy reads 1.00 for WRITE_SIZE) at
+# N=32768, skin 0.6, 8 lanes per atom (profiles/lj_list_sweep_N32768.md): reported as the roofline's
+# traffic when the run's tuned skin and lanes are those, null otherwise
+SWEEP_TRAFFIC_PROFILED = {(0.6, 8): (9976.0 + 2048.0) * 1024.0}
 
 
 def _omp_threads(n):
@@ -338,7 +343,8 @@ def main():
         "list_rebuilds_per_traj": rebuilds_per_traj,
         "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": None, "bytes_per_launch": nbytes, "flops_per_launch": flops,
+                     "traffic": SWEEP_TRAFFIC_PROFILED.get((round(float(skin), 2), int(lanes))),
+                     "bytes_per_launch": nbytes, "flops_per_launch": flops,
                      "tflops": flops / (ms_sweep * 1e-3) / 1e12, "ms_per_launch": ms_sweep,
                      "list_entries_per_atom": entries / N, "n_int_per_atom": nint / N,
                      "cell_candidates_per_atom": ncand_cell / N, "ms_rebuild": ms_rebuild},
