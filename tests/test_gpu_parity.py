@@ -277,6 +277,36 @@ def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
     assert np.all(dev[odd] < 2 * fcut + 5e-5 * frms), dev[odd]
 
 
+def test_list_sweep_lanes_agree(chmc):
+    """The list sweep at 4, 8 and 16 lanes per atom (first batches of 64, 112 and 112 entries read
+    without a count test from rows padded to a fixed span) gives the cell sweep's forces, also when the
+    lanes change after the lists were built (the padding does not depend on them).  The lane counts
+    differ only in the FP32 summation order."""
+    N, T = 4000, 0.722
+    e, L = make(chmc, N, 0.8442, T=T)
+    ec, _ = make(chmc, N, 0.8442, T=T, cell_md=True)
+    e.heatbath(9, 1)
+    e.integrate(6, 0.01, 10, 0.1932)                 # lists rebuilt along the way
+    ec.set_positions(e.positions())
+    ec.compute_force()
+    fc = ec.force()
+    frms = np.sqrt(np.mean(fc ** 2))
+    fcut = abs(48.0 * RC ** -6 * (RC ** -6 - 0.5) / RC)
+    forces = {}
+    for lanes in (8, 4, 16, 8):
+        e.set_list_lanes(lanes)
+        assert e.list_lanes() == lanes
+        e.integrate(6, 0.0, 1, 0.1932)               # dt = 0: one stored force of the list path
+        fl = e.force()
+        dev = np.max(np.abs(fl - fc), axis=1)
+        odd = np.flatnonzero(dev >= 5e-5 * frms)
+        print(f"lanes {lanes}: max deviation from the cell sweep {dev.max():.2e} (rms force {frms:.2f})")
+        assert len(odd) <= 2 and np.all(dev[odd] < 2 * fcut + 5e-5 * frms), (lanes, odd, dev[odd])
+        if lanes in forces:
+            assert np.array_equal(fl, forces[lanes])  # same lanes, same positions: bitwise
+        forces[lanes] = fl
+
+
 @pytest.mark.parametrize("N,rho,nb", [(512, 0.8, 200), (4000, 0.8442, 500)])
 def test_rdf_counts_bitexact(chmc, orc, N, rho, nb):
     e, L = make(chmc, N, rho)
