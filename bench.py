@@ -103,16 +103,17 @@ def other_workload(args):
     t0 = time.perf_counter()
     e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
     t_warm = time.perf_counter() - t0
-    # skin autotuner, as for the headline workload (wall time of a few trajectories per skin)
+    # skin autotuner, as for the headline workload (wall time of a few trajectories per skin), unless
+    # --skin fixes it (profiling runs); lanes per atom tuned unless --lanes fixes them
     skins = {}
-    for k, sk in enumerate((0.3, 0.4, 0.5, 0.6)):
+    for k, sk in enumerate((0.3, 0.4, 0.5, 0.6) if args.skin is None else ()):
         e.set_skin(sk)
         e.hmc(2, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=20 + k, use_graph=not args.no_graph)
         t1 = time.perf_counter()
         e.hmc(8, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=30 + k, use_graph=not args.no_graph)
         skins[sk] = (time.perf_counter() - t1) / 8 * 1e3
-    e.set_skin(min(skins, key=skins.get))
-    e.set_list_lanes(0)
+    e.set_skin(min(skins, key=skins.get) if args.skin is None else args.skin)
+    e.set_list_lanes(args.lanes)
     t0 = time.perf_counter()
     rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=not args.no_graph)
     el = time.perf_counter() - t0

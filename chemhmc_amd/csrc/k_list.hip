@@ -116,48 +116,84 @@ __global__ __launch_bounds__(256) void k_lcount(DevView v, int q, int cb)
 
 // chains with more list cells than kScanCells: one workgroup scans all counts (zero in chains that did
 // not count) and each rebuilding chain's cells get atom_off + the scan relative to the chain's first
-// cell, in the list offsets of the buffer being rebuilt
+// cell, in the list offsets of the buffer being rebuilt.  The workgroup runs on one CU, so every global
+// access is coalesced: rounds of 8192 counts go through LDS, each thread scans 8 contiguous ones, a
+// block scan over the threads and a running carry join the rounds.
 __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q, int cb)
 {
     if (*req_any(v, q) == 0) return;
+    constexpr int T = 1024, PT = 8;
+    __shared__ int buf[T * PT];
     __shared__ int wsum[16];
+    __shared__ int carry;
     const int* __restrict__ cnt = v.lcount + (size_t)cb * v.Cl;
-    const int nt = blockDim.x, tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = v.Cl;
-    const int per = (n + nt - 1) / nt;
-    const int beg = min(tid * per, n), end = min(beg + per, n);
-    int s = 0;
-    for (int c = beg; c < end; c++) s += cnt[c];
-    const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
-    int incl = s;
+    if (tid == 0) carry = 0;
+    for (int r0 = 0; r0 < n; r0 += T * PT) {
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    if (wid == 0) {
-        int w = lane < nw ? wsum[lane] : 0;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            const int t = __shfl_up(w, o, 64);
-            if (lane >= o) w += t;
+        for (int k = 0; k < PT; k++) {
+            const int c = r0 + k * T + tid;
+            buf[k * T + tid] = c < n ? cnt[c] : 0;
         }
-        if (lane < nw) wsum[lane] = w;
+        __syncthreads();
+        int vals[PT];
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            vals[k] = buf[tid * PT + k];
+            s += vals[k];
+        }
+        int incl = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        if (wid == 0) {
+            int w = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const int t = __shfl_up(w, o, 64);
+                if (lane >= o) w += t;
+            }
+            if (lane < 16) wsum[lane] = w;
+        }
+        __syncthreads();
+        int run = carry + incl - s + (wid > 0 ? wsum[wid - 1] : 0);
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            buf[tid * PT + k] = run;
+            run += vals[k];
+        }
+        __syncthreads();                      // every thread has read carry; buf holds the round's scan
+        if (tid == T - 1) carry = run;
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int c = r0 + k * T + tid;
+            if (c < n) v.lscan_tmp[c] = buf[k * T + tid];
+        }
+        __syncthreads();                      // buf free, carry final
     }
-    __syncthreads();
-    int run = incl - s + (wid > 0 ? wsum[wid - 1] : 0);
-    for (int c = beg; c < end; c++) {
-        v.lscan_tmp[c] = run;
-        run += cnt[c];
-    }
-    __syncthreads();
-    for (int c = beg; c < end; c++) {
+    // list offsets of the rebuilding chains (chains are contiguous in cell space: a thread's cells change
+    // chain rarely, so the chain's flag, buffer and base are looked up once per change)
+    int kprev = -1, base = 0;
+    bool rq = false;
+    int* loffs = nullptr;
+#pragma unroll 4
+    for (int c = tid; c < n; c += T) {
         const int k = v.lcell_chain[c];
-        if (*req(v, q, k) == 0) continue;
-        const ChainDev& C = v.chains[k];
-        list_buf(v, *list_start(v, k) ^ 1).loffs[c] = C.atom_off + v.lscan_tmp[c] - v.lscan_tmp[C.lcell_off];
+        const int sc = v.lscan_tmp[c];
+        if (k != kprev) {
+            kprev = k;
+            rq = *req(v, q, k) != 0;
+            const ChainDev& C = v.chains[k];
+            base = C.atom_off - v.lscan_tmp[C.lcell_off];
+            loffs = list_buf(v, *list_start(v, k) ^ 1).loffs;
+        }
+        if (rq) loffs[c] = base + sc;
     }
 }
 

@@ -228,8 +228,8 @@ def test_integrator_fp32_config2(chmc, orc):
     assert np.max(dx) < 2e-4, np.max(dx)
 
 
-@pytest.mark.parametrize("N,rho,Lb", [(32768, 0.8442, None), (131072, 0.8442, None), (125, 0.58, None),
-                                      (51, 0.8, None), (100, None, (8.0, 6.0, 4.5))])
+@pytest.mark.parametrize("N,rho,Lb", [(32768, 0.8442, None), (131072, 0.8442, None), (262144, 0.8442, None),
+                                      (125, 0.58, None), (51, 0.8, None), (100, None, (8.0, 6.0, 4.5))])
 def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
     """Neighbour-list MD (skin lists, rebuilt on displacement) against the 27-cell sweep with a
     re-binning per evaluation and against the FP64 oracle: 10 OMF2 steps from the same start.  Both FP32
@@ -237,7 +237,8 @@ def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
     small boxes put one or two list cells along a side, where the lists hold several images of a pair
     (the anisotropic box holds 100 atoms: its 300-atom lattice of the force tests, at rho = 1.39 with
     0.64 sigma spacing, blows up under OMF2 dt = 0.01 in any arithmetic).  N = 131072 has 19^3 list
-    cells, more than one build workgroup scans itself, so its rebuilds take the separate scan kernel."""
+    cells, more than one build workgroup scans itself, so its rebuilds take the separate scan kernel;
+    N = 262144 has 22^3 list cells, more than one 8192-count round of that kernel."""
     T = 0.722 if N >= 32768 else 1.0
     e, L = _eng(chmc, N, rho, Lb, T=T)
     ec, _ = _eng(chmc, N, rho, Lb, T=T, cell_md=True)
