@@ -16,14 +16,18 @@
 //             ranked by atom id give the compact slots and their slot data (x0, r0, FP32 xsl); the 27
 //             neighbour cells' members, ranked the same way, are staged in LDS in slot order with the
 //             image shift dd * lsize folded in (the way k_lj stages its j-list), and each own atom's
-//             list of the entries within rc + skin is appended in slot order
-// Sweep: 32 slots x LP lanes per workgroup (LP = 4, 8 or 16, tuned); each lane walks every LP-th list
-//   entry, gathers the neighbour's FP32 coordinates, and the partial sums meet in a butterfly.  Three
-//   lanes of an atom then kick p in FP64, optionally drift x in FP64 (identical_particles.cpp:651-673),
-//   and lane 0 writes the FP32 coordinates of the other parity and raises the rebuild request when the
-//   atom is beyond skin / 2 from x0.
+//             list of the entries within rc + skin is appended in slot order; rows are then padded
+//             past their count to kListSpan entries with no-op entries (the slot itself with the
+//             image code kFarCode)
+// Sweep: 32 slots x LP lanes per workgroup (LP = 4, 8 or 16, tuned); each lane reads its first
+//   kBatch entries without a count test (the padding), gathers the neighbours' FP32 coordinates, adds
+//   the image shift from an LDS table by 6-bit code (kFarCode: 1e18 away), walks the rest of the list,
+//   and the partial sums meet in a butterfly.  Three lanes of an atom then kick p in FP64, optionally
+//   drift x in FP64 (identical_particles.cpp:651-673), and lane 0 writes the FP32 coordinates of the
+//   other parity and raises the rebuild request when the atom is beyond skin / 2 from x0.
 // Energy: FP64 coordinates relative to the rebuild cell origins (rel64 = r0 + wrap(x - x0)), the
-//   potential's r < rc test (identical_particles.cpp:432-435), per-tile fixed-order partials.
+//   potential's r < rc test (identical_particles.cpp:432-435), entries in batches of 8 per lane,
+//   per-tile fixed-order partials.
 #include "chmc_internal.hpp"
 
 #pragma clang fp contract(off)
