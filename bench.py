@@ -40,6 +40,8 @@ SWEEP_BYTES_PER_ATOM = 148.0
 # N=32768, skin 0.6, 8 lanes per atom (profiles/lj_list_sweep_N32768.md): reported as the roofline's
 # traffic when the run's tuned skin and lanes are those, null otherwise
 SWEEP_TRAFFIC_PROFILED = {(0.6, 8): (9976.0 + 2048.0) * 1024.0}
+# the same at N=1,048,576 (workload c5), skin 0.5, 8 lanes (profiles/lj_list_sweep_N1048576.md)
+SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8): (322982.0 + 65536.0) * 1024.0}
 
 
 def _omp_threads(n):
@@ -140,7 +142,9 @@ def other_workload(args):
                    "ms_per_traj_by_skin": skins, "warmup_seconds": t_warm},
         "acceptance": float(nacc.sum()) / (steps * len(w.chains)), "ess_per_s_sum_over_chains": ess / el,
         "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": (SWEEP_TRAFFIC_PROFILED_C5.get((round(float(e.skin()), 2), int(e.list_lanes())))
+                                 if args.workload == "c5" else None),
                      "bytes_per_launch": nbytes, "flops_per_launch": flops, "tflops": flops / (ms_sweep * 1e-3) / 1e12,
                      "ms_per_launch": ms_sweep, "ms_rebuild": ms_rebuild,
                      "list_entries_per_atom": ent / nat, "n_int_per_atom": nint / nat},
