@@ -202,7 +202,7 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q, int cb)
 }
 
 constexpr int kLMaxJ = 1024;      // staged neighbour members per pass (a cell never spans two passes)
-constexpr int kJPad = 32;         // far entries past a pass's members: one iteration of the pair test
+constexpr int kJPad = 64;         // far entries past a pass's members: one iteration of the pair test
 
 // exclusive scan of n <= kScanCells values in LDS by one 256-thread workgroup
 __device__ __forceinline__ void block_scan_256(int* a, int n, int* wsum)
@@ -250,7 +250,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int c = xcd_block(blockIdx.x, v.Cl);
     const int ch = v.lcell_chain[c];
     if (*req_any(v, q) == 0 || *req(v, q, ch) == 0) return;
-    constexpr int IS = 4, JG = 16, NT = 256, W = 4, PT = (kLMaxJ + NT - 1) / NT;
+    // pair tests: IS own atoms x JG staged members per wave; two own atoms per chunk keep the four waves
+    // of a cell's ~25-40 atoms evenly loaded (measured against four: 4.6 % less build time at N=32768)
+    constexpr int IS = 2, JG = 32, NT = 256, W = 4, PT = (kLMaxJ + NT - 1) / NT;
     // staged j in slot order: coordinates relative to the i-cell origin, list entry in w; padded past the
     // pass's members to a whole iteration of the test loop with far-away entries
     __shared__ float4 sj[kLMaxJ + kJPad];
