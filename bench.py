@@ -70,7 +70,7 @@ def cpu_baseline(seconds_budget=20.0):
         s.hmc(1, 0, oracle.OMF2, DT, NSTEPS, lam=LAM, seed=1 + ntraj)
         ntraj += 1
         el = time.perf_counter() - t0
-        if el > seconds_budget or ntraj >= 50:
+        if el > seconds_budget or ntraj >= 2000:
             break
     out = {"value": N * NSTEPS * ntraj / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
            "sample": f"{ntraj} OMF2 trajectories of the N=32768 workload (FP64 oracle, O(N) binning), {el:.1f} s"}
