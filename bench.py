@@ -152,16 +152,12 @@ def other_workload(args):
     print(json.dumps(out))
 
 
-def surrogate_workload(args):
-    """--workload c4: the learned surrogate trained on configurations of plain-HMC chains of the N=512
-    system (rho*=0.8, T*=1.5; 5000 configurations of 512 atoms), then HMC at N=32768, rho*=0.8442,
-    T*=0.722 (OMF2 10 x 0.01) with surrogate-driven MD and the exact FP64 LJ accept step, against plain
-    HMC from the same state: acceptance, ms per trajectory and ESS/s of U/N of both."""
-    import chemhmc_amd as chmc
-    from chemhmc_amd import stats
-
+def train_base_surrogate(args, chmc):
+    """The surrogate of the c4 benchmark: 32-64-64-1 trained on 5000 configurations of plain-HMC chains
+    of the N=512 system (rho*=0.8, T*=1.5; 20 chains x 250 snapshots).  Returns its parameters, the
+    training rows, the collect and train seconds and the loss history."""
     rng = np.random.default_rng(4)
-    nch, nconf_per = 20, 250                       # 20 chains x 250 snapshots = 5000 configurations
+    nch, nconf_per = 20, 250
     Lc1 = (512 / 0.8) ** (1 / 3)
     tr = chmc.Engine([chmc.Chain(N=512, L=Lc1, cutoff=RC, beta=1 / 1.5)] * nch)
     tr.init_lattice(3, 0.05)
@@ -172,6 +168,7 @@ def surrogate_workload(args):
     tr.surrogate_set_params(p0)
     tr.surrogate_reserve(nconf_per)
     t0 = time.perf_counter()
+    rows = 0
     for k in range(nconf_per):
         tr.hmc(2, 0, chmc.LEAPFROG, 0.005, 20, seed=100 + k)
         rows = tr.surrogate_collect()
@@ -186,7 +183,75 @@ def surrogate_workload(args):
     t_train = time.perf_counter() - t0
     params = tr.surrogate_params()
     tr.close()
+    return params, rows, t_collect, t_train, hist
 
+
+def surrogate_c5_workload(args):
+    """--workload c5s: the c4 surrogate fine-tuned on 200 configurations of the N = 1,048,576 system
+    (rho*=0.8442, T*=0.722; 210 M atom rows held on the device, 27 GB), then surrogate-driven HMC at that
+    size against plain HMC from the same state (OMF2 10 x 0.01): acceptance and ms per trajectory of
+    both, and the fine-tuning loss."""
+    import chemhmc_amd as chmc
+    from chemhmc_amd import configs
+
+    params, rows0, _, _, _ = train_base_surrogate(args, chmc)
+    w = configs.WORKLOADS["c5"]()
+    nat = w.natoms
+    e = chmc.Engine(w.chains)
+    e.init_lattice(1, 0.05)
+    warm = args.warmup if args.warmup_given else 10
+    e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
+    e.set_skin(0.5)
+    e.surrogate_set_params(params)
+    nconf = 200
+    e.surrogate_reserve(nconf)
+    t0 = time.perf_counter()
+    rows = 0
+    for k in range(nconf):
+        e.hmc(1, 1, w.scheme, w.dt, w.steps, lam=w.lam, seed=200 + k, use_graph=not args.no_graph)
+        rows = e.surrogate_collect()
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = e.surrogate_train(args.finetune_steps, 4096, lr=3e-4, seed=7)
+    t_train = time.perf_counter() - t0
+    x0 = e.positions()
+    steps = args.steps if args.steps_given else 20
+    out = {}
+    for name, surr in (("plain", False), ("surrogate", True)):
+        e.set_positions(x0)
+        e.set_trajectory(warm + nconf)
+        e.kinetic()
+        t0 = time.perf_counter()
+        rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=not args.no_graph,
+                          surrogate=surr, fallback_window=0)
+        el = time.perf_counter() - t0
+        out[name] = {"acceptance": float(nacc[0]) / steps, "ms_per_traj": el / steps * 1e3,
+                     "u_per_n": float(rec["u"][:, 0].mean() / nat), "value": nat * w.steps * steps / el}
+    res = {
+        "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N", "value": out["surrogate"]["value"],
+        "unit": "atom-steps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": out["surrogate"]["ms_per_traj"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 MFMA surrogate MD, f64 LJ accept", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
+        "config": {"workload": "c5s: surrogate HMC N=1048576 rho=0.8442 T=0.722 OMF2(0.1932) 10x0.01; the c4 surrogate "
+                               "fine-tuned on 200 configurations of this system",
+                   "base_train_rows": rows0, "finetune_rows": rows, "finetune_steps": args.finetune_steps,
+                   "finetune_batch": 4096, "finetune_lr": 3e-4, "collect_seconds": t_collect, "finetune_seconds": t_train,
+                   "finetune_mse_first": float(hist[:20].mean()), "finetune_mse_last": float(hist[-100:].mean())},
+        "plain": out["plain"], "surrogate": out["surrogate"],
+        "ms_per_traj_ratio_surrogate_over_plain": out["surrogate"]["ms_per_traj"] / out["plain"]["ms_per_traj"],
+    }
+    print(json.dumps(res))
+
+
+def surrogate_workload(args):
+    """--workload c4: the learned surrogate trained on configurations of plain-HMC chains of the N=512
+    system (rho*=0.8, T*=1.5; 5000 configurations of 512 atoms), then HMC at N=32768, rho*=0.8442,
+    T*=0.722 (OMF2 10 x 0.01) with surrogate-driven MD and the exact FP64 LJ accept step, against plain
+    HMC from the same state: acceptance, ms per trajectory and ESS/s of U/N of both."""
+    import chemhmc_amd as chmc
+    from chemhmc_amd import stats
+
+    params, rows, t_collect, t_train, hist = train_base_surrogate(args, chmc)
     L = (N / RHO) ** (1 / 3)
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
     e.init_lattice(1, 0.05)
@@ -266,9 +331,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every trajectory directly instead of replaying the captured graph")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
-                    help="c2 (default, the headline line), c3 batched chains, c4 surrogate vs plain, c5 N=1M")
-    ap.add_argument("--train-steps", type=int, default=4000, help="c4: Adam steps of the surrogate")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c5s"],
+                    help="c2 (default, the headline line), c3 batched chains, c4 surrogate vs plain, c5 N=1M, "
+                         "c5s the surrogate fine-tuned and run at N=1M")
+    ap.add_argument("--train-steps", type=int, default=4000, help="c4, c5s: Adam steps of the surrogate")
+    ap.add_argument("--finetune-steps", type=int, default=1000, help="c5s: Adam steps of the fine-tuning")
     ap.add_argument("--cell-md", action="store_true",
                     help="c2 through the 27-cell sweep with a re-binning per evaluation (CHMC_CELL_MD)")
     args = ap.parse_args()
@@ -281,6 +348,9 @@ def main():
 
     if args.workload == "c4":
         surrogate_workload(args)
+        return
+    if args.workload == "c5s":
+        surrogate_c5_workload(args)
         return
     if args.workload != "c2":
         other_workload(args)
