@@ -108,7 +108,7 @@ def other_workload(args):
     # skin autotuner, as for the headline workload (wall time of a few trajectories per skin), unless
     # --skin fixes it (profiling runs); lanes per atom tuned unless --lanes fixes them
     skins = {}
-    for k, sk in enumerate((0.3, 0.4, 0.5, 0.6) if args.skin is None else ()):
+    for k, sk in enumerate((0.3, 0.4, 0.5, 0.6, 0.7, 0.8) if args.skin is None else ()):
         e.set_skin(sk)
         e.hmc(2, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=20 + k, use_graph=not args.no_graph)
         t1 = time.perf_counter()

@@ -706,6 +706,7 @@ static int list_setup(chmc_ctx* c, double skin)
         const double rho = cd.natoms / (cd.L[0] * cd.L[1] * cd.L[2]) * cd.sigma * cd.sigma * cd.sigma;
         const double dens = std::max(1.5, 1.5 * rho);
         cd.lcap = (int)std::ceil(dens * vol) + 16;
+        cd.lcap = std::min(cd.lcap, cd.natoms);     // a cell never holds more than the chain (small boxes)
         if (cd.lcap > 256)                          // one build thread per member of a cell
             return fail(c, CHMC_E_GEOMETRY, "chain " + std::to_string(k) + ": list cells too large for the list sweep (use CHMC_CELL_MD)");
         const double rls = rl / cd.sigma;
