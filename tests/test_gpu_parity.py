@@ -278,6 +278,39 @@ def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
     assert np.all(dev[odd] < 2 * fcut + 5e-5 * frms), dev[odd]
 
 
+def test_list_md_vs_cell_md_at_1m_atoms(chmc):
+    """The 1,048,576-atom workload (102^3 lattice sites, rho = 0.8442, T = 0.722) at full size, through
+    properties that do not need the oracle: 3 OMF2 steps of list MD (with its rebuilds and the scan
+    kernel of 42,875-cell chains) and of the 27-cell sweep from the same state stay within FP32
+    rounding of each other, the interacting-pair counts of the two paths are equal, and the stored list
+    force matches the cell sweep's force at the final positions."""
+    N, T = 1048576, 0.722
+    e, L = make(chmc, N, 0.8442, T=T)
+    ec, _ = make(chmc, N, 0.8442, T=T, cell_md=True)
+    e.set_skin(0.5)
+    e.heatbath(9, 1)
+    ec.set_momenta(e.momenta())
+    e.integrate(6, 0.01, 3, 0.1932)
+    ec.integrate(6, 0.01, 3, 0.1932)
+    dx = np.abs(min_image(e.positions() - ec.positions(), L))
+    print(f"list vs cell after 3 OMF2 steps: rms {np.sqrt(np.mean(dx ** 2)):.2e} max {dx.max():.2e}")
+    assert np.sqrt(np.mean(dx ** 2)) < 1e-6
+    assert dx.max() < 2e-4
+    _, nint, _ = e.list_stats()
+    assert nint == e.pair_counts(0)[1]
+    e.integrate(6, 0.0, 1, 0.1932)
+    ec.set_positions(e.positions())
+    ec.compute_force()
+    fl, fc = e.force(), ec.force()
+    frms = np.sqrt(np.mean(fc ** 2))
+    fcut = abs(48.0 * RC ** -6 * (RC ** -6 - 0.5) / RC)
+    dev = np.max(np.abs(fl - fc), axis=1)
+    odd = np.flatnonzero(dev >= 5e-5 * frms)
+    print(f"force: max deviation {dev.max():.2e} (rms force {frms:.2f}); atoms beyond 5e-5 rms: {len(odd)}")
+    assert len(odd) <= N // 20000, odd
+    assert np.all(dev[odd] < 2 * fcut + 5e-5 * frms), dev[odd]
+
+
 def test_list_sweep_lanes_agree(chmc):
     """The list sweep at 4, 8 and 16 lanes per atom (first batches of 64, 112 and 112 entries read
     without a count test from rows padded to a fixed span) gives the cell sweep's forces, also when the
