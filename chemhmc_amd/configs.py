@@ -4,7 +4,8 @@ c1  plain HMC, N=512, rho*=0.8, T*=1.5, leapfrog 20 x 0.005, FP64 forces (the re
 c2  plain HMC, N=32768 (32^3), rho*=0.8442, T*=0.722, OMF2 (lambda=0.1932) 10 x 0.01, FP32 forces
 c3  256 independent chains: N log-uniform in [256, 8192] rounded to a cube, rho* ~ U(0.3, 0.9),
     T* ~ U(0.7, 2.0), leapfrog 15 x 0.005, one launch per stage for the whole batch
-c4  as c2 with the surrogate-driven trajectory (not built yet)
+c4  as c2 with the surrogate-driven trajectory (a trained per-atom MLP drives the MD, the exact FP64 LJ
+    Hamiltonian judges each Metropolis step; the benchmark's surrogate workload builds it)
 c5  N=1,048,576 (first N sites of a 102^3 lattice), rho*=0.8442, T*=0.722, OMF2 10 x 0.01
 
 The box side follows from N / rho (L = 33.8585 at c2; the benchmark text's 33.68 corresponds to

@@ -389,6 +389,61 @@ def test_hmc_graph_equals_direct_and_batch_independence(chmc):
     assert np.array_equal(rb2[:, 0], rb[:, 0]) and np.array_equal(rb2[:, 2], rb[:, 2])
 
 
+def test_batched_chains_at_full_size(chmc):
+    """The 256-chain batch (607,966 atoms: N log-uniform in [256, 8192] on cubes, rho* in [0.3, 0.9],
+    T* in [0.7, 2.0], leapfrog 15 x 0.005) at full size, through properties that need no oracle:
+    per chain, list MD and the 27-cell sweep from the same state stay within FP32 rounding of each
+    other and count the same interacting pairs; whole trajectories of the graph-replayed batch equal
+    the directly launched ones bitwise; and a chain's records do not change when every other chain of
+    the batch is replaced by a small one (each chain has its own cell grid, lists, rebuild decisions,
+    heat-bath and Metropolis streams), nor when it runs alone at index 0."""
+    from chemhmc_amd import configs
+    w = configs.c3()
+    nch = len(w.chains)
+    assert nch == 256 and w.natoms == 607966
+    e = chmc.Engine(w.chains)
+    ec = chmc.Engine(w.chains, cell_md=True)
+    for eng in (e, ec):
+        eng.init_lattice(1, 0.05)
+    e.heatbath(9, 1)
+    for k in range(nch):
+        ec.set_momenta(e.momenta(k), k)
+    e.integrate(chmc.LEAPFROG, w.dt, w.steps)
+    ec.integrate(chmc.LEAPFROG, w.dt, w.steps)
+    worst = 0.0
+    for k, c in enumerate(w.chains):
+        dx = np.abs(min_image(e.positions(k) - ec.positions(k), c.L))
+        worst = max(worst, dx.max())
+        assert np.sqrt(np.mean(dx ** 2)) < 1e-6, k
+        assert dx.max() < 2e-4, k
+        _, nint, _ = e.list_stats(k)
+        assert nint == e.pair_counts(k)[1], k
+    print(f"list vs cell over {nch} chains after {w.steps} leapfrog steps: max deviation {worst:.2e}")
+    e.close()
+    ec.close()
+
+    def run(chains, graph=True):
+        eng = chmc.Engine(chains)
+        eng.init_lattice(1, 0.05)
+        r, nacc = eng.hmc(4, 0, w.scheme, w.dt, w.steps, seed=12, use_graph=graph)
+        eng.close()
+        return r, nacc
+
+    rb, nacc = run(w.chains)
+    print(f"acceptance over the batch: {nacc.sum() / (4 * nch):.3f}")
+    assert np.all(np.isfinite(rb["dh"]))
+    rd, _ = run(w.chains, graph=False)
+    assert np.array_equal(rb, rd)
+    ns = np.array([c.N for c in w.chains])
+    keep = sorted({0, 1, 37, 128, 255, int(np.argmax(ns)), int(np.argmin(ns))})
+    small = chmc.Chain(N=216, L=box(216, 0.4), cutoff=RC, beta=1.2)
+    rs, _ = run([c if k in keep else small for k, c in enumerate(w.chains)])
+    for k in keep:
+        assert np.array_equal(rs[:, k], rb[:, k]), k
+    r0, _ = run([w.chains[0]])
+    assert np.array_equal(r0[:, 0], rb[:, 0])
+
+
 @pytest.mark.parametrize("fp64", [True, False])
 def test_hmc_records_vs_oracle(chmc, orc, fp64):
     """Whole trajectories (heat bath, LEAP MD, FP64 V and K, Metropolis, restore on reject) against the
