@@ -203,6 +203,7 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q, int cb)
 
 constexpr int kLMaxJ = 1024;      // staged neighbour members per pass (a cell never spans two passes)
 constexpr int kJPad = 64;         // far entries past a pass's members: one iteration of the pair test
+constexpr int kBuildWaves = 4;    // waves of a build workgroup
 
 // exclusive scan of n <= kScanCells values in LDS by one 256-thread workgroup
 __device__ __forceinline__ void block_scan_256(int* a, int n, int* wsum)
@@ -230,6 +231,54 @@ __device__ __forceinline__ void block_scan_256(int* a, int n, int* wsum)
     __syncthreads();
 }
 
+// U groups of 64 staged members from sj[j0] against the wave's own atom: the wave's hit mask straight
+// from the compares (4: ordered less, 2: ordered greater), the positions from mbcnt, the count in a
+// scalar.  Row offsets in 32 bits: Ntot * nbcap < 2^31 (checked at setup).
+template <int U>
+__device__ __forceinline__ void build_step(const float4* sj, int j0, int lane, const float4& xi, float rl2,
+                                           int rowoff, int nbcap, char* __restrict__ nbb, int& count)
+{
+    float4 p[U];
+    float r2[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        p[u] = sj[j0 + u * 64 + lane];
+        const float dx = xi.x - p[u].x, dy = xi.y - p[u].y, dz = xi.z - p[u].z;
+        r2[u] = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const unsigned long long hits = __builtin_amdgcn_fcmpf(r2[u], rl2, 4) & __builtin_amdgcn_fcmpf(r2[u], 0.f, 2);
+        const int pos = count + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(hits >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)hits, 0u));
+        if (r2[u] < rl2 && r2[u] > 0.f && pos < nbcap)
+            *reinterpret_cast<int*>(nbb + (unsigned)((rowoff + pos) << 2)) = __float_as_int(p[u].w);
+        count += __popcll(hits);
+    }
+}
+
+// Pair tests of one staged pass: the own atoms si[0, ni) (slot in w, entry counts in icount) against the
+// staged members sj[0, jn), padded with kJPad far entries.  Each wave takes one own atom at a time and
+// its 64 lanes test 64 members each step, branch-free (the far padding fails every test); the hits are
+// appended in staging order.  With one own atom per wave the hit mask is wave-uniform (~12 VALU per
+// tested pair; two own atoms x 32 members took ~18 and 6-9 % more build time).  The loop reads two
+// groups of 64 while both end within the padding and one for the rest, so the padding stays one group.
+__device__ __forceinline__ void build_pass(const float4* sj, const float4* si, int* icount, int ni, int jn,
+                                           float rl2, int nbcap, int* __restrict__ nb)
+{
+    static_assert(kJPad >= 64, "every step reads fewer than 64 entries past the members");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    char* __restrict__ nbb = reinterpret_cast<char*>(nb);
+    for (int i = wave; i < ni; i += kBuildWaves) {
+        const float4 xi = si[i];
+        int count = icount[i];
+        const int rowoff = __float_as_int(xi.w) * nbcap;
+        int j0 = 0;
+        for (; j0 + 64 < jn; j0 += 128) build_step<2>(sj, j0, lane, xi, rl2, rowoff, nbcap, nbb, count);
+        if (j0 < jn) build_step<1>(sj, j0, lane, xi, rl2, rowoff, nbcap, nbb, count);
+        if (lane == 0) icount[i] = count;
+    }
+}
+
 // One workgroup per list cell of a rebuilding chain; it writes the buffer that is not the trajectory's
 // start buffer and makes it current.
 //   offsets: SCAN = the chain's counts scanned in LDS (chains of at most kScanCells cells), else the
@@ -250,9 +299,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const int c = xcd_block(blockIdx.x, v.Cl);
     const int ch = v.lcell_chain[c];
     if (*req_any(v, q) == 0 || *req(v, q, ch) == 0) return;
-    // pair tests: IS own atoms x JG staged members per wave; two own atoms per chunk keep the four waves
-    // of a cell's ~25-40 atoms evenly loaded (measured against four: 4.6 % less build time at N=32768)
-    constexpr int IS = 2, JG = 32, NT = 256, W = 4, PT = (kLMaxJ + NT - 1) / NT;
+    constexpr int NT = 256, W = kBuildWaves, PT = (kLMaxJ + NT - 1) / NT;
     // staged j in slot order: coordinates relative to the i-cell origin, list entry in w; padded past the
     // pass's members to a whole iteration of the test loop with far-away entries
     __shared__ float4 sj[kLMaxJ + kJPad];
@@ -337,11 +384,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     if (ni == 0) return;
 
     const float rl2 = C.rl2_f;
-    const int ii = lane % IS, g = lane / IS;
-    unsigned long long pattern = 0;
-#pragma unroll
-    for (int k = 0; k < JG; k++) pattern |= 1ull << (k * IS + ii);
-    const int nchunk = (ni + IS - 1) / IS;
     const int nbcap = v.nbcap;
     int ibeg = 0;
     for (int k0 = 0; k0 < 27;) {
@@ -417,43 +459,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         }
         if (tid < kJPad) sj[jn + tid] = make_float4(kFarShift, kFarShift, kFarShift, 0.f);
         __syncthreads();
-        for (int chunk = wave; chunk < nchunk; chunk += W) {
-            const int i = chunk * IS + ii;
-            const bool ival = i < ni;
-            const float4 xi = si[ival ? i : ni - 1];
-            const int slot_i = __float_as_int(xi.w);
-            const int count0 = icount[ival ? i : ni - 1];
-            int count = count0;
-            // row offsets in 32 bits: Ntot * nbcap < 2^31 (checked at setup)
-            const int rowoff = slot_i * nbcap;
-            // branch-free test of every lane (the far padding fails every test); the hits of each i-lane
-            // group are appended in j order: position = count + hits of the group's lower lanes
-            constexpr int U = kJPad / JG;         // two LDS reads in flight per iteration
-            char* __restrict__ nbb = reinterpret_cast<char*>(L.nb);
-            for (int j0 = 0; j0 < jn; j0 += JG * U) {
-                float4 p[U];
-                float r2[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    p[u] = sj[j0 + u * JG + g];
-                    const float dx = xi.x - p[u].x, dy = xi.y - p[u].y, dz = xi.z - p[u].z;
-                    r2[u] = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    // the wave's hit mask straight from the compares (4: ordered less, 2: ordered greater);
-                    // the lanes of an i-lane group share i and so i < ni, and the position and count of
-                    // an i beyond ni are never used, so the mask needs no i < ni term
-                    const unsigned long long hits = __builtin_amdgcn_fcmpf(r2[u], rl2, 4) & __builtin_amdgcn_fcmpf(r2[u], 0.f, 2);
-                    const unsigned long long msk = hits & pattern;
-                    const int pos = count + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
-                    if (ival && r2[u] < rl2 && r2[u] > 0.f && pos < nbcap)
-                        *reinterpret_cast<int*>(nbb + (unsigned)((rowoff + pos) << 2)) = __float_as_int(p[u].w);
-                    count += __popcll(msk);
-                }
-            }
-            if (ival && g == 0) icount[i] = count;
-        }
+        build_pass(sj, si, icount, ni, jn, rl2, nbcap, L.nb);
         k0 = k1;
     }
     __syncthreads();
