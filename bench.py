@@ -94,28 +94,34 @@ def other_workload(args):
     """--workload c3 (256 batched chains, LEAP 15 x 0.005) or c5 (N = 1,048,576, OMF2 10 x 0.01): the
     same measurement on those configurations, one JSON line."""
     import chemhmc_amd as chmc
-    from chemhmc_amd import configs, stats
+    from chemhmc_amd import configs, stats, tune
 
     w = configs.WORKLOADS[args.workload]()
     e = chmc.Engine(w.chains)
     e.init_lattice(1, 0.05)
     nat = w.natoms
     steps = args.steps if args.steps_given else (200 if args.workload == "c3" else 50)
-    warm = args.warmup if args.warmup_given else 30
+    # the batch relaxes from its lattice start over the first ~100 trajectories (more rebuilds early)
+    warm = args.warmup if args.warmup_given else (150 if args.workload == "c3" else 30)
     t0 = time.perf_counter()
     e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
     t_warm = time.perf_counter() - t0
-    # skin autotuner, as for the headline workload (wall time of a few trajectories per skin), unless
-    # --skin fixes it (profiling runs); lanes per atom tuned unless --lanes fixes them
+    # skin autotuner (wall time of a few trajectories per candidate), unless --skin fixes one skin for
+    # every chain (profiling runs): a batch of chains searches per-chain skins scaled with the chain's
+    # density, a single chain one skin; lanes per atom tuned unless --lanes fixes them
     skins = {}
-    for k, sk in enumerate((0.3, 0.4, 0.5, 0.6, 0.7, 0.8) if args.skin is None else ()):
-        e.set_skin(sk)
-        e.hmc(2, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=20 + k, use_graph=not args.no_graph)
-        t1 = time.perf_counter()
-        e.hmc(8, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=30 + k, use_graph=not args.no_graph)
-        skins[sk] = (time.perf_counter() - t1) / 8 * 1e3
-    e.set_skin(min(skins, key=skins.get) if args.skin is None else args.skin)
+    e.set_list_lanes(args.lanes)        # the skins are timed with the lanes the run will use
+    if args.skin is None:
+        def run(n, sd):
+            e.hmc(n, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=sd, use_graph=not args.no_graph)
+        if len(w.chains) > 1:
+            _, skins = tune.tune_skins(e, run)
+        else:
+            _, skins = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), warm=2, reps=6)
+    else:
+        e.set_skin(args.skin)
     e.set_list_lanes(args.lanes)
+    chain_skins = np.array([e.chain_skin(k) for k in range(len(w.chains))])
     t0 = time.perf_counter()
     rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=not args.no_graph)
     el = time.perf_counter() - t0
@@ -139,6 +145,7 @@ def other_workload(args):
         "config": {"workload": f"{args.workload}: {len(w.chains)} chain(s), {nat} atoms, "
                                f"{'LEAP' if w.scheme == chmc.LEAPFROG else 'OMF2'} {w.steps}x{w.dt}",
                    "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "skin": e.skin(),
+                   "chain_skin_range": [float(chain_skins.min()), float(chain_skins.max())],
                    "ms_per_traj_by_skin": skins, "warmup_seconds": t_warm},
         "acceptance": float(nacc.sum()) / (steps * len(w.chains)), "ess_per_s_sum_over_chains": ess / el,
         "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,

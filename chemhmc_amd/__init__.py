@@ -104,6 +104,8 @@ def _declare(L):
         "chmc_time_force": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
         "chmc_set_skin": ([P, _D], _I),
         "chmc_get_skin": ([P, C.POINTER(_D)], _I),
+        "chmc_set_chain_skins": ([P, _PD], _I),
+        "chmc_get_chain_skin": ([P, _I, C.POINTER(_D)], _I),
         "chmc_list_stats": ([P, _I, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], _I),
         "chmc_time_list": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
         "chmc_set_list_lanes": ([P, _I], _I),
@@ -133,7 +135,8 @@ EXPORTED = (
     "chmc_evaluate_potential", "chmc_virial", "chmc_pair_counts", "chmc_kinetic", "chmc_update_momenta", "chmc_update_positions",
     "chmc_kick", "chmc_integrate", "chmc_heatbath", "chmc_save_positions", "chmc_restore_positions",
     "chmc_hmc_run", "chmc_rdf", "chmc_potential_energy_open", "chmc_set_force_tiling",
-    "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_list_stats",
+    "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_set_chain_skins",
+    "chmc_get_chain_skin", "chmc_list_stats",
     "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_trajectory", "chmc_get_trajectory",
     "chmc_surrogate_set_params", "chmc_surrogate_get_params",
     "chmc_surrogate_eval", "chmc_surrogate_features", "chmc_surrogate_reserve", "chmc_surrogate_collect",
@@ -405,6 +408,17 @@ class Engine:
     def skin(self):
         v = _D()
         self._ck(lib().chmc_get_skin(self._h, C.byref(v)), "get_skin")
+        return v.value
+
+    def set_chain_skins(self, skins):
+        s = np.ascontiguousarray(skins, np.float64)
+        if s.shape != (self.nchains,):
+            raise ValueError(f"set_chain_skins: {self.nchains} skins expected, got shape {s.shape}")
+        self._ck(lib().chmc_set_chain_skins(self._h, s), "set_chain_skins")
+
+    def chain_skin(self, chain=0):
+        v = _D()
+        self._ck(lib().chmc_get_chain_skin(self._h, int(chain), C.byref(v)), "get_chain_skin")
         return v.value
 
     def list_stats(self, chain=0):

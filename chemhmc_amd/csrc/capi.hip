@@ -55,7 +55,8 @@ struct chmc_ctx {
     int adam_step = 0;
     bool md_surr = false;        // MD kicks use the surrogate's forces (set for the trajectories of an HMC run)
     // neighbour-list MD (allocated on first use)
-    double skin = 0.3;
+    double skin = 0.3;           // the skin set for every chain, or the largest per-chain skin
+    std::vector<double> chain_skin;  // per-chain skins when set individually (empty: skin for all)
     bool list_ready = false;
     int lpar = 0;                // list parity: which FP32 coordinate buffer / rebuild flag is current
     int cpar = 0;                // list count buffer of the next rebuild launch (alternates per launch)
@@ -667,9 +668,10 @@ static hipError_t lalloc(chmc_ctx* c, T** p, size_t n)
     return e;
 }
 
-// per-chain list grids (cells of side >= rc + skin; the skin shrinks where a box side is shorter than
-// rc + skin), 32-slot tiles, and the list buffers
-static int list_setup(chmc_ctx* c, double skin)
+// per-chain list grids (cells of side >= rc + skin of the chain; the skin shrinks where a box side is
+// shorter than rc + skin), 32-slot tiles, and the list buffers.  skins: one per chain, or empty for
+// `skin` in every chain.
+static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins)
 {
     if (c->lstate) return fail(c, CHMC_E_STATE, "internal: list buffers replaced while list MD owns the state");
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -682,7 +684,7 @@ static int list_setup(chmc_ctx* c, double skin)
     std::vector<int> hlc;
     for (int k = 0; k < c->nchains; k++) {
         ChainDev& cd = c->hchains[k];
-        double rl = cd.rc + skin;
+        double rl = cd.rc + (skins.empty() ? skin : skins[k]);
         for (int d = 0; d < 3; d++) rl = std::min(rl, cd.L[d]);
         const double sk = rl - cd.rc;
         double vol = 1.0;
@@ -802,7 +804,8 @@ static int list_setup(chmc_ctx* c, double skin)
     }
     HIPCHK(c, hipMemset(v.lref, 0, sizeof(float4) * 2 * N));
     HIPCHK(c, hipDeviceSynchronize());
-    c->skin = skin;
+    c->skin = skins.empty() ? skin : *std::max_element(skins.begin(), skins.end());
+    c->chain_skin = skins;
     c->lpar = 0;
     c->cpar = 0;
     c->list_valid = false;
@@ -813,7 +816,7 @@ static int list_setup(chmc_ctx* c, double skin)
 
 static int list_ensure(chmc_ctx* c)
 {
-    return c->list_ready ? CHMC_OK : list_setup(c, c->skin);
+    return c->list_ready ? CHMC_OK : list_setup(c, c->skin, c->chain_skin);
 }
 
 // list MD takes the state over: x, p in the current buffer's slot order (xl, pl); h_x is taken over by
@@ -1730,13 +1733,33 @@ extern "C" int chmc_set_skin(chmc_ctx* c, double skin)
     if (!(skin > 0.0) || skin > 2.0) return fail(c, CHMC_E_ARG, "set_skin: 0 < skin <= 2 required");
     int rc = flush_drift(c);
     if (rc) return rc;
-    return list_setup(c, skin);
+    return list_setup(c, skin, {});
 }
 
 extern "C" int chmc_get_skin(const chmc_ctx* c, double* skin)
 {
     if (!c || !skin) return CHMC_E_ARG;
     *skin = c->skin;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_set_chain_skins(chmc_ctx* c, const double* skins)
+{
+    CHECK_CTX(c);
+    if (!skins) return CHMC_E_ARG;
+    std::vector<double> sk(skins, skins + c->nchains);
+    for (int k = 0; k < c->nchains; k++)
+        if (!(sk[k] > 0.0) || sk[k] > 2.0)
+            return fail(c, CHMC_E_ARG, "set_chain_skins: 0 < skin <= 2 required (chain " + std::to_string(k) + ")");
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    return list_setup(c, c->skin, sk);
+}
+
+extern "C" int chmc_get_chain_skin(const chmc_ctx* c, int chain, double* skin)
+{
+    if (!c || !skin || chain < 0 || chain >= c->nchains) return CHMC_E_ARG;
+    *skin = c->chain_skin.empty() ? c->skin : c->chain_skin[chain];
     return CHMC_OK;
 }
 

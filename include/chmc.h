@@ -222,6 +222,11 @@ int chmc_time_force(chmc_ctx* ctx, int reps, double* ms_bin, double* ms_force);
  * and the number of rebuilds run since the context was created. */
 int chmc_set_skin(chmc_ctx* ctx, double skin);
 int chmc_get_skin(const chmc_ctx* ctx, double* skin);
+/* one skin per chain (skins[nchains], each 0 < skin <= 2): every chain's list cells, capacities and
+ * rebuild threshold follow its own skin; set_skin sets them all back to one value.  get_skin then
+ * reports the largest.  No reference counterpart (a tuning parameter of the list MD above). */
+int chmc_set_chain_skins(chmc_ctx* ctx, const double* skins);
+int chmc_get_chain_skin(const chmc_ctx* ctx, int chain, double* skin);
 int chmc_list_stats(chmc_ctx* ctx, int chain, long long* entries, long long* nint, long long* rebuilds);
 /* average device time (ms) of the list sweep with kick + drift (the MD step of the trajectory) and of
  * one rebuild, over reps, at the current positions (the positions do not change: dt = 0) */

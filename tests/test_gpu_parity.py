@@ -444,6 +444,59 @@ def test_batched_chains_at_full_size(chmc):
     assert np.array_equal(r0[:, 0], rb[:, 0])
 
 
+def test_per_chain_skins(chmc):
+    """Chains of a batch with skins of their own (list cells, capacities and rebuild threshold per chain):
+    list MD stays within FP32 rounding of the 27-cell sweep and counts the same interacting pairs in every
+    chain; a chain's trajectories do not depend on the skins of the other chains (bitwise) and equal the
+    same chain run alone with its skin; set_skin gives every chain one skin again."""
+    chains = [chmc.Chain(N=512, L=box(512, 0.8), cutoff=RC, beta=1 / 1.5),
+              chmc.Chain(N=1000, L=box(1000, 0.4), cutoff=RC, beta=1.0),
+              chmc.Chain(N=343, L=box(343, 0.6), cutoff=RC, beta=1 / 1.2)]
+    skins = [0.3, 0.9, 0.55]
+    e = chmc.Engine(chains)
+    ec = chmc.Engine(chains, cell_md=True)
+    for eng in (e, ec):
+        eng.init_lattice(7, 0.05)
+    e.set_chain_skins(skins)
+    assert [e.chain_skin(k) for k in range(3)] == skins and e.skin() == 0.9
+    e.heatbath(9, 1)
+    for k in range(3):
+        ec.set_momenta(e.momenta(k), k)
+    e.integrate(chmc.LEAPFROG, 0.005, 60)
+    ec.integrate(chmc.LEAPFROG, 0.005, 60)
+    _, _, rebuilds = e.list_stats(0)
+    print(f"rebuilds over 60 steps (all chains): {rebuilds}")
+    for k, c in enumerate(chains):
+        dx = np.abs(min_image(e.positions(k) - ec.positions(k), c.L))
+        assert np.sqrt(np.mean(dx ** 2)) < 1e-6, k
+        assert dx.max() < 2e-4, k
+        _, nint, _ = e.list_stats(k)
+        assert nint == e.pair_counts(k)[1], k
+    with pytest.raises(chmc.ChmcError):
+        e.set_chain_skins([0.3, 0.0, 0.5])
+    with pytest.raises(ValueError):
+        e.set_chain_skins([0.3, 0.5])
+    assert [e.chain_skin(k) for k in range(3)] == skins       # a rejected call keeps the skins
+    e.set_skin(0.5)
+    assert [e.chain_skin(k) for k in range(3)] == [0.5] * 3 and e.skin() == 0.5
+    e.close()
+    ec.close()
+
+    def run(chs, sk):
+        eng = chmc.Engine(chs)
+        eng.init_lattice(7, 0.05)
+        eng.set_chain_skins(sk)
+        r, _ = eng.hmc(8, 0, chmc.LEAPFROG, 0.005, 20, seed=4, use_graph=True)
+        eng.close()
+        return r
+
+    ra = run(chains, skins)
+    rb = run(chains, [0.6, 0.9, 0.55])
+    rc = run(chains, [0.3, 0.4, 1.2])
+    assert np.array_equal(ra[:, 1:], rb[:, 1:]) and np.array_equal(ra[:, 0], rc[:, 0])
+    assert np.array_equal(run(chains[:1], [0.3])[:, 0], ra[:, 0])
+
+
 @pytest.mark.parametrize("fp64", [True, False])
 def test_hmc_records_vs_oracle(chmc, orc, fp64):
     """Whole trajectories (heat bath, LEAP MD, FP64 V and K, Metropolis, restore on reject) against the
