@@ -90,6 +90,21 @@ def cpu_baseline(seconds_budget=20.0):
     return out
 
 
+def tuner_snapshot(e):
+    """Positions of every chain and the trajectory counter: restored after the autotuners, so that the
+    timed run continues from the warmed-up state whatever the tuners ran (at the headline
+    configuration a chain accepts a few percent of its trajectories, and the rebuild rate depends on
+    the configuration it sits in)."""
+    return [e.positions(k) for k in range(e.nchains)], e.trajectory()
+
+
+def tuner_restore(e, snap):
+    xs, traj = snap
+    for k, x in enumerate(xs):
+        e.set_positions(x, k)
+    e.set_trajectory(traj)
+
+
 def other_workload(args):
     """--workload c3 (256 batched chains, LEAP 15 x 0.005) or c5 (N = 1,048,576, OMF2 10 x 0.01): the
     same measurement on those configurations, one JSON line."""
@@ -110,6 +125,7 @@ def other_workload(args):
     # every chain (profiling runs): a batch of chains searches per-chain skins scaled with the chain's
     # density, a single chain one skin; lanes per atom tuned unless --lanes fixes them
     skins = {}
+    snap = tuner_snapshot(e)
     e.set_list_lanes(args.lanes)        # the skins are timed with the lanes the run will use
     if args.skin is None:
         def run(n, sd):
@@ -120,6 +136,7 @@ def other_workload(args):
             _, skins = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), warm=2, reps=6)
     else:
         e.set_skin(args.skin)
+    tuner_restore(e, snap)
     e.set_list_lanes(args.lanes)
     chain_skins = np.array([e.chain_skin(k) for k in range(len(w.chains))])
     t0 = time.perf_counter()
@@ -364,7 +381,7 @@ def main():
         return
 
     import chemhmc_amd as chmc
-    from chemhmc_amd import stats
+    from chemhmc_amd import stats, tune
 
     L = (N / RHO) ** (1 / 3)
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T), cell_md=args.cell_md)
@@ -374,17 +391,23 @@ def main():
         return
     if args.warmup > 0:     # accept-all thermalisation away from the lattice (thermalization_steps)
         e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
-    # skin autotuner: wall time of a short run per skin on the equilibrated state
+    # skin autotuner: wall time of a short run per skin on the equilibrated state (each skin timed twice,
+    # in opposite orders), with the lanes per atom of the list sweep (0: autotuned) set first and again
+    # for the chosen skin
+    snap = tuner_snapshot(e)
+    e.set_list_lanes(args.lanes)
     skins = {}
-    for k, sk in enumerate((0.3, 0.4, 0.5, 0.6, 0.7, 0.8) if args.skin is None else ()):
-        e.set_skin(sk)
-        e.hmc(4, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=20 + k, use_graph=not args.no_graph)
-        t0 = time.perf_counter()
-        e.hmc(args.tune, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=30 + k, use_graph=not args.no_graph)
-        skins[sk] = (time.perf_counter() - t0) / args.tune * 1e3
-    skin = min(skins, key=skins.get) if args.skin is None else args.skin
-    e.set_skin(skin)
-    e.set_list_lanes(args.lanes)                # lanes per atom of the list sweep (0: autotuned)
+    if args.skin is None:
+        def run(n, sd):
+            e.hmc(n, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=sd, use_graph=not args.no_graph)
+        _, table = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), warm=4,
+                                   reps=args.tune)
+        skins = {float(k.split("/")[0]): v for k, v in table.items()}
+    else:
+        e.set_skin(args.skin)
+    skin = e.skin()
+    tuner_restore(e, snap)
+    e.set_list_lanes(args.lanes)
     lanes = e.list_lanes()
     _, _, rebuilds0 = e.list_stats()
 
