@@ -90,21 +90,6 @@ def cpu_baseline(seconds_budget=20.0):
     return out
 
 
-def tuner_snapshot(e):
-    """Positions of every chain and the trajectory counter: restored after the autotuners, so that the
-    timed run continues from the warmed-up state whatever the tuners ran (at the headline
-    configuration a chain accepts a few percent of its trajectories, and the rebuild rate depends on
-    the configuration it sits in)."""
-    return [e.positions(k) for k in range(e.nchains)], e.trajectory()
-
-
-def tuner_restore(e, snap):
-    xs, traj = snap
-    for k, x in enumerate(xs):
-        e.set_positions(x, k)
-    e.set_trajectory(traj)
-
-
 def other_workload(args):
     """--workload c3 (256 batched chains, LEAP 15 x 0.005) or c5 (N = 1,048,576, OMF2 10 x 0.01): the
     same measurement on those configurations, one JSON line."""
@@ -116,28 +101,28 @@ def other_workload(args):
     e.init_lattice(1, 0.05)
     nat = w.natoms
     steps = args.steps if args.steps_given else (200 if args.workload == "c3" else 50)
-    # the batch relaxes from its lattice start over the first ~100 trajectories (more rebuilds early)
-    warm = args.warmup if args.warmup_given else (150 if args.workload == "c3" else 30)
+    # both workloads relax from their lattice starts over the first ~100 trajectories (more rebuilds
+    # early); the tuners and the timed run start after that
+    warm = args.warmup if args.warmup_given else 150
     t0 = time.perf_counter()
     e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
     t_warm = time.perf_counter() - t0
-    # skin autotuner (wall time of a few trajectories per candidate), unless --skin fixes one skin for
-    # every chain (profiling runs): a batch of chains searches per-chain skins scaled with the chain's
-    # density, a single chain one skin; lanes per atom tuned unless --lanes fixes them
+    # skin autotuner (wall time of a few trajectories per candidate, paired from the warmed-up state,
+    # which it restores), unless --skin fixes one skin for every chain (profiling runs): a batch of
+    # chains searches per-chain skins scaled with the chain's density, a single chain one skin; lanes
+    # per atom tuned unless --lanes fixes them
     skins = {}
-    snap = tuner_snapshot(e)
-    e.set_list_lanes(args.lanes)        # the skins are timed with the lanes the run will use
     if args.skin is None:
         def run(n, sd):
             e.hmc(n, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=sd, use_graph=not args.no_graph)
         if len(w.chains) > 1:
-            _, skins = tune.tune_skins(e, run)
+            _, skins = tune.tune_skins(e, run, lanes=args.lanes)
         else:
-            _, skins = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), warm=2, reps=6)
+            _, skins = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,),
+                                       lanes=args.lanes)
     else:
         e.set_skin(args.skin)
-    tuner_restore(e, snap)
-    e.set_list_lanes(args.lanes)
+        e.set_list_lanes(args.lanes)
     chain_skins = np.array([e.chain_skin(k) for k in range(len(w.chains))])
     t0 = time.perf_counter()
     rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=not args.no_graph)
@@ -391,23 +376,21 @@ def main():
         return
     if args.warmup > 0:     # accept-all thermalisation away from the lattice (thermalization_steps)
         e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
-    # skin autotuner: wall time of a short run per skin on the equilibrated state (each skin timed twice,
-    # in opposite orders), with the lanes per atom of the list sweep (0: autotuned) set first and again
-    # for the chosen skin
-    snap = tuner_snapshot(e)
-    e.set_list_lanes(args.lanes)
+    # skin autotuner: wall time of a short run per skin from the equilibrated state (paired: every skin
+    # starts from that state with the same random streams, and the state is restored afterwards, so the
+    # timed run continues from it whatever the tuner ran), with the lanes per atom of the list sweep
+    # (0: autotuned) chosen for every skin
     skins = {}
     if args.skin is None:
         def run(n, sd):
             e.hmc(n, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=sd, use_graph=not args.no_graph)
-        _, table = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), warm=4,
-                                   reps=args.tune)
+        _, table = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), reps=args.tune,
+                                   lanes=args.lanes)
         skins = {float(k.split("/")[0]): v for k, v in table.items()}
     else:
         e.set_skin(args.skin)
+        e.set_list_lanes(args.lanes)
     skin = e.skin()
-    tuner_restore(e, snap)
-    e.set_list_lanes(args.lanes)
     lanes = e.list_lanes()
     _, _, rebuilds0 = e.list_stats()
 

@@ -28,19 +28,31 @@ def density_scaled_skins(chains, base, exponent, rho_ref=0.6, lo=0.4, hi=2.0):
     return np.minimum(base * np.clip((rho_ref / rho) ** exponent, lo, hi), 2.0)
 
 
-def tune_skins(engine, run, bases=(0.35, 0.4, 0.45, 0.5, 0.6), exponents=(0.0, 1.0, 1.5), warm=8, reps=12,
-               seed=1000):
-    """Time `run(ntraj, seed)` (trajectories of the engine's workload) for every (base, exponent): warm
-    trajectories after the skins change (list set-up, graph capture, and the first rebuild interval:
-    lists are fresh after a change, so a short window would miss the rebuilds of a large skin and
-    favour it), then reps timed ones.  The
-    candidates are timed twice, in one order and then in the reverse order, and the two times summed,
-    so that a steady drift of the state (a batch still relaxing from its start) does not favour the
-    candidates timed last.  Set the list lanes first: with more lanes per atom than a sparse chain
-    needs, list entries cost little and larger skins look cheaper than they are at the right lanes.
-    The engine keeps the fastest skins.  Returns (skins, {"base/exponent": ms
-    per trajectory})."""
-    cands = []
+def snapshot(engine):
+    """Positions of every chain and the trajectory counter (the heat bath's stream position)."""
+    return [engine.positions(k) for k in range(engine.nchains)], engine.trajectory()
+
+
+def restore(engine, snap):
+    xs, traj = snap
+    for k, x in enumerate(xs):
+        engine.set_positions(x, k)
+    engine.set_trajectory(traj)
+
+
+def tune_skins(engine, run, bases=(0.35, 0.4, 0.45, 0.5, 0.6), exponents=(0.0, 1.0, 1.5), warm=0, reps=24,
+               seed=1000, lanes=0):
+    """Time `run(ntraj, seed)` (trajectories of the engine's workload) for every (base, exponent) as a
+    paired comparison: each candidate starts from the same positions and trajectory counter and runs the
+    same random streams, warm untimed trajectories and then reps timed ones in one call, as a timed run
+    that follows the tuner starts (fresh lists, then a rebuild whenever an atom has moved skin / 2; a
+    window that skips the first rebuild interval misranks the skins).  The lanes per atom of the list
+    sweep are set for every candidate (`lanes`, 0: autotuned at that candidate's lists), since the best
+    lanes change with the list length and a skin timed at the wrong lanes is misjudged.  The engine is
+    returned to the starting state with the fastest skins and their lanes.  Returns (skins,
+    {"base/exponent": ms per trajectory})."""
+    snap = snapshot(engine)
+    table, best = {}, None
     for ex in exponents:
         for b in bases:
             skins = density_scaled_skins(engine.chains, b, ex)
@@ -48,19 +60,20 @@ def tune_skins(engine, run, bases=(0.35, 0.4, 0.45, 0.5, 0.6), exponents=(0.0, 1
                 engine.set_chain_skins(skins)
             except ChmcError:
                 continue
-            cands.append((f"{b}/{ex}", skins))
-    if not cands:
-        raise ChmcError("tune_skins: no candidate skin fits the list MD")
-    table = {name: 0.0 for name, _ in cands}
-    k = 0
-    for order in (cands, cands[::-1]):
-        for name, skins in order:
-            engine.set_chain_skins(skins)
-            run(warm, seed + 2 * k)
+            restore(engine, snap)
+            engine.set_list_lanes(lanes)
+            lp = engine.list_lanes()
+            if warm > 0:
+                run(warm, seed)
             t0 = time.perf_counter()
-            run(reps, seed + 2 * k + 1)
-            table[name] += (time.perf_counter() - t0) / reps * 1e3 / 2
-            k += 1
-    name, skins = min(cands, key=lambda c: table[c[0]])
-    engine.set_chain_skins(skins)
-    return skins, table
+            run(reps, seed + 1)
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            table[f"{b}/{ex}"] = ms
+            if best is None or ms < best[0]:
+                best = (ms, skins, lp)
+    if best is None:
+        raise ChmcError("tune_skins: no candidate skin fits the list MD")
+    engine.set_chain_skins(best[1])
+    restore(engine, snap)
+    engine.set_list_lanes(best[2])
+    return best[1], table
