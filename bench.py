@@ -37,9 +37,9 @@ PEAK_HBM_GBS = 8000.0
 SWEEP_BYTES_PER_ATOM = 148.0
 # HBM traffic of one list sweep (force + kick + drift) measured with rocprofv3 --pmc FETCH_SIZE and
 # --pmc WRITE_SIZE (medians over 2850 dispatches, KiB; a plain copy reads 1.00 for WRITE_SIZE) at
-# N=32768, skin 0.6, 8 lanes per atom (profiles/lj_list_sweep_N32768.md): reported as the roofline's
+# N=32768, skins 0.6 and 0.7, 8 lanes per atom (profiles/lj_list_sweep_N32768.md): reported as the roofline's
 # traffic when the run's tuned skin and lanes are those, null otherwise
-SWEEP_TRAFFIC_PROFILED = {(0.6, 8): (9976.0 + 2048.0) * 1024.0}
+SWEEP_TRAFFIC_PROFILED = {(0.6, 8): (9976.0 + 2048.0) * 1024.0, (0.7, 8): (9970.0 + 2048.0) * 1024.0}
 # the same at N=1,048,576 (workload c5), skin 0.5, 8 lanes (profiles/lj_list_sweep_N1048576.md)
 SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8): (322982.0 + 65536.0) * 1024.0}
 
