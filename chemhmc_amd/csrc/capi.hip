@@ -55,7 +55,7 @@ struct chmc_ctx {
     int adam_step = 0;
     bool md_surr = false;        // MD kicks use the surrogate's forces (set for the trajectories of an HMC run)
     // neighbour-list MD (allocated on first use)
-    double skin = 0.3;           // the skin set for every chain, or the largest per-chain skin
+    double skin = 0.6;           // the skin set for every chain, or the largest per-chain skin
     std::vector<double> chain_skin;  // per-chain skins when set individually (empty: skin for all)
     bool list_ready = false;
     int lpar = 0;                // list parity: which FP32 coordinate buffer / rebuild flag is current
