@@ -260,8 +260,9 @@ __device__ __forceinline__ void build_step(const float4* sj, int j0, int lane, c
 // staged members sj[0, jn), padded with kJPad far entries.  Each wave takes one own atom at a time and
 // its 64 lanes test 64 members each step, branch-free (the far padding fails every test); the hits are
 // appended in staging order.  With one own atom per wave the hit mask is wave-uniform (~12 VALU per
-// tested pair; two own atoms x 32 members took ~18 and 6-9 % more build time).  The loop reads two
-// groups of 64 while both end within the padding and one for the rest, so the padding stays one group.
+// group of 64 tested pairs; two own atoms x 32 members took ~18 and 6-9 % more build time).  The loop
+// reads two groups of 64 while the second starts inside the members, then at most one more, so no read
+// passes jn + 63 and the padding stays one group.
 __device__ __forceinline__ void build_pass(const float4* sj, const float4* si, int* icount, int ni, int jn,
                                            float rl2, int nbcap, int* __restrict__ nb)
 {
