@@ -42,6 +42,9 @@ SWEEP_BYTES_PER_ATOM = 148.0
 SWEEP_TRAFFIC_PROFILED = {(0.6, 8): (9976.0 + 2048.0) * 1024.0, (0.7, 8): (9970.0 + 2048.0) * 1024.0}
 # the same at N=1,048,576 (workload c5), skin 0.5, 8 lanes (profiles/lj_list_sweep_N1048576.md)
 SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8): (322982.0 + 65536.0) * 1024.0}
+# the same for the 256-chain batch (workload c3) at the per-chain skins base 0.4, exponent 1.5 and 4
+# lanes per atom (profiles/lj_list_sweep_c3_batch.md), keyed by the skin family the run uses
+SWEEP_TRAFFIC_PROFILED_C3 = {("0.4/1.5", 4): (115006.0 + 38233.0) * 1024.0}
 
 
 def _omp_threads(n):
@@ -108,11 +111,17 @@ def other_workload(args):
     e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
     t_warm = time.perf_counter() - t0
     # skin autotuner (wall time of a few trajectories per candidate, paired from the warmed-up state,
-    # which it restores), unless --skin fixes one skin for every chain (profiling runs): a batch of
-    # chains searches per-chain skins scaled with the chain's density, a single chain one skin; lanes
-    # per atom tuned unless --lanes fixes them
+    # which it restores), unless --skin fixes one skin for every chain or --skin-family BASE/EXP fixes
+    # the per-chain skins (profiling runs): a batch of chains searches per-chain skins scaled with the
+    # chain's density, a single chain one skin; lanes per atom tuned unless --lanes fixes them
     skins = {}
-    if args.skin is None:
+    family = None
+    if args.skin_family is not None:
+        family = args.skin_family
+        fb, fe = (float(t) for t in family.split("/"))
+        e.set_chain_skins(tune.density_scaled_skins(w.chains, fb, fe))
+        e.set_list_lanes(args.lanes)
+    elif args.skin is None:
         def run(n, sd):
             e.hmc(n, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=sd, use_graph=not args.no_graph)
         if len(w.chains) > 1:
@@ -120,6 +129,7 @@ def other_workload(args):
         else:
             _, skins = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,),
                                        lanes=args.lanes)
+        family = min(skins, key=skins.get)
     else:
         e.set_skin(args.skin)
         e.set_list_lanes(args.lanes)
@@ -147,13 +157,15 @@ def other_workload(args):
         "config": {"workload": f"{args.workload}: {len(w.chains)} chain(s), {nat} atoms, "
                                f"{'LEAP' if w.scheme == chmc.LEAPFROG else 'OMF2'} {w.steps}x{w.dt}",
                    "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "skin": e.skin(),
-                   "chain_skin_range": [float(chain_skins.min()), float(chain_skins.max())],
+                   "chain_skin_range": [float(chain_skins.min()), float(chain_skins.max())], "skin_family": family,
                    "ms_per_traj_by_skin": skins, "warmup_seconds": t_warm},
         "acceptance": float(nacc.sum()) / (steps * len(w.chains)), "ess_per_s_sum_over_chains": ess / el,
         "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": (SWEEP_TRAFFIC_PROFILED_C5.get((round(float(e.skin()), 2), int(e.list_lanes())))
-                                 if args.workload == "c5" else None),
+                                 if args.workload == "c5" else
+                                 SWEEP_TRAFFIC_PROFILED_C3.get((family, int(e.list_lanes())))
+                                 if args.workload == "c3" else None),
                      "bytes_per_launch": nbytes, "flops_per_launch": flops, "tflops": flops / (ms_sweep * 1e-3) / 1e12,
                      "ms_per_launch": ms_sweep, "ms_rebuild": ms_rebuild,
                      "list_entries_per_atom": ent / nat, "n_int_per_atom": nint / nat},
@@ -334,6 +346,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100, help="untimed trajectories (also thermalisation)")
     ap.add_argument("--tune", type=int, default=40, help="trajectories per skin in the skin autotuner")
     ap.add_argument("--skin", type=float, default=None, help="fixed list skin (no skin autotuning)")
+    ap.add_argument("--skin-family", default=None,
+                    help="fixed per-chain skins BASE/EXP of the density-scaled family (c3, c5; no skin autotuning)")
     ap.add_argument("--lanes", type=int, default=0, choices=[0, 4, 8, 16],
                     help="fixed list-sweep lanes per atom (0: autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
