@@ -497,6 +497,43 @@ def test_per_chain_skins(chmc):
     assert np.array_equal(run(chains[:1], [0.3])[:, 0], ra[:, 0])
 
 
+def test_skin_tuner_on_the_engine(chmc):
+    """The skin autotuner on a real batch: it leaves the positions and the trajectory counter exactly as
+    it found them, keeps the candidate it reports as fastest and the lanes it timed that candidate
+    with, and a run after it equals a run from the same state with those skins set directly."""
+    from chemhmc_amd import tune
+    chains = [chmc.Chain(N=512, L=box(512, 0.8), cutoff=RC, beta=1 / 1.5),
+              chmc.Chain(N=1000, L=box(1000, 0.4), cutoff=RC, beta=1.0)]
+    e = chmc.Engine(chains)
+    e.init_lattice(7, 0.05)
+    e.hmc(4, 4, chmc.LEAPFROG, 0.005, 20, seed=3)
+    x0, t0 = [e.positions(k) for k in range(2)], e.trajectory()
+
+    def run(n, sd):
+        e.hmc(n, 0, chmc.LEAPFROG, 0.005, 20, seed=sd)
+
+    skins, table = tune.tune_skins(e, run, bases=(0.4, 0.6), exponents=(0.0, 1.0), reps=3, lanes=0)
+    assert len(table) == 4 and e.trajectory() == t0
+    assert all(np.array_equal(e.positions(k), x0[k]) for k in range(2))
+    best = min(table, key=table.get)
+    b, ex = (float(t) for t in best.split("/"))
+    assert np.allclose(skins, tune.density_scaled_skins(chains, b, ex))
+    assert np.allclose([e.chain_skin(k) for k in range(2)], skins)
+    lanes = e.list_lanes()
+    assert lanes in (4, 8, 16)
+    r1, _ = e.hmc(3, 0, chmc.LEAPFROG, 0.005, 20, seed=9)
+    e2 = chmc.Engine(chains)
+    for k in range(2):
+        e2.set_positions(x0[k], k)
+    e2.set_trajectory(t0)
+    e2.set_chain_skins(skins)
+    e2.set_list_lanes(lanes)
+    r2, _ = e2.hmc(3, 0, chmc.LEAPFROG, 0.005, 20, seed=9)
+    assert np.array_equal(r1, r2)
+    e.close()
+    e2.close()
+
+
 @pytest.mark.parametrize("fp64", [True, False])
 def test_hmc_records_vs_oracle(chmc, orc, fp64):
     """Whole trajectories (heat bath, LEAP MD, FP64 V and K, Metropolis, restore on reject) against the
