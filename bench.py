@@ -7,11 +7,17 @@ Metropolis, restore-on-reject), replayed from one captured hipGraph.  The timed 
 the state resident in HBM and ends after the device has finished the K-th trajectory.
 
   value      atom-steps/s = N * 10 MD steps * K trajectories / seconds
-  ess_per_s  ESS/s of U/N over the timed trajectories (Sokal-windowed tau_int)
-  roofline   the dominant kernel, the FP32 neighbour-list sweep (force + kick + drift), HBM-bound by its
-             arithmetic intensity: algorithmic bytes per launch (4 per list entry + 148 per atom, list
-             entries counted on the device for the current configuration) / its average duration
-             measured with HIP events on the context's stream (50 launches back to back)
+  thermalisation  a fixed burn-in of accept-all trajectories (the reference's thermalization_steps)
+             takes the chain off its lattice start before anything is tuned or timed, whatever --warmup
+             says; --warmup adds ordinary (Metropolis) untimed trajectories after it
+  ess        ESS/s of U/N (Sokal-windowed tau_int) from a run of at least ESS_TRAJ trajectories (the timed
+             run itself when it is that long, else a separate run after it, timed on its own); null when
+             U/N never changed in that run (no accepted trajectory: no information about tau_int)
+  roofline   the dominant kernel, the FP32 neighbour-list sweep (force + kick + drift), against the FP32
+             VALU peak with the algorithmic work of the reference's 27-cell sweep it replaces:
+             8 * N_cand + 13 * N_int flops per atom-evaluation (N_cand, N_int counted on the device for the
+             current configuration) / its average duration measured with HIP events on the context's
+             stream (50 launches back to back); its HBM bytes (algorithmic and rocprof-measured) beside it
   cpu_baseline  the FP64 C oracle (a port of the reference algorithm, cpu-opt binning) timed on this
              host's cores for a bounded sample of the same workload
 """
@@ -30,6 +36,8 @@ sys.path.insert(0, ROOT)
 
 N, RHO, T, RC = 32768, 0.8442, 0.722, 2.5
 DT, NSTEPS, LAM = 0.01, 10, 0.1932
+THERM = 200                     # accept-all burn-in trajectories off the lattice start
+ESS_TRAJ = 4000                 # trajectories of the run the ESS/s figure is measured on
 PEAK_FP32_TFLOPS = 157.3        # MI355X vector FP32 (vendor figure; packed FP32)
 PEAK_HBM_GBS = 8000.0
 # list sweep bytes per atom besides its list row: own FP32 coordinates 16, entry count 4, p 24 + 24 and
@@ -57,6 +65,18 @@ def _omp_threads(n):
         return False
 
 
+def _cpu_model():
+    """The host CPU's model name (lscpu's 'Model name', read from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(seconds_budget=20.0):
     """FP64 oracle (OpenMP) on the host, on the N=32768 workload: OMF2 trajectories with the O(N)
     counting-sort binning on all cores until ~budget (the reported value), one trajectory on one core,
@@ -76,6 +96,7 @@ def cpu_baseline(seconds_budget=20.0):
         if el > seconds_budget or ntraj >= 2000:
             break
     out = {"value": N * NSTEPS * ntraj / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+           "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
            "sample": f"{ntraj} OMF2 trajectories of the N=32768 workload (FP64 oracle, O(N) binning), {el:.1f} s"}
     if _omp_threads(1):
         t0 = time.perf_counter()
@@ -105,10 +126,14 @@ def other_workload(args):
     nat = w.natoms
     steps = args.steps if args.steps_given else (200 if args.workload == "c3" else 50)
     # both workloads relax from their lattice starts over the first ~100 trajectories (more rebuilds
-    # early); the tuners and the timed run start after that
-    warm = args.warmup if args.warmup_given else 150
+    # early): 150 accept-all trajectories whatever --warmup says, then --warmup ordinary ones; the tuners
+    # and the timed run start after that
+    therm = 150
+    warm = args.warmup if args.warmup_given else 0
     t0 = time.perf_counter()
-    e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
+    e.hmc(therm, therm, w.scheme, w.dt, w.steps, lam=w.lam, seed=10, use_graph=not args.no_graph)
+    if warm > 0:
+        e.hmc(warm, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
     t_warm = time.perf_counter() - t0
     # skin autotuner (wall time of a few trajectories per candidate, paired from the warmed-up state,
     # which it restores), unless --skin fixes one skin for every chain or --skin-family BASE/EXP fixes
@@ -148,7 +173,10 @@ def other_workload(args):
     nbytes = 4.0 * ent + SWEEP_BYTES_PER_ATOM * nat
     achieved = nbytes / (ms_sweep * 1e-3) / 1e9
     u = rec["u"] / np.array([c.N for c in w.chains])[None, :]
-    ess = sum(steps / (2 * stats.tau_int(u[:, k])[0]) for k in range(len(w.chains)))
+    # chains whose U/N never changed in the timed run carry no tau_int and contribute no ESS
+    taus = [stats.tau_int(u[:, k])[0] for k in range(len(w.chains))]
+    moving = [t for t in taus if np.isfinite(t)]
+    ess = sum(steps / (2 * t) for t in moving)
     out = {
         "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N", "value": value, "unit": "atom-steps/s",
         "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": el / steps * 1e3, "higher_is_better": True,
@@ -159,7 +187,8 @@ def other_workload(args):
                    "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "skin": e.skin(),
                    "chain_skin_range": [float(chain_skins.min()), float(chain_skins.max())], "skin_family": family,
                    "ms_per_traj_by_skin": skins, "warmup_seconds": t_warm},
-        "acceptance": float(nacc.sum()) / (steps * len(w.chains)), "ess_per_s_sum_over_chains": ess / el,
+        "acceptance": float(nacc.sum()) / (steps * len(w.chains)),
+        "ess_per_s_sum_over_chains": ess / el if moving else None, "chains_with_tau_int": len(moving),
         "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": (SWEEP_TRAFFIC_PROFILED_C5.get((round(float(e.skin()), 2), int(e.list_lanes())))
@@ -220,8 +249,11 @@ def surrogate_c5_workload(args):
     nat = w.natoms
     e = chmc.Engine(w.chains)
     e.init_lattice(1, 0.05)
-    warm = args.warmup if args.warmup_given else 10
+    warm = 10                   # accept-all trajectories off the lattice start
     e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
+    if args.warmup_given and args.warmup > 0:
+        e.hmc(args.warmup, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=9, use_graph=not args.no_graph)
+        warm += args.warmup
     e.set_skin(0.5)
     e.surrogate_set_params(params)
     nconf = 200
@@ -276,14 +308,16 @@ def surrogate_workload(args):
     L = (N / RHO) ** (1 / 3)
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
     e.init_lattice(1, 0.05)
-    e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11)
+    e.hmc(THERM, THERM, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10)
+    if args.warmup > 0:
+        e.hmc(args.warmup, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11)
     e.surrogate_set_params(params)
     x0 = e.positions()
     steps = args.steps if args.steps_given else 500
     out = {}
     for name, surr in (("plain", False), ("surrogate", True)):
         e.set_positions(x0)
-        e.set_trajectory(args.warmup)
+        e.set_trajectory(THERM + args.warmup)
         e.kinetic()
         t0 = time.perf_counter()
         rec, nacc = e.hmc(steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph,
@@ -291,8 +325,10 @@ def surrogate_workload(args):
         el = time.perf_counter() - t0
         uu = rec["u"][:, 0] / N
         tau, _ = stats.tau_int(uu)
+        ok = bool(np.isfinite(tau))
         out[name] = {"acceptance": float(nacc[0]) / steps, "ms_per_traj": el / steps * 1e3,
-                     "u_per_n": float(uu.mean()), "tau_int_u": tau, "ess_per_s": steps / (2 * tau) / el,
+                     "u_per_n": float(uu.mean()), "tau_int_u": tau if ok else None,
+                     "ess_per_s": steps / (2 * tau) / el if ok else None, "reliable": stats.reliable(uu),
                      "value": N * NSTEPS * steps / el}
     res = {
         "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N", "value": out["surrogate"]["value"],
@@ -305,7 +341,9 @@ def surrogate_workload(args):
                    "collect_seconds": t_collect, "train_seconds": t_train,
                    "train_mse_first": float(hist[:20].mean()), "train_mse_last": float(hist[-100:].mean())},
         "plain": out["plain"], "surrogate": out["surrogate"],
-        "ess_per_s_ratio_surrogate_over_plain": out["surrogate"]["ess_per_s"] / out["plain"]["ess_per_s"],
+        "ess_per_s_ratio_surrogate_over_plain": (out["surrogate"]["ess_per_s"] / out["plain"]["ess_per_s"]
+                                                  if out["surrogate"]["ess_per_s"] and out["plain"]["ess_per_s"]
+                                                  else None),
     }
     print(json.dumps(res))
 
@@ -313,14 +351,16 @@ def surrogate_workload(args):
 def cell_md_run(args, e, chmc, stats):
     """The same c2 measurement through the 27-cell sweep with a re-binning per force evaluation (the
     reference's cost model, CHMC_CELL_MD) instead of the neighbour lists."""
+    e.hmc(THERM, THERM, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10, use_graph=not args.no_graph)
     if args.warmup > 0:
-        e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
+        e.hmc(args.warmup, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
     e.set_force_tiling(0)
     t0 = time.perf_counter()
     rec, nacc = e.hmc(args.steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph)
     el = time.perf_counter() - t0
     u = rec["u"][:, 0] / N
     tau, _ = stats.tau_int(u)
+    tau = tau if np.isfinite(tau) else None
     ncand, nint = e.pair_counts(0)
     flops = 8.0 * ncand + 13.0 * nint
     ms_bin, ms_force = e.time_force(reps=50)
@@ -331,7 +371,8 @@ def cell_md_run(args, e, chmc, stats):
            "dtype": "f32 pair forces, f64 x/p/dH", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
            "config": {"workload": "plain HMC N=32768 rho=0.8442 T=0.722 rc=2.5 OMF2(lambda=0.1932) 10x0.01",
                       "md_path": "27-cell sweep, re-binning per evaluation", "force_tiling": e.force_tiling()},
-           "acceptance": float(nacc[0]) / args.steps, "tau_int_u": tau, "ess_per_s": args.steps / (2 * tau) / el,
+           "acceptance": float(nacc[0]) / args.steps, "tau_int_u": tau,
+           "ess_per_s": args.steps / (2 * tau) / el if tau else None,
            "roofline": {"bound": "valu", "kernel": "k_lj<float> force+kick sweep", "achieved": achieved,
                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                         "traffic": None, "flops_per_launch": flops, "ms_per_launch": ms_force, "ms_binning": ms_bin,
@@ -388,8 +429,10 @@ def main():
     if args.cell_md:
         cell_md_run(args, e, chmc, stats)
         return
-    if args.warmup > 0:     # accept-all thermalisation away from the lattice (thermalization_steps)
-        e.hmc(args.warmup, args.warmup, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
+    # accept-all thermalisation away from the lattice (thermalization_steps), then the untimed warm-up
+    e.hmc(THERM, THERM, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10, use_graph=not args.no_graph)
+    if args.warmup > 0:
+        e.hmc(args.warmup, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
     # skin autotuner: wall time of a short run per skin from the equilibrated state (paired: every skin
     # starts from that state with the same random streams, and the state is restored afterwards, so the
     # timed run continues from it whatever the tuner ran), with the lanes per atom of the list sweep
@@ -413,24 +456,40 @@ def main():
     el = time.perf_counter() - t0
     ms = el / args.steps * 1e3
     value = N * NSTEPS * args.steps / el
-
-    u = rec["u"][:, 0] / N
-    tau, _ = stats.tau_int(u)
-    ess = args.steps / (2 * tau)
-
-    # dominant kernel: the neighbour-list FP32 sweep (force + kick + drift).  Its arithmetic intensity
-    # (~2.6 flop/B) is below the ridge (157.3 TF/s / 8 TB/s = 19.7 flop/B), so the roofline that bounds
-    # it is HBM: algorithmic bytes per launch = 4 B per list entry + SWEEP_BYTES_PER_ATOM per atom
-    # (own coordinates, entry count, rebuild reference, p and x read and written, next coordinates);
-    # the neighbour coordinates it gathers are cache hits and not counted.  Flops: 8 per list entry
-    # (candidate) + 13 per interacting pair.
     entries, nint, rebuilds1 = e.list_stats()
     rebuilds_per_traj = (rebuilds1 - rebuilds0 - 1) / args.steps       # list_stats itself rebuilds once
-    flops = 8.0 * entries + 13.0 * nint
-    nbytes = 4.0 * entries + SWEEP_BYTES_PER_ATOM * N
+
+    # ESS/s of U/N on a run long enough for tau_int (the timed run itself when it is)
+    if args.steps >= ESS_TRAJ:
+        ess_rec, ess_acc, ess_el = rec, nacc, el
+    else:
+        t0 = time.perf_counter()
+        ess_rec, ess_acc = e.hmc(ESS_TRAJ, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=13, use_graph=not args.no_graph)
+        ess_el = time.perf_counter() - t0
+    u = ess_rec["u"][:, 0] / N
+    nt = u.size
+    tau, window, closed = stats.tau_window(u)
+    informative = bool(np.isfinite(tau))
+    ess_info = {"trajectories": nt, "seconds": ess_el, "acceptance": float(ess_acc[0]) / nt,
+                "u_per_n": float(u.mean()), "tau_int_u": tau if informative else None,
+                "sokal_window": window, "window_closed": closed, "reliable": stats.reliable(u),
+                "ess_per_s": nt / (2 * tau) / ess_el if informative else None,
+                "run": "the timed run" if ess_rec is rec else "a separate run after the timed one"}
+    if not informative:
+        ess_info["note"] = "U/N never changed (no accepted trajectory): tau_int and ESS undefined"
+
+    # dominant kernel: the neighbour-list FP32 sweep (force + kick + drift), against the FP32 VALU peak with
+    # the algorithmic work of the 27-cell sweep it replaces (8 flops per candidate pair of the 27-cell
+    # stencil + 13 per interacting pair, per atom-evaluation; candidates and interacting pairs counted on
+    # the device).  Its HBM side: 4 B per list entry + SWEEP_BYTES_PER_ATOM per atom algorithmic, and the
+    # rocprofv3-measured traffic when the tuned skin and lanes are the profiled ones.
     ms_sweep, ms_rebuild = e.time_list(reps=50)
-    achieved = nbytes / (ms_sweep * 1e-3) / 1e9
-    ncand_cell, _ = e.pair_counts(0)
+    ncand_cell, nint_cell = e.pair_counts(0)
+    flops_alg = 8.0 * ncand_cell + 13.0 * nint_cell
+    flops_exec = 8.0 * entries + 13.0 * nint
+    nbytes = 4.0 * entries + SWEEP_BYTES_PER_ATOM * N
+    tflops = flops_alg / (ms_sweep * 1e-3) / 1e12
+    traffic = SWEEP_TRAFFIC_PROFILED.get((round(float(skin), 2), int(lanes)))
 
     out = {
         "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N",
@@ -439,17 +498,21 @@ def main():
         "dtype": "f32 pair forces, f64 x/p/dH", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": "plain HMC N=32768 rho=0.8442 T=0.722 rc=2.5 OMF2(lambda=0.1932) 10x0.01",
                    "N": N, "trajectories": args.steps, "md_steps_per_traj": NSTEPS, "force_evals_per_traj": 2 * NSTEPS + 1,
-                   "md_path": "neighbour lists", "skin": skin, "list_lanes": lanes, "ms_per_traj_by_skin": skins},
+                   "thermalisation": THERM, "md_path": "neighbour lists", "skin": skin, "list_lanes": lanes,
+                   "ms_per_traj_by_skin": skins},
         "acceptance": float(nacc[0]) / args.steps,
-        "u_per_n": float(u.mean()), "tau_int_u": tau, "ess_per_s": ess / el,
+        "ess_per_s": ess_info["ess_per_s"], "ess": ess_info,
         "list_rebuilds_per_traj": rebuilds_per_traj,
-        "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": SWEEP_TRAFFIC_PROFILED.get((round(float(skin), 2), int(lanes))),
-                     "bytes_per_launch": nbytes, "flops_per_launch": flops,
-                     "tflops": flops / (ms_sweep * 1e-3) / 1e12, "ms_per_launch": ms_sweep,
-                     "list_entries_per_atom": entries / N, "n_int_per_atom": nint / N,
-                     "cell_candidates_per_atom": ncand_cell / N, "ms_rebuild": ms_rebuild},
+        "roofline": {"bound": "valu", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": tflops,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFLOPS,
+                     "traffic": traffic, "flops_per_launch": flops_alg, "ms_per_launch": ms_sweep,
+                     "work": "8 * N_cand + 13 * N_int per atom-evaluation, N_cand of the 27-cell stencil",
+                     "cell_candidates_per_atom": ncand_cell / N, "n_int_per_atom": nint_cell / N,
+                     "executed_flops_per_launch": flops_exec, "list_entries_per_atom": entries / N,
+                     "hbm": {"bytes_per_launch": nbytes, "achieved_gbs": nbytes / (ms_sweep * 1e-3) / 1e9,
+                             "frac": nbytes / (ms_sweep * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                             "measured_traffic_gbs": traffic / (ms_sweep * 1e-3) / 1e9 if traffic else None},
+                     "ms_rebuild": ms_rebuild},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -18,7 +18,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libchmc.so")
 
-LEAPFROG, OMF4, EULER, RUTH, OMF2 = 0, 2, 4, 5, 6
+LEAPFROG, LP_LEAPFROG, OMF4, LP_OMF4, EULER, RUTH, OMF2 = 0, 1, 2, 3, 4, 5, 6
 FORCE_FP32, FORCE_FP64, ORDER_ATOM_ID, CELL_MD = 0, 1, 2, 4
 OMF2_LAMBDA_REFERENCE = 0.1938       # code/modules/integrator.cpp:72
 
@@ -93,6 +93,8 @@ def _declare(L):
         "chmc_update_positions": ([P, _D], _I),
         "chmc_kick": ([P, _D], _I),
         "chmc_integrate": ([P, _I, _D, _I, _D], _I),
+        "chmc_set_lp_exponent": ([P, _I], _I),
+        "chmc_get_lp_exponent": ([P, C.POINTER(_I)], _I),
         "chmc_heatbath": ([P, _U64, _U64], _I),
         "chmc_save_positions": ([P], _I),
         "chmc_restore_positions": ([P], _I),
@@ -133,7 +135,7 @@ EXPORTED = (
     "chmc_set_positions", "chmc_get_positions", "chmc_set_momenta", "chmc_get_momenta", "chmc_init_lattice",
     "chmc_bin", "chmc_get_bins", "chmc_compute_force", "chmc_get_force", "chmc_compute_potential",
     "chmc_evaluate_potential", "chmc_virial", "chmc_pair_counts", "chmc_kinetic", "chmc_update_momenta", "chmc_update_positions",
-    "chmc_kick", "chmc_integrate", "chmc_heatbath", "chmc_save_positions", "chmc_restore_positions",
+    "chmc_kick", "chmc_integrate", "chmc_set_lp_exponent", "chmc_get_lp_exponent", "chmc_heatbath", "chmc_save_positions", "chmc_restore_positions",
     "chmc_hmc_run", "chmc_rdf", "chmc_potential_energy_open", "chmc_set_force_tiling",
     "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_set_chain_skins",
     "chmc_get_chain_skin", "chmc_list_stats",
@@ -311,6 +313,15 @@ class Engine:
 
     def integrate(self, scheme, dt, steps, lam=OMF2_LAMBDA_REFERENCE):
         self._ck(lib().chmc_integrate(self._h, int(scheme), float(dt), int(steps), float(lam)), "integrate")
+
+    def set_lp_exponent(self, n_prec):
+        """Rounding grid 10^-n_prec of the LP_LEAPFROG / LP_OMF4 schemes (set_integrator's exponent)."""
+        self._ck(lib().chmc_set_lp_exponent(self._h, int(n_prec)), "set_lp_exponent")
+
+    def lp_exponent(self):
+        n = _I()
+        self._ck(lib().chmc_get_lp_exponent(self._h, C.byref(n)), "get_lp_exponent")
+        return n.value
 
     def heatbath(self, seed, traj):
         self._ck(lib().chmc_heatbath(self._h, int(seed), int(traj)), "heatbath")

@@ -85,6 +85,20 @@ int main()
         }
     report("omf2 reversibility (fp32)", dmax < 1e-4, dmax);
 
+    // set_integrator(LP_LEAPFROG / LP_OMF4, exponent): every update rounded to the grid 10^-exponent
+    for (size_t integs : { (size_t)LP_LEAPFROG, (size_t)LP_OMF4 }) {
+        std::unique_ptr<integrator> lp(set_integrator(integs, 6));
+        lp->integrate(P, h, md);
+        double off = 0;
+        for (int i = 0; i < N; i++)
+            for (int d = 0; d < 3; d++) {
+                off = std::fmax(off, std::fabs(std::round(x[i][d] * 1e6) / 1e6 - x[i][d]));
+                off = std::fmax(off, std::fabs(std::round(p[i][d] * 1e6) / 1e6 - p[i][d]));
+            }
+        report(integs == LP_LEAPFROG ? "lp_leapfrog(6) states on the 1e-6 grid" : "lp_omf4(6) states on the 1e-6 grid",
+               off == 0.0 && lp->exponent() == 6 && lp->scheme() == (int)integs, off);
+    }
+
     // ---- HMC_class: host-stepped loop vs the device-resident loop, same seeds ----
     // (in-cell order by atom id, so that both runs sum the FP32 forces in the same order although the
     // first one leaves a different binning history behind)

@@ -34,6 +34,7 @@ struct chmc_ctx {
     int force_kind = 0;          // 0 none, 1 fp32 stored, 2 fp64 stored
     int islots = 82;            // LJ sweep tiling: i-slots * 10 + waves per workgroup
     double pending_drift = 0.0;  // drift deferred into the next binning (integrator sequencing)
+    int lp_exponent = 16;        // n_prec of the lp_* integrators (lp_leapfrog() : n_prec(16))
     int traj_done = 0;           // trajectories completed on the device counter
     std::string err;
     double* d_uniforms = nullptr;
@@ -757,7 +758,7 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
     HIPCHK(c, lalloc(c, &v.linv, 2 * N));
     HIPCHK(c, lalloc(c, &v.nb, 2 * N * nbcap));
     HIPCHK(c, lalloc(c, &v.nbn, 2 * N));
-    HIPCHK(c, lalloc(c, &v.lbuf, 2 * (size_t)c->nchains));
+    HIPCHK(c, lalloc(c, &v.lbuf, 3 * (size_t)c->nchains));
     HIPCHK(c, lalloc(c, &v.tcur, (size_t)v.ntiles));
     HIPCHK(c, lalloc(c, &v.lflag, 4));
     HIPCHK(c, lalloc(c, &v.lreq, 2 * ((size_t)c->nchains + 1)));
@@ -775,6 +776,7 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
     HIPCHK(c, hipMemset(v.lflag, 0, sizeof(int) * 4));
     HIPCHK(c, hipMemset(v.lreq, 0, sizeof(int) * 2 * (c->nchains + 1)));
     HIPCHK(c, hipMemset(v.lbuf, 0, sizeof(int) * 2 * c->nchains));
+    HIPCHK(c, launch_fill_int(v.lbuf + 2 * c->nchains, -1, c->nchains, c->stream));
     HIPCHK(c, hipMemset(v.tcur, 0, sizeof(int) * v.ntiles));
     // loffs of a chain's first cell is its atom offset; the end entry is Ntot
     std::vector<int> lo0(lcells + 1, 0);
@@ -951,9 +953,34 @@ static int do_surr_force_kick(chmc_ctx* c, double dt, bool storef);
 static int surr_geometry(chmc_ctx* c);
 static int surr_check_flag(chmc_ctx* c);
 
-// rel64_last: a closing kick also writes the FP64 positions of the energy that follows
-static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double lambda, bool storef, bool rel64_last = false)
+static bool lp_scheme(int s) { return s == CHMC_LP_LEAPFROG || s == CHMC_LP_OMF4; }
+
+static double lp_scale_of(int n_prec)
 {
+    double s = 1.0;
+    for (int i = 0; i < n_prec; i++) s *= 10.0;      // exact for n_prec <= 22
+    return s;
+}
+
+static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double lambda, bool storef, bool rel64_last = false);
+
+// lp_leapfrog / lp_omf4 (code-carsten/integrator.hh:51-79, 156-197): the leapfrog / OMF4 sequence with
+// every kernel that kicks or drifts rounding the updated variable (DevView::lp_scale); a drift left
+// pending for the next binning is applied before the rounding is switched off again
+static int integrate_lp(chmc_ctx* c, int scheme, double dt, int steps, double lambda, bool storef, bool rel64_last)
+{
+    c->v.lp_scale = lp_scale_of(c->lp_exponent);
+    int rc = integrate_seq(c, scheme == CHMC_LP_LEAPFROG ? CHMC_LEAPFROG : CHMC_OMF4, dt, steps, lambda, storef,
+                           rel64_last);
+    if (!rc) rc = flush_drift(c);
+    c->v.lp_scale = 0.0;
+    return rc;
+}
+
+// rel64_last: a closing kick also writes the FP64 positions of the energy that follows
+static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double lambda, bool storef, bool rel64_last)
+{
+    if (lp_scheme(scheme)) return integrate_lp(c, scheme, dt, steps, lambda, storef, rel64_last);
     std::vector<MdOp> ops;
     if (build_ops(scheme, dt, steps, lambda, ops) != 0) return fail(c, CHMC_E_ARG, "unknown integrator scheme");
     if (list_md(c) && !c->md_surr) {
@@ -995,7 +1022,22 @@ static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double l
 
 static bool scheme_ok(int s)
 {
-    return s == CHMC_LEAPFROG || s == CHMC_OMF2 || s == CHMC_OMF4 || s == CHMC_EULER || s == CHMC_RUTH;
+    return s == CHMC_LEAPFROG || s == CHMC_OMF2 || s == CHMC_OMF4 || s == CHMC_EULER || s == CHMC_RUTH || lp_scheme(s);
+}
+
+extern "C" int chmc_set_lp_exponent(chmc_ctx* c, int n_prec)
+{
+    CHECK_CTX(c);
+    if (n_prec < 0 || n_prec > 18) return fail(c, CHMC_E_ARG, "set_lp_exponent: n_prec must be in [0, 18]");
+    c->lp_exponent = n_prec;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_get_lp_exponent(const chmc_ctx* c, int* n_prec)
+{
+    if (!c || !n_prec) return CHMC_E_ARG;
+    *n_prec = c->lp_exponent;
+    return CHMC_OK;
 }
 
 extern "C" int chmc_integrate(chmc_ctx* c, int scheme, double dt, int steps, double lambda)

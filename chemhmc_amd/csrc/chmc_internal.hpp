@@ -48,7 +48,9 @@
 //   lflag         : int [4]                   ([2]: capacity overflow, [3]: chain rebuilds run)
 //   The per-slot list state (loffs, lperm, linv, lref, xsl, nb, nbn) exists twice: a rebuild writes
 //   the buffer that is not the trajectory's start buffer, so a rejected trajectory goes back to the
-//   start buffer (valid for the restored positions) instead of rebuilding.  lbuf[k] / lbuf[nchains + k]
+//   start buffer (valid for the restored positions) instead of rebuilding; the momenta pl are left in the
+//   slot order of the buffer the trajectory ended in (lbuf[2 nchains + k], -1 when that is the current
+//   one), which the export follows.  lbuf[k] / lbuf[nchains + k]
 //   are chain k's current and start buffers; tcur[t] repeats the current buffer of tile t's chain, so
 //   that a sweep workgroup finds its buffer with one load that depends on nothing but its index.
 //   lscan_tmp     : int [Cl]                  (global exclusive scan of the counts, rebuild scratch)
@@ -129,13 +131,21 @@ struct DevView {
     int ldense;           // member slots of all chains (sum of lncells * lcap)
     int lscan_in_build;   // every chain has at most kScanCells list cells: the build scans the counts
     int lcap_max;         // largest list-cell member capacity over the chains (at most 256)
-    int* lbuf;            // [2][nchains]: current list buffer, buffer of the trajectory's start
+    int* lbuf;            // [3][nchains]: current list buffer, buffer of the trajectory's start, buffer whose
+                          // slot order pl is in when that is not the current one (-1 otherwise)
     int* tcur;            // [ntiles]: current list buffer of each tile's chain (kept equal to lbuf)
     float4* lref;
     double* rel64;        // [Ntot][4] by slot: FP64 position relative to the rebuild cell origin (x, y, z, -)
     float4* xsl;
     double* tile_part;    // [ntiles * 4]: energy, virial, interacting pairs, kinetic energy
+    double lp_scale;      // 10^n_prec inside an lp_leapfrog / lp_omf4 integration, 0 otherwise
 };
+
+// The updates of lp_leapfrog / lp_omf4 (code-carsten/integrator.hh:51-79, 156-197) round the updated
+// momentum or position to the grid 1/s, s = 10^n_prec: v -> round(v s) / s, half away from zero (the
+// bit-reversible scheme of Levesque & Verlet 1993); positions are rounded before the wrap.  s == 0:
+// the plain update.
+__device__ __forceinline__ double lp_round(double val, double s) { return s != 0.0 ? round(val * s) / s : val; }
 
 // LJ_REL64 (list sweep without drift): also the FP64 cell-relative positions of the energy (rel64)
 enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8, LJ_DRIFT = 16, LJ_REL64 = 32 };
@@ -161,6 +171,7 @@ __device__ __forceinline__ ListBuf list_buf(const DevView& v, int b)
 }
 __device__ __forceinline__ int* list_cur(const DevView& v, int chain) { return v.lbuf + chain; }
 __device__ __forceinline__ int* list_start(const DevView& v, int chain) { return v.lbuf + v.nchains + chain; }
+__device__ __forceinline__ int* list_pord(const DevView& v, int chain) { return v.lbuf + 2 * v.nchains + chain; }
 // a thread of chain C that owns slot s marks its tile's current buffer when s begins a tile
 __device__ __forceinline__ void tile_cur_mark(const DevView& v, const ChainDev& C, int s, int b)
 {

@@ -41,6 +41,7 @@ __global__ __launch_bounds__(256) void k_count(DevView v, double drift_dt, int d
             double xv = v.x[d * N + a];
             if (do_drift) {
                 xv += drift_dt * coeff_x * v.p[d * N + a];
+                xv = lp_round(xv, v.lp_scale);
                 xv -= C.L[d] * floor(xv / C.L[d]);
                 v.x[d * N + a] = xv;
             }
@@ -66,6 +67,7 @@ __global__ __launch_bounds__(256) void k_drift(DevView v, double dt)
     for (int d = 0; d < 3; d++) {
         double xv = v.x[d * N + a];
         xv += dt * coeff_x * v.p[d * N + a];
+        xv = lp_round(xv, v.lp_scale);
         xv -= C.L[d] * floor(xv / C.L[d]);
         v.x[d * N + a] = xv;
     }

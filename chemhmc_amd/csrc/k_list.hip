@@ -527,6 +527,7 @@ __device__ __forceinline__ void list_drift_slot(const DevView& v, const ListBuf&
     for (int d = 0; d < 3; d++) {
         double xv = v.xl[d * N + slot];
         xv += dt * coeff_x * pn[d];
+        xv = lp_round(xv, v.lp_scale);
         xv -= C.L[d] * floor(xv / C.L[d]);
         v.xl[d * N + slot] = xv;
         const double r = list_rel(C, ref, d, xv);
@@ -638,13 +639,14 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         double pn = p_d;
         if constexpr ((MODE & LJ_KICK) != 0) {
             const double kdt = kick_dt * C.beta;       // dt * coeff_p, coeff_p = beta (:642-644)
-            pn = pn - kdt * (double)fd;
+            pn = lp_round(pn - kdt * (double)fd, v.lp_scale);
             v.pl[d * N + s] = pn;
         }
         if constexpr ((MODE & LJ_DRIFT) != 0) {
             const double coeff_x = C.beta / C.mass;    // compute_coeff_position, identical_particles.cpp:646-648
             double xv = x_d;
             xv += drift_dt * coeff_x * pn;
+            xv = lp_round(xv, v.lp_scale);
             xv -= C.L[d] * floor(xv / C.L[d]);
             v.xl[d * N + s] = xv;
             const double r = list_rel(C, ref, d, xv);
@@ -802,8 +804,10 @@ __global__ __launch_bounds__(256) void k_limport(DevView v)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= v.Ntot) return;
-    const int a = list_buf(v, *list_cur(v, v.atom_chain[s])).lperm[s];
+    const int ch = v.atom_chain[s];
+    const int a = list_buf(v, *list_cur(v, ch)).lperm[s];
     const int N = v.Ntot;
+    if (s == v.chains[ch].atom_off) *list_pord(v, ch) = -1;       // pl in the current buffer's order
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         v.xl[d * N + s] = v.x[d * N + a];
@@ -817,12 +821,14 @@ __global__ __launch_bounds__(256) void k_lexport(DevView v, int saved)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= v.Ntot) return;
     const int ch = v.atom_chain[s];
-    const int a = list_buf(v, *list_cur(v, ch)).lperm[s];
+    const int cur = *list_cur(v, ch), pord = *list_pord(v, ch);
+    const int a = list_buf(v, cur).lperm[s];
+    const int ap = pord >= 0 ? list_buf(v, pord).lperm[s] : a;     // pl after a rejection: its own order
     const int N = v.Ntot;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         v.x[d * N + a] = v.xl[d * N + s];
-        v.p[d * N + a] = v.pl[d * N + s];
+        v.p[d * N + ap] = v.pl[d * N + s];
     }
     if (saved) {
         const int as = list_buf(v, *list_start(v, ch)).lperm[s];

@@ -239,7 +239,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_lj(DevView v, double kick_dt, d
                         }
                         if constexpr ((MODE & LJ_KICK) != 0) {
                             const double fd = (double)f3[d];
-                            pn[d] = v.p[d * N + atom] - kdt * fd;
+                            pn[d] = lp_round(v.p[d * N + atom] - kdt * fd, v.lp_scale);
                             v.p[d * N + atom] = pn[d];
                         }
                     }
@@ -252,6 +252,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_lj(DevView v, double kick_dt, d
                         for (int d = 0; d < 3; d++) {
                             double xv = v.x[d * N + atom];
                             xv += drift_dt * coeff_x * pn[d];
+                            xv = lp_round(xv, v.lp_scale);
                             xv -= C.L[d] * floor(xv / C.L[d]);
                             v.x[d * N + atom] = xv;
                             x3[d] = xv;

@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) void k_surr_force(DevView v, SurrView s, const
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             if (store) v.f32[d * N + atom] = f3[d];
-            if (kick_dt != 0.0) v.p[d * N + atom] = v.p[d * N + atom] - kdt * (double)f3[d];
+            if (kick_dt != 0.0) v.p[d * N + atom] = lp_round(v.p[d * N + atom] - kdt * (double)f3[d], v.lp_scale);
         }
     }
 }

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(256) void k_heatbath(DevView v, uint64_t seed, long
     double s = 0.0;
     const int* lperm = slots ? list_buf(v, *list_cur(v, ch)).lperm : nullptr;
     double* pout = slots ? v.pl : v.p;
+    if (slots && ck.begin == C.atom_off && threadIdx.x == 0) *list_pord(v, ch) = -1;   // pl in the current order
     for (int i = ck.begin + (int)threadIdx.x; i < ck.end; i += 256) {
         const int a = slots ? lperm[i] : i;       // i: the atom, or the slot of atom a
         const u64x4 r = philox4x64_10((uint64_t)(a - C.atom_off), traj, (uint64_t)ch, 0, seed, kKeyHeatbath);
@@ -227,7 +228,13 @@ __global__ __launch_bounds__(256) void k_save_restore_slots(DevView v, int mode,
         return;
     }
     const int b = *list_start(v, ch);
-    if (s == C.atom_off) *list_cur(v, ch) = b;
+    if (s == C.atom_off) {
+        // the momenta stay in the slot order of the buffer the trajectory ended in (the reference keeps
+        // the post-MD momenta of a rejected trajectory, HMC.cpp:111): recorded for the export
+        const int cur = *list_cur(v, ch);
+        *list_pord(v, ch) = cur != b ? cur : -1;
+        *list_cur(v, ch) = b;
+    }
     tile_cur_mark(v, C, s, b);
     const ListBuf L = list_buf(v, b);
     float f3[3];
@@ -252,7 +259,7 @@ __global__ __launch_bounds__(256) void k_kick(DevView v, double dt, int fp64)
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const double fd = fp64 ? v.f64[d * N + a] : (double)v.f32[d * N + a];
-        v.p[d * N + a] = v.p[d * N + a] - kdt * fd;
+        v.p[d * N + a] = lp_round(v.p[d * N + a] - kdt * fd, v.lp_scale);
     }
 }
 

@@ -57,8 +57,12 @@ enum {
                                   of the skin neighbour lists */
 };
 
-/* integrator schemes: the numbering of code-carsten/integrator.hh:14 (set_integrator) */
-enum { CHMC_LEAPFROG = 0, CHMC_OMF4 = 2, CHMC_EULER = 4, CHMC_RUTH = 5, CHMC_OMF2 = 6 };
+/* integrator schemes: the numbering of code-carsten/integrator.hh:14 (set_integrator).  The lp_*
+ * schemes (lp_leapfrog, integrator.hh:51-79; lp_omf4, :156-197) are the leapfrog and OMF4 sequences
+ * whose every momentum and position update is followed by rounding of the updated variable to the
+ * grid 10^-n_prec (n_prec from chmc_set_lp_exponent); positions are rounded before the wrap. */
+enum { CHMC_LEAPFROG = 0, CHMC_LP_LEAPFROG = 1, CHMC_OMF4 = 2, CHMC_LP_OMF4 = 3, CHMC_EULER = 4, CHMC_RUTH = 5,
+       CHMC_OMF2 = 6 };
 
 /* ---- lifetime ------------------------------------------------------------------------------ */
 /* Replaces the identical_particles constructor + cutoff_binning + parallel_binning_init
@@ -116,6 +120,11 @@ int chmc_kick(chmc_ctx* ctx, double dt);
  * ruth (integrator.hh:200-240); lambda is OMF2's (reference: 0.1938, integrator.cpp:72).  Each
  * drift is fused into the binning of the following force evaluation. */
 int chmc_integrate(chmc_ctx* ctx, int scheme, double dt, int steps, double lambda);
+/* the exponent set_integrator(integs, exponent) hands to lp_leapfrog(n) / lp_omf4(n)
+ * (code-carsten/integrator.hh:249, :257): rounding grid 10^-n_prec of the CHMC_LP_* schemes, used by
+ * chmc_integrate and chmc_hmc_run; 0 <= n_prec <= 18, default 16 (lp_leapfrog() : n_prec(16), :53) */
+int chmc_set_lp_exponent(chmc_ctx* ctx, int n_prec);
+int chmc_get_lp_exponent(const chmc_ctx* ctx, int* n_prec);
 
 /* ---- HMC pieces ------------------------------------------------------------------------------- */
 /* hb (identical_particles.cpp:283-296): p ~ N(0, m/beta), Philox4x64-10 key {seed, "HEAT"},

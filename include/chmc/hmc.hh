@@ -189,7 +189,8 @@ enum integrators { LEAPFROG = 0, LP_LEAPFROG = 1, OMF4_ = 2, LP_OMF4 = 3, EULER 
 
 class integrator {
 public:
-    explicit integrator(int scheme, double lam = 0.1938) : scheme_(scheme), lam_(lam) {}
+    // exponent: n_prec of the lp_* schemes (their grid 10^-n_prec), ignored by the others
+    explicit integrator(int scheme, double lam = 0.1938, int exponent = 16) : scheme_(scheme), lam_(lam), exponent_(exponent) {}
     // the system's force field takes the role of the monomial list; the field's configs are uploaded,
     // integrated on the GPU and downloaded again
     void integrate(identical_particles& sys, hamiltonian_field<coordinate>& h, const md_params& params, bool restore = true)
@@ -198,22 +199,28 @@ public:
         sys.set_x(*h.U);
         sys.set_p(*h.momenta);
         const double dtau = params.gettau() / double(params.getnsteps());
+        if (scheme_ == CHMC_LP_LEAPFROG || scheme_ == CHMC_LP_OMF4)
+            check(chmc_set_lp_exponent(sys.ctx(), exponent_), sys.ctx(), "set_lp_exponent");
         check(chmc_integrate(sys.ctx(), scheme_, dtau, (int)params.getnsteps(), lam_), sys.ctx(), "integrate");
         sys.get_x(*h.U);
         sys.get_p(*h.momenta);
     }
     int scheme() const { return scheme_; }
+    int exponent() const { return exponent_; }
 
 private:
     int scheme_;
     double lam_;
+    int exponent_;
 };
 
-// integrator.hh:243-278; the low-precision variants and unknown values fall back to leapfrog, as there
-inline integrator* set_integrator(size_t integs, size_t exponent = 0)
+// integrator.hh:243-278: lp_leapfrog(exponent) and lp_omf4(exponent) round every update to the grid
+// 10^-exponent; unknown values fall back to leapfrog, as there (:272-275)
+inline integrator* set_integrator(size_t integs, size_t exponent = 16)
 {
-    (void)exponent;
     switch (integs) {
+    case LP_LEAPFROG: return new integrator(CHMC_LP_LEAPFROG, 0.1938, (int)exponent);
+    case LP_OMF4: return new integrator(CHMC_LP_OMF4, 0.1938, (int)exponent);
     case OMF4_: return new integrator(CHMC_OMF4);
     case EULER: return new integrator(CHMC_EULER);
     case RUTH: return new integrator(CHMC_RUTH);

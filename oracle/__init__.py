@@ -26,7 +26,7 @@ _PU64 = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 _L3 = _D * 3
 _I3 = _I * 3
 
-LEAPFROG, OMF4, EULER, RUTH, OMF2 = 0, 2, 4, 5, 6
+LEAPFROG, LP_LEAPFROG, OMF4, LP_OMF4, EULER, RUTH, OMF2 = 0, 1, 2, 3, 4, 5, 6
 BIN_COUNTING, BIN_REFALGO = 0, 1
 
 
@@ -60,7 +60,7 @@ class System(C.Structure):
         ("nbin", _I3), ("bintot", _I), ("sizebin", _L3), ("bin_mode", _I),
         ("x", C.POINTER(_D)), ("p", C.POINTER(_D)), ("f", C.POINTER(_D)), ("x_saved", C.POINTER(_D)),
         ("perm", C.POINTER(_I)), ("perm_tmp", C.POINTER(_I)), ("bincount", C.POINTER(_I)),
-        ("binoffsets", C.POINTER(_I)), ("force_evals", C.c_longlong),
+        ("binoffsets", C.POINTER(_I)), ("force_evals", C.c_longlong), ("lp_scale", _D), ("lp_active", _I),
     ]
 
 
@@ -102,6 +102,10 @@ def _declare(L):
     L.orc_kinetic.restype = _D
     L.orc_kick.argtypes = [_I, _PD, _PD, _D, _D]
     L.orc_drift.argtypes = [_I, _PD, _PD, _D, _D, _L3]
+    L.orc_lp_scale.argtypes = [_I]
+    L.orc_lp_scale.restype = _D
+    L.orc_kick_lp.argtypes = [_I, _PD, _PD, _D, _D, _D]
+    L.orc_drift_lp.argtypes = [_I, _PD, _PD, _D, _D, _L3, _D]
     L.orc_rdf.argtypes = [_I, _PD, _L3, _I, _D, _PU64, _PD]
     L.orc_system_init.argtypes = [C.POINTER(System), _I, _L3, _D, _D, _D, _D, _D, _I]
     L.orc_system_init.restype = _I
@@ -329,6 +333,10 @@ class Sim:
 
     def update_positions(self, dt):
         lib().orc_sys_update_positions(C.byref(self._s), float(dt))
+
+    def set_lp_exponent(self, n_prec):
+        """Rounding grid 10^-n_prec of the lp_* integrators (set_integrator's exponent)."""
+        self._s.lp_scale = lib().orc_lp_scale(int(n_prec))
 
     def integrate(self, scheme, dt, steps, lam=0.1938):
         lib().orc_sys_integrate(C.byref(self._s), int(scheme), float(dt), int(steps), float(lam))

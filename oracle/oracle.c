@@ -596,6 +596,7 @@ int orc_system_init(orc_system* s, int N, const double L[3], double cutoff, doub
     s->bincount = (int*)calloc((size_t)s->bintot, sizeof(int));
     s->binoffsets = (int*)calloc((size_t)s->bintot, sizeof(int));
     for (int i = 0; i < N; i++) s->perm[i] = i;       /* identical_particles_binning.cpp:37-44 */
+    s->lp_scale = orc_lp_scale(16);                   /* lp_leapfrog() : n_prec(16), integrator.hh:53 */
     return 0;
 }
 
@@ -631,17 +632,52 @@ double orc_sys_compute_potential(orc_system* s)
                                  s->cutoff, s->sigma, s->eps);
 }
 
-/* identical_particles.cpp:691-694 (coeff_p = beta, :642-644) */
+/* lp_leapfrog / lp_omf4 (code-carsten/integrator.hh:51-79, 156-197) call round_and_update_momenta and
+ * round_and_update_gauge with N = 10^n_prec (:61, :166); their definitions (update_momenta.hh,
+ * update_gauge.hh) are not part of the reference tree.  Restated as: the update, then the updated
+ * variable rounded to the grid 1/N, v -> round(v N) / N (round half away from zero) -- the
+ * bit-reversible integration of Levesque & Verlet, J. Stat. Phys. 72, 519 (1993): with x and p on the
+ * grid every update adds a rounded increment, and round(-d) = -round(d) makes the map exactly
+ * reversible.  Positions are rounded before the wrap into the box. */
+double orc_lp_scale(int n_prec)
+{
+    double s = 1.0;
+    for (int i = 0; i < n_prec; i++) s *= 10.0;      /* exact for n_prec <= 22 */
+    return s;
+}
+
+static inline double lp_round(double v, double scale) { return round(v * scale) / scale; }
+
+void orc_kick_lp(int N, double* p, const double* f, double dt, double coeff_p, double scale)
+{
+    for (int i = 0; i < 3 * N; i++) p[i] = lp_round(p[i] - dt * coeff_p * f[i], scale);
+}
+
+void orc_drift_lp(int N, double* x, const double* p, double dt, double coeff_x, const double L[3], double scale)
+{
+    for (int i = 0; i < N; i++)
+        for (int d = 0; d < 3; d++) {
+            double v = x[3 * i + d];
+            v += dt * coeff_x * p[3 * i + d];
+            v = lp_round(v, scale);
+            v -= L[d] * floor(v / L[d]);
+            x[3 * i + d] = v;
+        }
+}
+
+/* identical_particles.cpp:691-694 (coeff_p = beta, :642-644); rounded inside an lp_* integration */
 void orc_sys_update_momenta(orc_system* s, double dt)
 {
     orc_sys_compute_force(s);
-    orc_kick(s->N, s->p, s->f, dt, s->beta);
+    if (s->lp_active) orc_kick_lp(s->N, s->p, s->f, dt, s->beta, s->lp_scale);
+    else orc_kick(s->N, s->p, s->f, dt, s->beta);
 }
 
-/* identical_particles.cpp:671-673 (coeff_x = beta / mass, :646-648) */
+/* identical_particles.cpp:671-673 (coeff_x = beta / mass, :646-648); rounded inside an lp_* integration */
 void orc_sys_update_positions(orc_system* s, double dt)
 {
-    orc_drift(s->N, s->x, s->p, dt, s->beta / s->mass, s->L);
+    if (s->lp_active) orc_drift_lp(s->N, s->x, s->p, dt, s->beta / s->mass, s->L, s->lp_scale);
+    else orc_drift(s->N, s->x, s->p, dt, s->beta / s->mass, s->L);
 }
 
 /* integrator.cpp:52-131 (LEAP, OMF2, OMF4) and code-carsten/integrator.hh:200-240 (euler, ruth).
@@ -650,6 +686,14 @@ void orc_sys_update_positions(orc_system* s, double dt)
 void orc_sys_integrate(orc_system* s, int scheme, double dt, int steps, double lambda)
 {
     if (steps < 1) return;
+    /* lp_leapfrog / lp_omf4: the leapfrog / OMF4 sequences with rounded updates (integrator.hh:51-79,
+     * 156-197; the exponent given to set_integrator, :249, :257) */
+    if (scheme == ORC_LP_LEAPFROG || scheme == ORC_LP_OMF4) {
+        s->lp_active = 1;
+        orc_sys_integrate(s, scheme == ORC_LP_LEAPFROG ? ORC_LEAPFROG : ORC_OMF4, dt, steps, lambda);
+        s->lp_active = 0;
+        return;
+    }
     if (scheme == ORC_LEAPFROG) {
         orc_sys_update_momenta(s, dt / 2.);
         orc_sys_update_positions(s, dt);

@@ -65,13 +65,17 @@ double orc_potential_energy_open(int N, const double* x, int n, double a, double
 double orc_kinetic(int N, const double* p, double mass);
 void   orc_kick(int N, double* p, const double* f, double dt, double coeff_p);
 void   orc_drift(int N, double* x, const double* p, double dt, double coeff_x, const double L[3]);
+/* the rounded updates of lp_leapfrog / lp_omf4 (scale = 10^n_prec = orc_lp_scale(n_prec)) */
+double orc_lp_scale(int n_prec);
+void   orc_kick_lp(int N, double* p, const double* f, double dt, double coeff_p, double scale);
+void   orc_drift_lp(int N, double* x, const double* p, double dt, double coeff_x, const double L[3], double scale);
 
 /* ---- RDF ------------------------------------------------------------------------------- */
 void orc_rdf(int N, const double* x, const double L[3], int nbins, double size_bin,
              unsigned long long* counts, double* rdf);
 
 /* ---- a whole system: integrators and the HMC loop -------------------------------------- */
-enum { ORC_LEAPFROG = 0, ORC_OMF4 = 2, ORC_EULER = 4, ORC_RUTH = 5, ORC_OMF2 = 6 };
+enum { ORC_LEAPFROG = 0, ORC_LP_LEAPFROG = 1, ORC_OMF4 = 2, ORC_LP_OMF4 = 3, ORC_EULER = 4, ORC_RUTH = 5, ORC_OMF2 = 6 };
 enum { ORC_BIN_COUNTING = 0, ORC_BIN_REFALGO = 1 };
 
 typedef struct {
@@ -83,6 +87,8 @@ typedef struct {
     double *x, *p, *f, *x_saved;
     int *perm, *perm_tmp, *bincount, *binoffsets;
     long long force_evals;
+    double lp_scale;    /* 10^n_prec of the lp_* integrators (default n_prec = 16) */
+    int lp_active;      /* inside an lp_* integration: kicks and drifts round */
 } orc_system;
 
 int    orc_system_init(orc_system* s, int N, const double L[3], double cutoff, double sigma,

@@ -159,3 +159,84 @@ def adam_step(p, grads, state, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         vh = v / (1 - b2 ** t)
         p[k] = p[k] - lr * mh / (np.sqrt(vh) + eps)
     return p
+
+
+def lj_forces(x, L, rc, eps=1.0, sigma=1.0):
+    """+grad U of the truncated LJ potential (the engine's force convention), FP64, minimum image."""
+    i, j, d, r = pairs(x, L, rc)
+    sr6 = (sigma / r) ** 6
+    coef = 48 * eps * sr6 * (0.5 - sr6) / r ** 2
+    f = np.zeros_like(x)
+    np.add.at(f, i, coef[:, None] * d)
+    return f
+
+
+def dsilu2(a):
+    """Second derivative of silu: s (1 - s) (2 + a (1 - 2 s))."""
+    s = 1 / (1 + np.exp(-a))
+    return s * (1 - s) * (2 + a * (1 - 2 * s))
+
+
+def force_loss_grad(p, configs, L, rc, w_force, bf=False):
+    """Energy + force matching on whole configurations (a list of (N, 3) position arrays of one box):
+
+      loss = mean_i (e_i - u_i)^2 + w_force * mean_{i,d} (dE/dx_id - dU/dx_id)^2
+
+    over all atoms i of all configurations, and its gradient w.r.t. W1, b1, W2, b2, w3, b3.  The force
+    term needs the mixed derivative d(dE/dx)/dtheta: with R_i = dE/dx_i - dU/dx_i the loss depends on the
+    input gradients g through v_i = dL/dg_i = c sum_j phi'(r_ij) (x_i - x_j)/r_ij . (R_i - R_j),
+    c = 2 w_force / (3 n), and S = sum_i g_i . v_i is differentiated by forward mode along u = v * scale
+    (t1 = W1 u, ta1 = silu'(a1) t1, t2 = W2 ta1) followed by reverse mode.  Returns (energy MSE, force MSE,
+    gradient dict).  bf: the bfloat16 forward of mlp() (weights and layer inputs rounded), the tangent
+    pass in FP64 on the rounded weights, as the training kernel runs it."""
+    q = (lambda a: bf16(a)) if bf else (lambda a: a)
+    W1, W2, w3 = q(p["W1"]), q(p["W2"]), q(p["w3"])
+    rows = []
+    for x in configs:
+        i, j, d, r = pairs(x, L, rc)
+        phi, dphi = basis(r, rc)
+        G = np.zeros((x.shape[0], NFEAT))
+        np.add.at(G, i, phi)
+        rows.append((x, i, j, d, r, dphi, G))
+    n = sum(x.shape[0] for x, *_ in rows)
+    c = 2.0 * w_force / (3.0 * n)
+    gr = {k: np.zeros_like(p[k]) for k in ("W1", "b1", "W2", "b2", "w3", "b3")}
+    se = sf = 0.0
+    for x, i, j, d, r, dphi, G in rows:
+        e, (z, a1, h1, a2, h2) = mlp(G, p, bf)
+        g = input_grad(G, p, bf, (z, a1, h1, a2, h2))
+        coef = ((g[i] + g[j]) * dphi).sum(1) / r
+        gradE = np.zeros_like(x)
+        np.add.at(gradE, i, coef[:, None] * d)
+        u = lj_atom_energies(x, L, rc)
+        R = gradE - lj_forces(x, L, rc)
+        res = e - u
+        se += np.sum(res ** 2)
+        sf += np.sum(R ** 2)
+        de = 2 * res / n
+        # v_i[k] = c sum_j dphi_k(r_ij) (d_ij / r_ij) . (R_i - R_j)
+        proj = (d / r[:, None] * (R[i] - R[j])).sum(1)
+        v = np.zeros((x.shape[0], NFEAT))
+        np.add.at(v, i, c * dphi * proj[:, None])
+        uu = v * p["scale"][None, :]
+        # backward quantities of e (de = 1): delta2 = w3 silu'(a2), gamma1 = delta2 W2, delta1 = silu'(a1) gamma1
+        s1, s2 = dsilu(a1), dsilu(a2)
+        delta2 = w3[None, :] * s2
+        gamma1 = q(delta2) @ W2
+        delta1 = gamma1 * s1
+        # tangent along u, then its reverse pass
+        t1 = uu @ W1.T
+        ta1 = s1 * t1
+        t2 = ta1 @ W2.T
+        zb2 = w3[None, :] * dsilu2(a2) * t2
+        ab1 = zb2 @ W2
+        zb1 = dsilu2(a1) * t1 * gamma1 + s1 * ab1
+        D2 = de[:, None] * delta2 + zb2
+        D1 = de[:, None] * delta1 + zb1
+        gr["w3"] += h2.T @ de + (s2 * t2).sum(0)
+        gr["b3"] += np.array([de.sum()])
+        gr["W2"] += D2.T @ h1 + delta2.T @ ta1
+        gr["b2"] += D2.sum(0)
+        gr["W1"] += D1.T @ z + delta1.T @ uu
+        gr["b1"] += D1.sum(0)
+    return se / n, sf / (3 * n), gr

@@ -557,6 +557,34 @@ def test_hmc_records_vs_oracle(chmc, orc, fp64):
     assert np.max(np.abs(r["u"] - ro["u"])) < tol * N
 
 
+def test_rejected_trajectory_keeps_post_md_momenta(chmc):
+    """A rejected trajectory restores x and keeps the post-MD momenta (HMC.cpp:111: only x is restored),
+    also when the lists were rebuilt during its MD, which leaves the momenta in the rebuilt buffer's slot
+    order while the restore returns to the start buffer: p after hmc() equals p after the same MD run by
+    integrate() from the same start and heat bath."""
+    N, rho, T = 4000, 0.8442, 0.722
+
+    def eng():
+        e, _ = make(chmc, N, rho, T=T, seed=3)
+        e.set_skin(0.3)
+        e.list_stats()
+        return e
+    e1 = eng()
+    x0 = e1.positions()
+    _, _, rb0 = e1.list_stats()
+    r, _ = e1.hmc(1, 0, chmc.LEAPFROG, 0.02, 20, seed=8, use_graph=False)
+    _, _, rb1 = e1.list_stats()
+    print("dH", r["dh"][0, 0], "rebuilds", rb1 - rb0)
+    assert r["accepted"][0, 0] == 0 and rb1 - rb0 >= 3       # rejected, with rebuilds inside the MD
+    assert np.array_equal(e1.positions(), x0)
+    e2 = eng()
+    e2.heatbath(8, 1)
+    e2.integrate(chmc.LEAPFROG, 0.02, 20)
+    assert np.array_equal(e1.momenta(), e2.momenta())
+    e1.close()
+    e2.close()
+
+
 def test_error_paths(chmc):
     with pytest.raises(chmc.ChmcError):
         chmc.Engine(chmc.Chain(N=10, L=2.0, cutoff=2.5))            # box <= cutoff
@@ -566,6 +594,6 @@ def test_error_paths(chmc):
     with pytest.raises(chmc.ChmcError):
         e.set_positions(x)                                          # check_in_volume
     with pytest.raises(chmc.ChmcError):
-        e.integrate(3, 0.01, 5)                                     # unknown scheme
+        e.integrate(7, 0.01, 5)                                     # unknown scheme
     with pytest.raises(chmc.ChmcError):
         e.integrate(0, 0.01, 0)                                     # steps < 1 (reference loops forever)

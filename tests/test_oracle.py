@@ -13,6 +13,8 @@ Which check pins which oracle function:
   orc_binning(_refalgo) the O(N*cells) restatement of binning.cpp, invariants, stability
   orc_virial_binning    two-particle closed form and -dU/dlambda under affine scaling
   orc_kick / orc_drift  time reversibility of the integrators and dH ~ dt^2 (LEAP, OMF2) / dt^4 (OMF4)
+  orc_kick_lp / orc_drift_lp (lp_leapfrog, lp_omf4)  bit-exact reversibility (Levesque-Verlet) on the grid,
+                        states on the grid, the plain scheme at n_prec = 16
   orc_rdf               brute-force numpy minimum-image histogram, ideal-gas g(r) -> 1, pair-count sum rule
   orc_heatbath          moments <p^2> = m/beta and <beta K> = 3N/2
   orc_hmc_run           Creutz equality <exp(-dH)> = 1 and acceptance bookkeeping
@@ -222,6 +224,65 @@ def test_integrator_reversible(orc, scheme):
     d = s.x - x0
     d -= L * np.round(d / L)
     assert np.max(np.abs(d)) < 1e-9
+
+
+def _blob(orc, n_prec, scheme_seed=5):
+    """64 atoms (4 x 4 x 4 at spacing 1.1, jittered) in the middle of a 20-sided box: no atom reaches a box
+    face in a short run; positions and momenta on the grid 10^-n_prec."""
+    L = 20.0
+    s = orc.Sim(64, [L] * 3, cutoff=RC, beta=1 / 1.5)
+    s.set_lp_exponent(n_prec)
+    g = 10.0 ** n_prec
+    rng = np.random.default_rng(scheme_seed)
+    site = np.stack(np.meshgrid(*[np.arange(4)] * 3, indexing="ij"), -1).reshape(64, 3)
+    s.x[:] = np.round((8.35 + 1.1 * site + rng.uniform(-0.05, 0.05, (64, 3))) * g) / g
+    s.p[:] = np.round(orc.heatbath(64, 1.0, 1 / 1.5, scheme_seed, 0, 1) * g) / g
+    return s, g
+
+
+def _on_grid(v, g):
+    return bool(np.all(np.round(v * g) / g == v))
+
+
+@pytest.mark.parametrize("scheme,plain", [(1, 0), (3, 2)])
+def test_lp_integrators_bit_reversible(orc, scheme, plain):
+    """lp_leapfrog / lp_omf4 (code-carsten/integrator.hh:51-79, 156-197) round every update to the grid
+    10^-n: forward, flip p, forward again and flip back returns the start state bit for bit (Levesque-Verlet
+    reversibility), and every state stays on the grid.  The plain scheme from the same start comes back only
+    to rounding level, so a restatement that skips the rounding of either variable fails."""
+    s, g = _blob(orc, 6)
+    x0, p0 = s.x.copy(), s.p.copy()
+    s.integrate(scheme, 0.004, 12, 0.1932)
+    assert np.max(np.abs(s.x - x0)) > 1e-3
+    assert _on_grid(s.x, g) and _on_grid(s.p, g)
+    s.p[:] *= -1
+    s.integrate(scheme, 0.004, 12, 0.1932)
+    s.p[:] *= -1
+    assert np.array_equal(s.x, x0) and np.array_equal(s.p, p0)
+    t, _ = _blob(orc, 6)
+    t.integrate(plain, 0.004, 12, 0.1932)
+    t.p[:] *= -1
+    t.integrate(plain, 0.004, 12, 0.1932)
+    t.p[:] *= -1
+    dx = np.max(np.abs(t.x - x0))
+    assert 0 < dx < 1e-10 or not np.array_equal(t.p, p0)
+
+
+@pytest.mark.parametrize("scheme,plain", [(1, 0), (3, 2)])
+def test_lp_integrators_fine_grid_is_plain(orc, scheme, plain):
+    """With n_prec = 16 (the reference's default) the rounding is below FP64 resolution of the state, so
+    lp_* follows the plain scheme to rounding level; with a coarse grid (n_prec = 3) it visibly does not."""
+    a, _ = _sim(orc)
+    b, _ = _sim(orc)
+    a.set_lp_exponent(16)
+    a.integrate(scheme, 0.004, 8, 0.1932)
+    b.integrate(plain, 0.004, 8, 0.1932)
+    assert np.max(np.abs(a.x - b.x)) < 1e-12 and np.max(np.abs(a.p - b.p)) < 1e-11
+    c, _ = _sim(orc)
+    c.set_lp_exponent(3)
+    c.integrate(scheme, 0.004, 8, 0.1932)
+    assert _on_grid(c.x, 1e3) and _on_grid(c.p, 1e3)
+    assert np.max(np.abs(c.x - b.x)) > 1e-5
 
 
 @pytest.mark.parametrize("scheme,order", [(0, 2), (6, 2), (2, 4)])

@@ -27,3 +27,19 @@ def test_mean_err_and_pressure():
     assert abs(m - 3.0) < 4 * err and err == pytest.approx(2.0 / np.sqrt(40000), rel=0.1)
     assert stats.zscore((1.0, 0.3), (1.0, 0.4)) == 0.0
     assert stats.pressure(np.array([0.0]), 100, 125.0, 0.5)[0] == pytest.approx(100 / 125.0 / 0.5)
+
+
+def test_no_information_series():
+    """A chain that never moved (constant U/N, as with zero acceptance) has no tau_int: NaN, not n / 2."""
+    x = np.full(20, -5.3600555704884645)
+    tau, w = stats.tau_int(x)
+    assert np.isnan(tau) and w == 0
+    assert np.isnan(stats.ess(x)) and not stats.reliable(x)
+    # rounding-level spread around a large mean is no information either
+    y = 1e6 + np.array([0.0, 1e-10, 0.0, -1e-10] * 5)
+    assert np.isnan(stats.tau_int(y)[0])
+    # a chain with a handful of accepted moves is informative but not reliable
+    z = np.repeat([1.0, 2.0, 1.5, 3.0], 5)
+    assert np.isfinite(stats.tau_int(z)[0]) and not stats.reliable(z)
+    m, err, t = stats.mean_err(x)
+    assert m == pytest.approx(x[0], rel=1e-15) and np.isnan(err) and np.isnan(t)

@@ -129,3 +129,64 @@ def test_adam_matches_torch_optimizer():
         assert np.allclose(p["W"], t.detach().numpy(), rtol=0, atol=1e-15)
         if k == 0:
             assert np.allclose(p["W"], w0 - 1e-3 * grads[0] / (np.abs(grads[0]) + 1e-8), atol=1e-15)
+
+
+def _torch_force_loss(configs, L, p, w_force):
+    """The energy + force-matching loss built in torch (FP64), forces by autograd with create_graph, so
+    that its parameter gradient contains the mixed derivative d(dE/dx)/dtheta by double backward."""
+    T = {k: torch.tensor(v, dtype=torch.float64, requires_grad=k not in ("shift", "scale")) for k, v in p.items()}
+    mu = torch.tensor(S.MU)
+    n = sum(len(x) for x in configs)
+    le = lf = 0.0
+    for x in configs:
+        xt = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+        d = xt[:, None, :] - xt[None, :, :]
+        d = d - L * torch.round(d / L)
+        r2 = (d ** 2).sum(-1) + torch.eye(len(x), dtype=torch.float64) * 1e6
+        r = torch.sqrt(r2)
+        mask = (r < RC).double()
+        e = torch.exp(-((r[..., None] - mu) ** 2) / (2 * S.WIDTH ** 2))
+        fc = 0.5 * (torch.cos(np.pi * r / RC) + 1) * mask
+        G = (e * fc[..., None]).sum(1)
+        z = (G - T["shift"]) * T["scale"]
+        h1 = torch.nn.functional.silu(z @ T["W1"].T + T["b1"])
+        h2 = torch.nn.functional.silu(h1 @ T["W2"].T + T["b2"])
+        e_at = h2 @ T["w3"] + T["b3"][0]
+        gradE, = torch.autograd.grad(e_at.sum(), xt, create_graph=True)
+        u = torch.tensor(S.lj_atom_energies(x, L, RC))
+        R = gradE - torch.tensor(S.lj_forces(x, L, RC))
+        le = le + ((e_at - u) ** 2).sum()
+        lf = lf + (R ** 2).sum()
+    loss = le / n + w_force * lf / (3 * n)
+    loss.backward()
+    return (le / n).item(), (lf / (3 * n)).item(), {k: T[k].grad.numpy() for k in ("W1", "b1", "W2", "b2", "w3", "b3")}
+
+
+def test_force_matching_gradient_vs_torch_double_backward():
+    """The analytic force-matching gradient (forward-mode tangent + reverse pass) against torch's double
+    backward on two configurations; a dropped tangent term or sign fails it."""
+    cfgs = [_config(N=64, rho=0.6, seed=s)[0] for s in (12, 13)]
+    L = _config(N=64, rho=0.6, seed=12)[1]
+    p = S.init_params(8)
+    p["shift"] = np.full(32, 0.2)
+    p["scale"] = np.linspace(0.5, 1.5, 32)
+    p["b1"] = np.random.default_rng(2).normal(0, 0.3, 64)
+    p["b2"] = np.random.default_rng(3).normal(0, 0.3, 64)
+    me, mf, grads = S.force_loss_grad(p, cfgs, L, RC, w_force=0.7)
+    te, tf, tg = _torch_force_loss(cfgs, L, p, 0.7)
+    assert me == pytest.approx(te, rel=1e-12) and mf == pytest.approx(tf, rel=1e-10)
+    for k, g in grads.items():
+        assert np.allclose(g, tg[k], rtol=1e-8, atol=1e-11), (k, np.abs(g - tg[k]).max())
+    # without the force term it is the energy-only gradient
+    _, _, g0 = S.force_loss_grad(p, cfgs, L, RC, w_force=0.0)
+    G = np.concatenate([S.descriptors(x, L, RC) for x in cfgs])
+    u = np.concatenate([S.lj_atom_energies(x, L, RC) for x in cfgs])
+    _, ge = S.loss_grad(p, G, u)
+    for k in ge:
+        assert np.allclose(g0[k], ge[k], rtol=1e-12, atol=1e-14)
+
+
+def test_lj_forces_finite_difference(orc):
+    x, L = _config(N=125, rho=0.7, seed=14)          # L >= 2 r_c: minimum image = all images
+    f = S.lj_forces(x, L, RC)
+    assert np.allclose(f, orc.force_all(x, [L] * 3, RC), rtol=1e-10, atol=1e-10)
