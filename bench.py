@@ -380,6 +380,47 @@ def cell_md_run(args, e, chmc, stats):
     print(json.dumps(out))
 
 
+def kernels_workload(args):
+    """--workload kernels: each kernel of the hot path exercised on the N=32768 workload so that a
+    kernel trace (rocprofv3) times them all in one run: the O(N) binning and the 27-cell FP32 sweep
+    (chmc_time_force), the RDF in 500 bins, graph-replayed HMC trajectories (heat bath, list sweeps and
+    rebuilds, FP64 energy, Metropolis, accept/restore) and the surrogate's descriptor, MLP and force
+    kernels.  Prints one JSON line with the host-visible timings and the algorithmic units per launch
+    (atoms, pairs) that the per-kernel rates divide by."""
+    import chemhmc_amd as chmc
+
+    L = (N / RHO) ** (1 / 3)
+    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
+    e.init_lattice(1, 0.05)
+    e.hmc(THERM, THERM, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10)
+    out = {"workload": "kernels at N=32768 rho=0.8442 T=0.722", "N": N}
+    ms_bin, ms_force = e.time_force(reps=args.reps)
+    ncand, nint = e.pair_counts(0)
+    out["binning_ms"], out["cell_sweep_ms"] = ms_bin, ms_force
+    out["cell_sweep_flops"] = 8.0 * ncand + 13.0 * nint
+    nb = 500
+    t0 = time.perf_counter()
+    for _ in range(args.reps):
+        e.rdf(nb, L / (2 * nb))
+    out["rdf_ms_host"] = (time.perf_counter() - t0) / args.reps * 1e3
+    out["rdf_pairs"] = N * (N - 1) / 2
+    t0 = time.perf_counter()
+    e.hmc(args.reps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11)
+    out["hmc_ms_per_traj_host"] = (time.perf_counter() - t0) / args.reps * 1e3
+    rng = np.random.default_rng(1)
+    p = {"W1": rng.normal(0, 1 / np.sqrt(32), (64, 32)), "b1": np.zeros(64),
+         "W2": rng.normal(0, 1 / np.sqrt(64), (64, 64)), "b2": np.zeros(64),
+         "w3": rng.normal(0, 1 / np.sqrt(64), 64), "b3": np.zeros(1), "shift": np.zeros(32), "scale": np.ones(32)}
+    e.surrogate_set_params(p)
+    t0 = time.perf_counter()
+    for _ in range(args.reps):
+        e.surrogate_eval(store_force=True)
+    out["surrogate_eval_ms_host"] = (time.perf_counter() - t0) / args.reps * 1e3
+    out["surrogate_mlp_flops_fwd"] = 2.0 * (32 * 64 + 64 * 64 + 64) * N
+    e.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -395,9 +436,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every trajectory directly instead of replaying the captured graph")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c5s"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c5s", "kernels"],
                     help="c2 (default, the headline line), c3 batched chains, c4 surrogate vs plain, c5 N=1M, "
-                         "c5s the surrogate fine-tuned and run at N=1M")
+                         "c5s the surrogate fine-tuned and run at N=1M, kernels: every hot-path kernel a few "
+                         "times at N=32768 (for kernel traces)")
+    ap.add_argument("--reps", type=int, default=20, help="kernels: repetitions of each kernel group")
     ap.add_argument("--train-steps", type=int, default=4000, help="c4, c5s: Adam steps of the surrogate")
     ap.add_argument("--finetune-steps", type=int, default=1000, help="c5s: Adam steps of the fine-tuning")
     ap.add_argument("--cell-md", action="store_true",
@@ -410,6 +453,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.workload == "kernels":
+        kernels_workload(args)
+        return
     if args.workload == "c4":
         surrogate_workload(args)
         return

@@ -22,6 +22,8 @@
 #include "chmc_internal.hpp"
 #include "surrogate.hpp"
 
+#include <algorithm>
+
 namespace chmc {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -479,26 +481,28 @@ __global__ __launch_bounds__(128) void k_surr_grad(SurrView s, const float* __re
 }
 
 // fixed-order sum of the workgroup partials (no update): gradient checks against the oracle
-__global__ __launch_bounds__(256) void k_surr_sum(const float* __restrict__ partial, int nparts, float* __restrict__ out)
+// (partials of ngrad floats: the trained parameters, then ngrad - kSurrNTrain loss sums)
+__global__ __launch_bounds__(256) void k_surr_sum(const float* __restrict__ partial, int nparts, int ngrad,
+                                                  float* __restrict__ out)
 {
     const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= kSurrNGrad) return;
+    if (p >= ngrad) return;
     float gsum = 0.f;
-    for (int b = 0; b < nparts; b++) gsum += partial[(size_t)b * kSurrNGrad + p];
+    for (int b = 0; b < nparts; b++) gsum += partial[(size_t)b * ngrad + p];
     out[p] = gsum;
 }
 
 // one thread per trainable parameter: fixed-order sum over workgroup partials, then Adam
-__global__ __launch_bounds__(256) void k_surr_adam(SurrView s, const float* __restrict__ partial, int nparts,
+__global__ __launch_bounds__(256) void k_surr_adam(SurrView s, const float* __restrict__ partial, int nparts, int ngrad,
                                                    float lr, float beta1, float beta2, float eps, int step,
                                                    float* __restrict__ loss_out)
 {
     const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= kSurrNGrad) return;
+    if (p >= ngrad) return;
     float gsum = 0.f;
-    for (int b = 0; b < nparts; b++) gsum += partial[(size_t)b * kSurrNGrad + p];
-    if (p == kSurrNGrad - 1) {                   // the loss slot
-        if (loss_out) *loss_out = gsum;
+    for (int b = 0; b < nparts; b++) gsum += partial[(size_t)b * ngrad + p];
+    if (p >= kSurrNTrain) {                      // the loss slots
+        if (loss_out) loss_out[p - kSurrNTrain] = gsum;
         return;
     }
     float m = s.adam_m[p] = beta1 * s.adam_m[p] + (1.f - beta1) * gsum;
@@ -535,6 +539,347 @@ __global__ __launch_bounds__(256) void k_surr_targets(DevView v, SurrView s, flo
         acc += sr6 * (sr6 - 1.0);
     }
     u_out[a] = (float)(2.0 * C.eps * acc);
+}
+
+// --------------------------------------------------------------------------------------------------
+// force matching: loss = mean_i (e_i - u_i)^2 + w_f mean_{i,d} (dE/dx_id - dU/dx_id)^2 over whole
+// configurations (all atoms of every chain at the current positions)
+// --------------------------------------------------------------------------------------------------
+__device__ __forceinline__ float d2silu_f(float a)
+{
+    const float s = 1.0f / (1.0f + __expf(-a));
+    return s * (1.0f - s) * (2.0f + a * (1.0f - 2.0f * s));
+}
+
+// per slot: R = dE/dx - dU/dx (surrogate gradient from g = dE/dG, LJ gradient 48 eps sr6 (1/2 - sr6) / r^2 d
+// in FP32 from the pair list) and the per-atom LJ energy u = 2 eps sum_j sr6 (sr6 - 1) (FP64 pairs);
+// 32 lanes per atom: lane f takes feature f of every pair for dE/dx and pairs f, f + 32, ... for U
+__global__ __launch_bounds__(256) void k_surr_fm_resid(DevView v, SurrView s, const float* __restrict__ g,
+                                                       float4* __restrict__ R, float* __restrict__ u_out)
+{
+    const int a = blockIdx.x * 8 + (threadIdx.x >> 5);
+    const int f = threadIdx.x & 31;
+    if (a >= v.Ntot) return;
+    const int atom = __float_as_int(v.xs[a].w);
+    const ChainDev& C = v.chains[v.atom_chain[atom]];
+    const float mu = mu_k(f), c2 = 1.0f / (2.0f * kSurrWidth * kSurrWidth), iw2 = 1.0f / (kSurrWidth * kSurrWidth);
+    const float gi = g[(size_t)a * kSurrFeat + f];
+    const float sig2 = (float)C.sigma2, c48 = (float)(48.0 * C.eps);
+    const int n = s.nb_n[a];
+    float ex_ = 0.f, ey = 0.f, ez = 0.f, lx = 0.f, ly = 0.f, lz = 0.f;
+    double uacc = 0.0;
+    for (int q = 0; q < n; q++) {
+        const size_t e = (size_t)a * kSurrMaxNb + q;
+        const float4 d = s.nb_d[e];
+        const float2 fc = s.nb_f[e];
+        const int j = s.nb_j[e];
+        const float t = d.w - mu;
+        const float ex = __expf(-t * t * c2);
+        const float dphi = ex * (fc.y - t * iw2 * fc.x);
+        const float term = (gi + g[(size_t)j * kSurrFeat + f]) * dphi / d.w;
+        ex_ += term * d.x;
+        ey += term * d.y;
+        ez += term * d.z;
+        if ((q & 31) == f) {
+            const float sr2 = sig2 / (d.w * d.w), sr6 = sr2 * sr2 * sr2;
+            const float cf = c48 * sr6 * (0.5f - sr6) / (d.w * d.w);
+            lx += cf * d.x;
+            ly += cf * d.y;
+            lz += cf * d.z;
+            const double r = d.w, dr2 = C.sigma2 / (r * r), dr6 = dr2 * dr2 * dr2;
+            uacc += dr6 * (dr6 - 1.0);
+        }
+    }
+#pragma unroll
+    for (int m = 1; m < 32; m <<= 1) {
+        ex_ += __shfl_xor(ex_, m, 64);
+        ey += __shfl_xor(ey, m, 64);
+        ez += __shfl_xor(ez, m, 64);
+        lx += __shfl_xor(lx, m, 64);
+        ly += __shfl_xor(ly, m, 64);
+        lz += __shfl_xor(lz, m, 64);
+        uacc += __shfl_xor(uacc, m, 64);
+    }
+    if (f == 0) {
+        R[a] = make_float4(ex_ - lx, ey - ly, ez - lz, 0.f);
+        u_out[a] = (float)(2.0 * C.eps * uacc);
+    }
+}
+
+// per slot and feature: v_k(i) = c sum_j phi'_k(r_ij) (d_ij / r_ij) . (R_i - R_j), c = 2 w_f / (3 n): the
+// derivative of the force term with respect to g(i) = dE/dG(i)
+__global__ __launch_bounds__(256) void k_surr_fm_v(DevView v, SurrView s, const float4* __restrict__ R, float cw,
+                                                   float* __restrict__ vout)
+{
+    const int a = blockIdx.x * 8 + (threadIdx.x >> 5);
+    const int f = threadIdx.x & 31;
+    if (a >= v.Ntot) return;
+    const float mu = mu_k(f), c2 = 1.0f / (2.0f * kSurrWidth * kSurrWidth), iw2 = 1.0f / (kSurrWidth * kSurrWidth);
+    const float4 Ri = R[a];
+    const int n = s.nb_n[a];
+    float acc = 0.f;
+    for (int q = 0; q < n; q++) {
+        const size_t e = (size_t)a * kSurrMaxNb + q;
+        const float4 d = s.nb_d[e];
+        const float2 fc = s.nb_f[e];
+        const float4 Rj = R[s.nb_j[e]];
+        const float t = d.w - mu;
+        const float dphi = __expf(-t * t * c2) * (fc.y - t * iw2 * fc.x);
+        acc += dphi * (d.x * (Ri.x - Rj.x) + d.y * (Ri.y - Rj.y) + d.z * (Ri.z - Rj.z)) / d.w;
+    }
+    vout[(size_t)a * kSurrFeat + f] = cw * acc;
+}
+
+constexpr int kFmRows = 32;            // rows per chunk of the force-matching gradient (2 waves x 16)
+
+struct FmSmem {
+    MlpSmem m;
+    float D1[kFmRows][kSurrHid], D2[kFmRows][kSurrHid];      // de delta + tangent terms
+    float dl1[kFmRows][kSurrHid], dl2[kFmRows][kSurrHid];    // delta1, delta2 (backward of e with de = 1)
+    float q3[kFmRows][kSurrHid];                             // h2 de + silu'(a2) t2  (w3 gradient)
+    __bf16 h1[kFmRows][kSurrHid], ta1[kFmRows][kSurrHid];
+    __bf16 z[kFmRows][kSurrFeat], uu[kFmRows][kSurrFeat];
+    float de[kFmRows];
+    float lsum[2][2];                                        // per wave: sum (e - u)^2, sum |R|^2
+};
+
+// D layout -> bf16 A-layout transpose buffer of this wave, then the two K = 32 fragments of a 64-wide row
+__device__ __forceinline__ void act_store(MlpSmem& sm, const f32x4 x[4])
+{
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) sm.act[w][4 * (l >> 4) + i][16 * t + (l & 15)] = (__bf16)x[t][i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// x[16 rows][64] (act buffer) times a 64 x 64 bf16 matrix given by rows B[n][k] (B = W2: x W2^T; B = W2T: x W2)
+__device__ __forceinline__ void act_times(MlpSmem& sm, const __bf16* B, f32x4 out[4])
+{
+    const int w = threadIdx.x >> 6;
+    const bf16x8 a0 = frag_rows(&sm.act[w][0][0], kSurrHid, 0);
+    const bf16x8 a1 = frag_rows(&sm.act[w][0][0], kSurrHid, 32);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, frag_rows(B + 16 * t * kSurrHid, kSurrHid, 0), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, frag_rows(B + 16 * t * kSurrHid, kSurrHid, 32), acc, 0, 0, 0);
+        out[t] = acc;
+    }
+}
+
+// Gradient of the energy + force-matching loss over rows [0, nrows) (sorted slots of the current
+// configuration).  With d = dL/dg = v (from k_surr_fm_v) the force term is differentiated by a forward
+// tangent along u = v * scale through the input gradient, then reverse mode:
+//   delta2 = w3 silu'(a2), gamma1 = delta2 W2, delta1 = gamma1 silu'(a1)           (de = 1 backward)
+//   t1 = W1 u, ta1 = silu'(a1) t1, t2 = W2 ta1, zb2 = w3 silu''(a2) t2, ab1 = zb2 W2,
+//   zb1 = silu''(a1) t1 gamma1 + silu'(a1) ab1, D2 = de delta2 + zb2, D1 = de delta1 + zb1
+//   dW2 = D2^T h1 + delta2^T ta1, dW1 = D1^T z + delta1^T u, db = sum D, dw3 = h2^T de + sum silu'(a2) t2
+// bfloat16 operands of the MFMA products (delta2, u, ta1, zb2) are rounded once and the rounded ta1 and
+// u enter the weight gradients.  Each workgroup takes rows_per_wg consecutive rows in chunks of 32 and
+// writes one partial of kSurrNTrain + 2 floats (the last two: sum (e - u)^2 and sum |R|^2).
+__global__ __launch_bounds__(128) void k_surr_grad_fm(SurrView s, const float* __restrict__ G,
+                                                      const float* __restrict__ u, const float* __restrict__ vin,
+                                                      const float4* __restrict__ R, int nrows, int rows_per_wg,
+                                                      float two_over_n, float* __restrict__ partial)
+{
+    __shared__ FmSmem sm;
+    load_weights(sm.m, s);
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+    if (tid < 4) sm.lsum[tid >> 1][tid & 1] = 0.f;
+    __syncthreads();
+    // register-blocked weight gradients: blocks b = tid + 128 j of 4 x 4 (W2: b < 256, W1: 256 <= b < 384)
+    float acc[3][4][4];
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int y = 0; y < 4; y++) acc[j][x][y] = 0.f;
+    float accb = 0.f, accw3 = 0.f, accb3 = 0.f;
+    const int wg0 = blockIdx.x * rows_per_wg;
+    const int wg1 = min(nrows, wg0 + rows_per_wg);
+    for (int base = wg0; base < wg1; base += kFmRows) {
+        const int r0 = base + 16 * w;
+        WaveOut o;
+        mlp_forward_wave(sm.m, G, r0, wg1, o);
+        float de[4];
+        float le = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int rr = r0 + 4 * (l >> 4) + i;
+            const float res = rr < wg1 ? o.e[i] - u[rr] : 0.f;
+            de[i] = two_over_n * res;
+            le += res * res;
+        }
+        f32x4 s1[4], s2[4], dl2[4], gam[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int n = 16 * t + (l & 15);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                s1[t][i] = dsilu_f(o.a1[t][i]);
+                s2[t][i] = dsilu_f(o.a2[t][i]);
+                dl2[t][i] = sm.m.w3[n] * s2[t][i];
+            }
+        }
+        act_store(sm.m, dl2);
+        act_times(sm.m, &sm.m.W2T[0][0], gam);              // gamma1 = bf16(delta2) W2
+        // u = bf16(v * scale) in A layout: row l % 16, features 8 (l / 16) .. + 7
+        bf16x8 ua;
+        {
+            const int m = l & 15, k0 = 8 * (l >> 4);
+            const int rr = r0 + m;
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                ua[i] = (__bf16)(rr < wg1 ? vin[(size_t)rr * kSurrFeat + k0 + i] * sm.m.scale[k0 + i] : 0.f);
+            const int mm = 16 * w + m;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                sm.uu[mm][k0 + i] = ua[i];
+                const float gv = rr < wg1 ? G[(size_t)rr * kSurrFeat + k0 + i] : 0.f;
+                sm.z[mm][k0 + i] = (__bf16)((gv - sm.m.shift[k0 + i]) * sm.m.scale[k0 + i]);
+            }
+        }
+        f32x4 t1[4], ta1[4], t2[4], zb2[4], ab1[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            f32x4 a = { 0.f, 0.f, 0.f, 0.f };
+            t1[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ua, frag_rows(&sm.m.W1[16 * t][0], kSurrFeat, 0), a, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) ta1[t][i] = s1[t][i] * t1[t][i];
+        }
+        act_store(sm.m, ta1);
+        act_times(sm.m, &sm.m.W2[0][0], t2);                // t2 = bf16(ta1) W2^T
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int n = 16 * t + (l & 15);
+#pragma unroll
+            for (int i = 0; i < 4; i++) zb2[t][i] = sm.m.w3[n] * d2silu_f(o.a2[t][i]) * t2[t][i];
+        }
+        // rows of ta1 (rounded as the product saw them) into the gradient stage before act is reused
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) sm.ta1[16 * w + 4 * (l >> 4) + i][16 * t + (l & 15)] = (__bf16)ta1[t][i];
+        act_store(sm.m, zb2);
+        act_times(sm.m, &sm.m.W2T[0][0], ab1);              // ab1 = bf16(zb2) W2
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int n = 16 * t + (l & 15);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = 16 * w + 4 * (l >> 4) + i;
+                const bool valid = r0 + 4 * (l >> 4) + i < wg1;
+                const float dl1 = gam[t][i] * s1[t][i];
+                const float zb1 = d2silu_f(o.a1[t][i]) * t1[t][i] * gam[t][i] + s1[t][i] * ab1[t][i];
+                const float h2 = (float)(__bf16)silu_f(o.a2[t][i]);
+                sm.D1[m][n] = valid ? de[i] * dl1 + zb1 : 0.f;
+                sm.D2[m][n] = valid ? de[i] * dl2[t][i] + zb2[t][i] : 0.f;
+                sm.dl1[m][n] = valid ? dl1 : 0.f;
+                sm.dl2[m][n] = valid ? dl2[t][i] : 0.f;
+                sm.q3[m][n] = valid ? h2 * de[i] + s2[t][i] * t2[t][i] : 0.f;
+                sm.h1[m][n] = (__bf16)silu_f(o.a1[t][i]);
+            }
+        }
+        if ((l & 15) == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) sm.de[16 * w + 4 * (l >> 4) + i] = de[i];
+        }
+        // loss sums: (e - u)^2 over the lane's rows (lanes l % 16 == 0 hold them), |R|^2 over rows l < 16
+        float lf = 0.f;
+        if (l < 16 && r0 + l < wg1) {
+            const float4 q = R[r0 + l];
+            lf = q.x * q.x + q.y * q.y + q.z * q.z;
+        }
+        le = (l & 15) == 0 ? le : 0.f;
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            le += __shfl_xor(le, m, 64);
+            lf += __shfl_xor(lf, m, 64);
+        }
+        if (l == 0) {
+            sm.lsum[w][0] += le;
+            sm.lsum[w][1] += lf;
+        }
+        __syncthreads();
+        // weight gradients of this chunk
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int b = tid + 128 * j;
+            if (b < 256) {
+                const int o0 = 4 * (b >> 4), i0 = 4 * (b & 15);
+                for (int m = 0; m < kFmRows; m++) {
+                    const float4 dv = *reinterpret_cast<const float4*>(&sm.D2[m][o0]);
+                    const float4 ev = *reinterpret_cast<const float4*>(&sm.dl2[m][o0]);
+                    const float d4[4] = { dv.x, dv.y, dv.z, dv.w }, e4[4] = { ev.x, ev.y, ev.z, ev.w };
+#pragma unroll
+                    for (int y = 0; y < 4; y++) {
+                        const float h = (float)sm.h1[m][i0 + y], ta = (float)sm.ta1[m][i0 + y];
+#pragma unroll
+                        for (int x = 0; x < 4; x++) acc[j][x][y] += d4[x] * h + e4[x] * ta;
+                    }
+                }
+            } else {
+                const int bb = b - 256, o0 = 4 * (bb >> 3), k0 = 4 * (bb & 7);
+                for (int m = 0; m < kFmRows; m++) {
+                    const float4 dv = *reinterpret_cast<const float4*>(&sm.D1[m][o0]);
+                    const float4 ev = *reinterpret_cast<const float4*>(&sm.dl1[m][o0]);
+                    const float d4[4] = { dv.x, dv.y, dv.z, dv.w }, e4[4] = { ev.x, ev.y, ev.z, ev.w };
+#pragma unroll
+                    for (int y = 0; y < 4; y++) {
+                        const float zz = (float)sm.z[m][k0 + y], uv = (float)sm.uu[m][k0 + y];
+#pragma unroll
+                        for (int x = 0; x < 4; x++) acc[j][x][y] += d4[x] * zz + e4[x] * uv;
+                    }
+                }
+            }
+        }
+        if (tid < 64) {
+            for (int m = 0; m < kFmRows; m++) {
+                accb += sm.D1[m][tid];
+                accw3 += sm.q3[m][tid];
+            }
+        } else {
+            for (int m = 0; m < kFmRows; m++) accb += sm.D2[m][tid - 64];
+        }
+        if (tid == 0)
+            for (int m = 0; m < kFmRows; m++) accb3 += sm.de[m];
+        __syncthreads();
+    }
+    float* out = partial + (size_t)blockIdx.x * (kSurrNTrain + 2);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int b = tid + 128 * j;
+        if (b < 256) {
+            const int o0 = 4 * (b >> 4), i0 = 4 * (b & 15);
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++) out[kOffW2 + (o0 + x) * kSurrHid + i0 + y] = acc[j][x][y];
+        } else {
+            const int bb = b - 256, o0 = 4 * (bb >> 3), k0 = 4 * (bb & 7);
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++) out[kOffW1 + (o0 + x) * kSurrFeat + k0 + y] = acc[j][x][y];
+        }
+    }
+    if (tid < 64) {
+        out[kOffB1 + tid] = accb;
+        out[kOffW3 + tid] = accw3;
+    } else {
+        out[kOffB2 + tid - 64] = accb;
+    }
+    if (tid == 0) {
+        out[kOffB3] = accb3;
+        out[kSurrNTrain] = sm.lsum[0][0] + sm.lsum[1][0];
+        out[kSurrNTrain + 1] = sm.lsum[0][1] + sm.lsum[1][1];
+    }
 }
 
 // ---- launch wrappers --------------------------------------------------------------------------------
@@ -575,8 +920,8 @@ hipError_t launch_surr_train_step(const SurrView& s, const float* G, const float
     hipLaunchKernelGGL(k_surr_grad, dim3(nb), dim3(128), 0, st, s, G, u, row0, nrows, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_surr_adam, dim3((kSurrNGrad + 255) / 256), dim3(256), 0, st, s, partial, nb, lr, 0.9f,
-                       0.999f, 1e-8f, step, loss_out);
+    hipLaunchKernelGGL(k_surr_adam, dim3((kSurrNGrad + 255) / 256), dim3(256), 0, st, s, partial, nb, kSurrNGrad, lr,
+                       0.9f, 0.999f, 1e-8f, step, loss_out);
     return hipGetLastError();
 }
 
@@ -587,7 +932,38 @@ hipError_t launch_surr_grad_only(const SurrView& s, const float* G, const float*
     hipLaunchKernelGGL(k_surr_grad, dim3(nb), dim3(128), 0, st, s, G, u, row0, nrows, partial);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_surr_sum, dim3((kSurrNGrad + 255) / 256), dim3(256), 0, st, partial, nb, grad_out);
+    hipLaunchKernelGGL(k_surr_sum, dim3((kSurrNGrad + 255) / 256), dim3(256), 0, st, partial, nb, kSurrNGrad, grad_out);
+    return hipGetLastError();
+}
+
+// force matching at the current configuration: residuals, v, per-workgroup partials; then either the
+// summed gradient (grad_out, no update) or an Adam step (lr > 0) with the two loss sums in loss_out[2]
+int surr_fm_nparts(int nrows)
+{
+    const int rows_per_wg = std::max(kFmRows, ((nrows + 255) / 256 + kFmRows - 1) / kFmRows * kFmRows);
+    return (nrows + rows_per_wg - 1) / rows_per_wg;
+}
+
+hipError_t launch_surr_fm(const DevView& v, const SurrView& s, const float* G, const float* g, float4* R, float* u,
+                          float* vbuf, float w_force, float* partial, float* grad_out, float lr, int step,
+                          float* loss_out, hipStream_t st)
+{
+    const int nrows = v.Ntot;
+    const int rows_per_wg = std::max(kFmRows, ((nrows + 255) / 256 + kFmRows - 1) / kFmRows * kFmRows);
+    const int nb = (nrows + rows_per_wg - 1) / rows_per_wg;
+    hipLaunchKernelGGL(k_surr_fm_resid, dim3((nrows + 7) / 8), dim3(256), 0, st, v, s, g, R, u);
+    hipLaunchKernelGGL(k_surr_fm_v, dim3((nrows + 7) / 8), dim3(256), 0, st, v, s, R,
+                       (float)(2.0 * w_force / (3.0 * nrows)), vbuf);
+    hipLaunchKernelGGL(k_surr_grad_fm, dim3(nb), dim3(128), 0, st, s, G, u, vbuf, R, nrows, rows_per_wg,
+                       (float)(2.0 / nrows), partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    constexpr int ng = kSurrNTrain + 2;
+    if (grad_out)
+        hipLaunchKernelGGL(k_surr_sum, dim3((ng + 255) / 256), dim3(256), 0, st, partial, nb, ng, grad_out);
+    else
+        hipLaunchKernelGGL(k_surr_adam, dim3((ng + 255) / 256), dim3(256), 0, st, s, partial, nb, ng, lr, 0.9f, 0.999f,
+                           1e-8f, step, loss_out);
     return hipGetLastError();
 }
 

@@ -36,5 +36,9 @@ hipError_t launch_surr_train_step(const SurrView& s, const float* G, const float
 hipError_t launch_surr_grad_only(const SurrView& s, const float* G, const float* u, long long row0, int nrows,
                                  float* partial, float* grad_out, hipStream_t st);
 hipError_t launch_surr_refresh(const SurrView& s, hipStream_t st);
+int surr_fm_nparts(int nrows);
+hipError_t launch_surr_fm(const DevView& v, const SurrView& s, const float* G, const float* g, float4* R, float* u,
+                          float* vbuf, float w_force, float* partial, float* grad_out, float lr, int step,
+                          float* loss_out, hipStream_t st);
 
 }  // namespace chmc

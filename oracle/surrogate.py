@@ -42,7 +42,32 @@ def init_params(seed=0):
 
 
 def pairs(x, L, rc):
-    """All ordered pairs i != j with r_ij < rc under the minimum image (valid for L >= 2 rc)."""
+    """All ordered pairs i != j with r_ij < rc under the minimum image (valid for L >= 2 rc), sorted by
+    (i, j).  Above 2000 atoms the candidates come from a periodic k-d tree (scipy) instead of the dense
+    N x N distance table; both give the same pairs (tested)."""
+    if x.shape[0] > 2000:
+        return pairs_tree(x, L, rc)
+    return pairs_dense(x, L, rc)
+
+
+def pairs_tree(x, L, rc):
+    from scipy.spatial import cKDTree
+    L = np.broadcast_to(np.asarray(L, float), (3,))
+    xw = x - L * np.floor(x / L)
+    xw[xw >= L] = 0.0                    # a coordinate that rounds up to L wraps to 0
+    ij = cKDTree(xw, boxsize=L).query_pairs(rc * (1 + 1e-9), output_type="ndarray")
+    i = np.concatenate([ij[:, 0], ij[:, 1]])
+    j = np.concatenate([ij[:, 1], ij[:, 0]])
+    d = x[i] - x[j]
+    d -= L * np.round(d / L)
+    r2 = (d ** 2).sum(-1)
+    keep = r2 < rc * rc
+    i, j, d, r2 = i[keep], j[keep], d[keep], r2[keep]
+    order = np.lexsort((j, i))
+    return i[order], j[order], d[order], np.sqrt(r2[order])
+
+
+def pairs_dense(x, L, rc):
     L = np.broadcast_to(np.asarray(L, float), (3,))
     d = x[:, None, :] - x[None, :, :]
     d -= L * np.round(d / L)
@@ -187,8 +212,10 @@ def force_loss_grad(p, configs, L, rc, w_force, bf=False):
     input gradients g through v_i = dL/dg_i = c sum_j phi'(r_ij) (x_i - x_j)/r_ij . (R_i - R_j),
     c = 2 w_force / (3 n), and S = sum_i g_i . v_i is differentiated by forward mode along u = v * scale
     (t1 = W1 u, ta1 = silu'(a1) t1, t2 = W2 ta1) followed by reverse mode.  Returns (energy MSE, force MSE,
-    gradient dict).  bf: the bfloat16 forward of mlp() (weights and layer inputs rounded), the tangent
-    pass in FP64 on the rounded weights, as the training kernel runs it."""
+    gradient dict).  bf: the bfloat16 forward of mlp() (weights and layer inputs rounded), and the
+    tangent pass on the rounded weights with the operands the training kernel feeds to its bfloat16
+    products rounded the same way (u, ta1, zb2; the rounded u and ta1 also enter the weight
+    gradients), FP64 otherwise."""
     q = (lambda a: bf16(a)) if bf else (lambda a: a)
     W1, W2, w3 = q(p["W1"]), q(p["W2"]), q(p["w3"])
     rows = []
@@ -218,7 +245,7 @@ def force_loss_grad(p, configs, L, rc, w_force, bf=False):
         proj = (d / r[:, None] * (R[i] - R[j])).sum(1)
         v = np.zeros((x.shape[0], NFEAT))
         np.add.at(v, i, c * dphi * proj[:, None])
-        uu = v * p["scale"][None, :]
+        uu = q(v * p["scale"][None, :])
         # backward quantities of e (de = 1): delta2 = w3 silu'(a2), gamma1 = delta2 W2, delta1 = silu'(a1) gamma1
         s1, s2 = dsilu(a1), dsilu(a2)
         delta2 = w3[None, :] * s2
@@ -226,10 +253,10 @@ def force_loss_grad(p, configs, L, rc, w_force, bf=False):
         delta1 = gamma1 * s1
         # tangent along u, then its reverse pass
         t1 = uu @ W1.T
-        ta1 = s1 * t1
+        ta1 = q(s1 * t1)
         t2 = ta1 @ W2.T
         zb2 = w3[None, :] * dsilu2(a2) * t2
-        ab1 = zb2 @ W2
+        ab1 = q(zb2) @ W2
         zb1 = dsilu2(a1) * t1 * gamma1 + s1 * ab1
         D2 = de[:, None] * delta2 + zb2
         D1 = de[:, None] * delta1 + zb1

@@ -190,3 +190,13 @@ def test_lj_forces_finite_difference(orc):
     x, L = _config(N=125, rho=0.7, seed=14)          # L >= 2 r_c: minimum image = all images
     f = S.lj_forces(x, L, RC)
     assert np.allclose(f, orc.force_all(x, [L] * 3, RC), rtol=1e-10, atol=1e-10)
+
+
+def test_tree_pairs_equal_dense_pairs():
+    """The k-d tree pair search (used above 2000 atoms) against the dense minimum-image table."""
+    for N, rho, seed in ((1000, 0.8442, 21), (2197, 0.5, 22)):
+        x, L = _config(N=N, rho=rho, seed=seed)
+        a = S.pairs_dense(x, L, RC)
+        b = S.pairs_tree(x, L, RC)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert np.allclose(a[2], b[2], rtol=0, atol=1e-12) and np.allclose(a[3], b[3], rtol=0, atol=1e-12)
