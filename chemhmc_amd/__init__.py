@@ -101,6 +101,7 @@ def _declare(L):
         "chmc_hmc_run": ([P, C.POINTER(HmcParams), C.c_void_p, C.c_void_p], _I),
         "chmc_rdf": ([P, _I, _I, _D, C.c_void_p, C.c_void_p], _I),
         "chmc_potential_energy_open": ([_PD, _I, _I, _D, _D, C.POINTER(_D)], _I),
+        "chmc_potential_energy_open_noalloc": ([C.c_void_p, _I, _I, _D, _D, C.c_void_p, C.c_void_p], _I),
         "chmc_set_force_tiling": ([P, _I], _I),
         "chmc_get_force_tiling": ([P, C.POINTER(_I)], _I),
         "chmc_time_force": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
@@ -136,7 +137,8 @@ EXPORTED = (
     "chmc_bin", "chmc_get_bins", "chmc_compute_force", "chmc_get_force", "chmc_compute_potential",
     "chmc_evaluate_potential", "chmc_virial", "chmc_pair_counts", "chmc_kinetic", "chmc_update_momenta", "chmc_update_positions",
     "chmc_kick", "chmc_integrate", "chmc_set_lp_exponent", "chmc_get_lp_exponent", "chmc_heatbath", "chmc_save_positions", "chmc_restore_positions",
-    "chmc_hmc_run", "chmc_rdf", "chmc_potential_energy_open", "chmc_set_force_tiling",
+    "chmc_hmc_run", "chmc_rdf", "chmc_potential_energy_open", "chmc_potential_energy_open_noalloc",
+    "chmc_set_force_tiling",
     "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_set_chain_skins",
     "chmc_get_chain_skin", "chmc_list_stats",
     "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_trajectory", "chmc_get_trajectory",
@@ -146,6 +148,7 @@ EXPORTED = (
 )
 
 SURR_NPARAMS, SURR_NGRAD, SURR_NFEAT, SURR_NHID = 6401, 6338, 32, 64
+OPEN_NOALLOC_MAX = 65536             # CHMC_OPEN_NOALLOC_MAX: atoms the static scratch of the no-alloc sum holds
 # flat parameter layout: name, shape (W1[64][32], b1, W2[64][64], b2, w3, b3, shift, scale)
 SURR_LAYOUT = (("W1", (64, 32)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("w3", (64,)), ("b3", (1,)),
                ("shift", (32,)), ("scale", (32,)))

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1462,6 +1463,52 @@ extern "C" int chmc_potential_energy_open(const double* xyz, int n_particles, in
     return CHMC_OK;
 }
 
+
+// ---- the same sum without allocation ---------------------------------------------------------------
+// Block partials of chmc_potential_energy_open_noalloc live in this file-scope array, one copy per GPU,
+// held until the process ends.  Host code never passes the array's name to a kernel (on the host it is
+// only a stand-in); it uses the device address that hipGetSymbolAddress returns for the current device.
+static constexpr int kOpenScratchAtoms = CHMC_OPEN_NOALLOC_MAX;
+static constexpr int kOpenScratchTiles = (kOpenScratchAtoms + 255) / 256;
+static __device__ __align__(256) double scratch[kOpenScratchTiles * kOpenScratchTiles];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device, looked up once per device
+static hipError_t open_scratch(double** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+extern "C" int chmc_potential_energy_open_noalloc(const double* xyz_dev, int n_particles, int n, double a, double b,
+                                                  double* result_dev, void* stream)
+{
+    if (!result_dev || n_particles < 0 || (n_particles > 0 && !xyz_dev)) return fail(nullptr, CHMC_E_ARG, "potential_energy_open_noalloc: null pointer or negative size");
+    if (n_particles > kOpenScratchAtoms)
+        return fail(nullptr, CHMC_E_ARG, "potential_energy_open_noalloc: n_particles " + std::to_string(n_particles) +
+                                             " exceeds the scratch capacity of " + std::to_string(kOpenScratchAtoms) + " atoms");
+    double* part = nullptr;
+    hipError_t e = open_scratch(&part);
+    if (e == hipSuccess)
+        e = launch_open_potential_aos(n_particles >= 2 ? xyz_dev : nullptr, n_particles >= 2 ? n_particles : 0, n, a, b,
+                                      part, result_dev, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail(nullptr, CHMC_E_HIP, std::string("potential_energy_open_noalloc: ") + hipGetErrorString(e));
+    return CHMC_OK;
+}
 
 // ---- learned surrogate ---------------------------------------------------------------------------
 static int surr_ensure(chmc_ctx* c)

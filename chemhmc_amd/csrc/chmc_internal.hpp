@@ -261,6 +261,8 @@ hipError_t launch_rdf(const DevView& v, int chain, int natoms, int nbins, double
                       unsigned long long* counts, hipStream_t s);
 hipError_t launch_open_potential(const double* xyz_soa, int n, int nexp, double a, double b,
                                  double* partials, int* nparts, hipStream_t s);
+hipError_t launch_open_potential_aos(const double* xyz_aos, int n, int nexp, double a, double b, double* partials,
+                                     double* result, hipStream_t s);
 hipError_t launch_fill_int(int* p, int value, int n, hipStream_t s);
 
 }  // namespace chmc

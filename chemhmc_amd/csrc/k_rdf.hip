@@ -58,8 +58,9 @@ __global__ __launch_bounds__(256) void k_rdf(DevView v, int chain, int nbins, do
     }
 }
 
-__global__ __launch_bounds__(256) void k_open_potential(const double* __restrict__ xyz, int n, int nexp,
-                                                        double a, double b, double* __restrict__ partials)
+// coordinate d of atom i is xyz[i * si + d * sd]: SoA (si = 1, sd = n) or AoS double[3] (si = 3, sd = 1)
+__global__ __launch_bounds__(256) void k_open_potential(const double* __restrict__ xyz, int n, int si, int sd,
+                                                        int nexp, double a, double b, double* __restrict__ partials)
 {
     __shared__ double jx[256], jy[256], jz[256];
     __shared__ double wpart[4];
@@ -72,8 +73,8 @@ __global__ __launch_bounds__(256) void k_open_potential(const double* __restrict
     }
     const int i = I * 256 + tid, j0 = J * 256 + tid;
     double xi = 0, yi = 0, zi = 0;
-    if (i < n) { xi = xyz[i]; yi = xyz[n + i]; zi = xyz[2 * n + i]; }
-    if (j0 < n) { jx[tid] = xyz[j0]; jy[tid] = xyz[n + j0]; jz[tid] = xyz[2 * n + j0]; }
+    if (i < n) { xi = xyz[i * si]; yi = xyz[i * si + sd]; zi = xyz[i * si + 2 * sd]; }
+    if (j0 < n) { jx[tid] = xyz[j0 * si]; jy[tid] = xyz[j0 * si + sd]; jz[tid] = xyz[j0 * si + 2 * sd]; }
     __syncthreads();
     double s = 0.0;
     if (i < n) {
@@ -108,7 +109,32 @@ hipError_t launch_open_potential(const double* xyz_soa, int n, int nexp, double 
     const int T = (n + 255) / 256;
     *nparts = T * T;
     if (T == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_open_potential, dim3(T, T), dim3(256), 0, s, xyz_soa, n, nexp, a, b, partials);
+    hipLaunchKernelGGL(k_open_potential, dim3(T, T), dim3(256), 0, s, xyz_soa, n, 1, n, nexp, a, b, partials);
+    return hipGetLastError();
+}
+
+// the partials of k_open_potential summed by one thread in block order (the order of the host sum of
+// chmc_potential_energy_open, so both entry points return the same bits); the lower-triangle blocks
+// hold +0.0 and are skipped, which leaves every intermediate sum unchanged
+__global__ void k_open_sum(const double* __restrict__ partials, int T, double* __restrict__ result)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double r = 0.0;
+    for (int I = 0; I < T; I++)
+        for (int J = I; J < T; J++) r += partials[I * T + J];
+    *result = r;
+}
+
+hipError_t launch_open_potential_aos(const double* xyz_aos, int n, int nexp, double a, double b, double* partials,
+                                     double* result, hipStream_t s)
+{
+    const int T = (n + 255) / 256;
+    if (T > 0) {
+        hipLaunchKernelGGL(k_open_potential, dim3(T, T), dim3(256), 0, s, xyz_aos, n, 3, 1, nexp, a, b, partials);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_open_sum, dim3(1), dim3(64), 0, s, partials, T, result);
     return hipGetLastError();
 }
 

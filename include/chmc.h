@@ -184,6 +184,17 @@ int chmc_rdf(chmc_ctx* ctx, int chain, int nbins, double size_bin, unsigned long
  * LennardJones(|r_i - r_j|, n, a, b) (lennardjones.hh:5-9) of an AoS double[3] set, open boundaries */
 int chmc_potential_energy_open(const double* xyz, int n_particles, int n, double a, double b, double* result);
 
+/* No counterpart in the reference.  chmc_potential_energy_open for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): xyz_dev is a device AoS double[3] array of
+ * n_particles atoms, result_dev a device double, stream a hipStream_t (NULL: the null stream).  The work
+ * is queued on the stream and the call returns without waiting; the result has the same bits as
+ * chmc_potential_energy_open's.  Block partials go to a static device buffer, one per GPU, sized for
+ * at most CHMC_OPEN_NOALLOC_MAX atoms (512 KB); larger sets fail with CHMC_E_ARG.  Calls that overlap
+ * on one GPU share that buffer and race.  No hipMalloc, hipFree or stream creation is made. */
+#define CHMC_OPEN_NOALLOC_MAX 65536
+int chmc_potential_energy_open_noalloc(const double* xyz_dev, int n_particles, int n, double a, double b,
+                                       double* result_dev, void* stream);
+
 /* ---- learned surrogate (no reference counterpart) ----------------------------------------------
  * A per-atom MLP energy E = sum_i e(G_i) over 32 radial Gaussian descriptors G_i (centres 0.8..2.5,
  * width 0.1, cosine cutoff at the chain's r_c), 32 -> 64 -> 64 -> 1 with SiLU, bfloat16 MFMA layers
