@@ -123,6 +123,7 @@ def other_workload(args):
     w = configs.WORKLOADS[args.workload]()
     e = chmc.Engine(w.chains)
     e.init_lattice(1, 0.05)
+    e.set_list_check(args.list_check)
     nat = w.natoms
     steps = args.steps if args.steps_given else (200 if args.workload == "c3" else 50)
     # both workloads relax from their lattice starts over the first ~100 trajectories (more rebuilds
@@ -184,7 +185,7 @@ def other_workload(args):
         "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": f"{args.workload}: {len(w.chains)} chain(s), {nat} atoms, "
                                f"{'LEAP' if w.scheme == chmc.LEAPFROG else 'OMF2'} {w.steps}x{w.dt}",
-                   "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "skin": e.skin(),
+                   "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "list_check": e.list_check(), "skin": e.skin(),
                    "chain_skin_range": [float(chain_skins.min()), float(chain_skins.max())], "skin_family": family,
                    "ms_per_traj_by_skin": skins, "warmup_seconds": t_warm},
         "acceptance": float(nacc.sum()) / (steps * len(w.chains)),
@@ -202,49 +203,15 @@ def other_workload(args):
     print(json.dumps(out))
 
 
-def train_base_surrogate(args, chmc):
-    """The surrogate of the c4 benchmark: 32-64-64-1 trained on 5000 configurations of plain-HMC chains
-    of the N=512 system (rho*=0.8, T*=1.5; 20 chains x 250 snapshots).  Returns its parameters, the
-    training rows, the collect and train seconds and the loss history."""
-    rng = np.random.default_rng(4)
-    nch, nconf_per = 20, 250
-    Lc1 = (512 / 0.8) ** (1 / 3)
-    tr = chmc.Engine([chmc.Chain(N=512, L=Lc1, cutoff=RC, beta=1 / 1.5)] * nch)
-    tr.init_lattice(3, 0.05)
-    tr.hmc(100, 101, chmc.LEAPFROG, 0.005, 20, seed=5)          # accept-all burn-in
-    p0 = {"W1": rng.normal(0, 1 / np.sqrt(32), (64, 32)), "b1": np.zeros(64),
-          "W2": rng.normal(0, 1 / np.sqrt(64), (64, 64)), "b2": np.zeros(64),
-          "w3": rng.normal(0, 1 / np.sqrt(64), 64), "b3": np.zeros(1), "shift": np.zeros(32), "scale": np.ones(32)}
-    tr.surrogate_set_params(p0)
-    tr.surrogate_reserve(nconf_per)
-    t0 = time.perf_counter()
-    rows = 0
-    for k in range(nconf_per):
-        tr.hmc(2, 0, chmc.LEAPFROG, 0.005, 20, seed=100 + k)
-        rows = tr.surrogate_collect()
-    t_collect = time.perf_counter() - t0
-    G, u = tr.surrogate_training_data(0, min(rows, 200000))
-    p0["shift"] = G.mean(0).astype(np.float64)
-    p0["scale"] = 1.0 / (G.std(0).astype(np.float64) + 1e-3)
-    p0["b3"] = np.array([float(u.mean())])
-    tr.surrogate_set_params(p0)
-    t0 = time.perf_counter()
-    hist = tr.surrogate_train(args.train_steps, 4096, lr=1e-3, seed=6)
-    t_train = time.perf_counter() - t0
-    params = tr.surrogate_params()
-    tr.close()
-    return params, rows, t_collect, t_train, hist
-
-
 def surrogate_c5_workload(args):
-    """--workload c5s: the c4 surrogate fine-tuned on 200 configurations of the N = 1,048,576 system
-    (rho*=0.8442, T*=0.722; 210 M atom rows held on the device, 27 GB), then surrogate-driven HMC at that
-    size against plain HMC from the same state (OMF2 10 x 0.01): acceptance and ms per trajectory of
-    both, and the fine-tuning loss."""
+    """--workload c5s: the c4 surrogate fine-tuned with energy + force matching on 200 snapshots of the
+    N = 1,048,576 system (rho*=0.8442, T*=0.722; 5 GB of positions held on the device), then
+    surrogate-driven HMC at that size against plain HMC from the same state (OMF2 10 x 0.01): acceptance
+    and ms per trajectory of both, and the fine-tuning losses."""
     import chemhmc_amd as chmc
     from chemhmc_amd import configs
 
-    params, rows0, _, _, _ = train_base_surrogate(args, chmc)
+    params, _, _, _, _ = train_c4_surrogate(args, chmc)
     w = configs.WORKLOADS["c5"]()
     nat = w.natoms
     e = chmc.Engine(w.chains)
@@ -257,22 +224,22 @@ def surrogate_c5_workload(args):
     e.set_skin(0.5)
     e.surrogate_set_params(params)
     nconf = 200
-    e.surrogate_reserve(nconf)
+    e.surrogate_reserve_configs(nconf)
     t0 = time.perf_counter()
-    rows = 0
     for k in range(nconf):
-        e.hmc(1, 1, w.scheme, w.dt, w.steps, lam=w.lam, seed=200 + k, use_graph=not args.no_graph)
-        rows = e.surrogate_collect()
+        e.hmc(1, 10 ** 9, w.scheme, w.dt, w.steps, lam=w.lam, seed=200 + k, use_graph=False)
+        e.surrogate_collect_config()
     t_collect = time.perf_counter() - t0
     t0 = time.perf_counter()
-    hist = e.surrogate_train(args.finetune_steps, 4096, lr=3e-4, seed=7)
+    le, lf = e.surrogate_train_fm(args.finetune_steps, C4_W_FORCE, lr=3e-4, seed=7)
     t_train = time.perf_counter() - t0
+    e.hmc(2, 10 ** 9, w.scheme, w.dt, w.steps, lam=w.lam, seed=299, use_graph=False)     # off the last snapshot
     x0 = e.positions()
     steps = args.steps if args.steps_given else 20
     out = {}
     for name, surr in (("plain", False), ("surrogate", True)):
         e.set_positions(x0)
-        e.set_trajectory(warm + nconf)
+        e.set_trajectory(warm + nconf + 2)
         e.kinetic()
         t0 = time.perf_counter()
         rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=not args.no_graph,
@@ -287,37 +254,82 @@ def surrogate_c5_workload(args):
         "dtype": "bf16 MFMA surrogate MD, f64 LJ accept", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": "c5s: surrogate HMC N=1048576 rho=0.8442 T=0.722 OMF2(0.1932) 10x0.01; the c4 surrogate "
                                "fine-tuned on 200 configurations of this system",
-                   "base_train_rows": rows0, "finetune_rows": rows, "finetune_steps": args.finetune_steps,
-                   "finetune_batch": 4096, "finetune_lr": 3e-4, "collect_seconds": t_collect, "finetune_seconds": t_train,
-                   "finetune_mse_first": float(hist[:20].mean()), "finetune_mse_last": float(hist[-100:].mean())},
+                   "base_train_configs": args.train_configs, "finetune_configs": nconf, "finetune_steps": args.finetune_steps,
+                   "w_force": C4_W_FORCE, "finetune_lr": 3e-4, "collect_seconds": t_collect, "finetune_seconds": t_train,
+                   "energy_mse_first": float(le[:20].mean()), "energy_mse_last": float(le[-100:].mean()),
+                   "force_mse_first": float(lf[:20].mean()), "force_mse_last": float(lf[-100:].mean())},
         "plain": out["plain"], "surrogate": out["surrogate"],
         "ms_per_traj_ratio_surrogate_over_plain": out["surrogate"]["ms_per_traj"] / out["plain"]["ms_per_traj"],
     }
     print(json.dumps(res))
 
 
+C4_NCONF = 5000                 # training snapshots of the N=32768 plain-HMC chain
+C4_W_FORCE = 1.0                # weight of the force-matching term in the training loss
+
+
+def train_c4_surrogate(args, chmc):
+    """The surrogate of the c4 benchmark: 32-64-64-1 on 32 radial descriptors, trained on the GPU with
+    energy + force matching on C4_NCONF snapshots of the plain-HMC chain of the N=32768 system itself
+    (rho*=0.8442, T*=0.722, OMF2 10 x 0.01): one snapshot after each accept-all trajectory (the
+    thermalisation phase of the reference's loop) following the burn-in off the lattice.  Returns the
+    parameters, the collect and train seconds and the two loss histories."""
+    L = (N / RHO) ** (1 / 3)
+    rng = np.random.default_rng(4)
+    tr = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
+    tr.init_lattice(3, 0.05)
+    rec, _ = tr.hmc(THERM, THERM + 1, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=5)          # accept-all burn-in
+    p0 = {"W1": rng.normal(0, 1 / np.sqrt(32), (64, 32)), "b1": np.zeros(64),
+          "W2": rng.normal(0, 1 / np.sqrt(64), (64, 64)), "b2": np.zeros(64),
+          "w3": rng.normal(0, 1 / np.sqrt(64), 64), "b3": np.zeros(1), "shift": np.zeros(32), "scale": np.ones(32)}
+    tr.surrogate_set_params(p0)
+    tr.surrogate_eval()
+    G, _ = tr.surrogate_features()
+    p0["shift"] = G.mean(0).astype(np.float64)
+    p0["scale"] = 1.0 / (G.std(0).astype(np.float64) + 1e-3)
+    p0["b3"] = np.array([float(rec["u"][-1, 0]) / N])
+    tr.surrogate_set_params(p0)
+    nconf = args.train_configs
+    tr.surrogate_reserve_configs(nconf)
+    t0 = time.perf_counter()
+    for k in range(nconf):
+        tr.hmc(1, 10 ** 9, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=1000 + k, use_graph=False)
+        tr.surrogate_collect_config()
+    t_collect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    le, lf = tr.surrogate_train_fm(args.train_steps, C4_W_FORCE, lr=1e-3, seed=6)
+    t_train = time.perf_counter() - t0
+    params = tr.surrogate_params()
+    tr.close()
+    return params, t_collect, t_train, le, lf
+
+
 def surrogate_workload(args):
-    """--workload c4: the learned surrogate trained on configurations of plain-HMC chains of the N=512
-    system (rho*=0.8, T*=1.5; 5000 configurations of 512 atoms), then HMC at N=32768, rho*=0.8442,
-    T*=0.722 (OMF2 10 x 0.01) with surrogate-driven MD and the exact FP64 LJ accept step, against plain
-    HMC from the same state: acceptance, ms per trajectory and ESS/s of U/N of both."""
+    """--workload c4: the learned surrogate (trained by train_c4_surrogate) drives the MD of HMC at
+    N=32768, rho*=0.8442, T*=0.722 (OMF2 10 x 0.01) with the exact FP64 LJ accept step, against plain
+    HMC from the same state and random streams: acceptance, ms per trajectory and ESS/s of U/N of both."""
     import chemhmc_amd as chmc
     from chemhmc_amd import stats
 
-    params, rows, t_collect, t_train, hist = train_base_surrogate(args, chmc)
+    params, t_collect, t_train, le, lf = train_c4_surrogate(args, chmc)
     L = (N / RHO) ** (1 / 3)
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
     e.init_lattice(1, 0.05)
+    e.set_list_check(args.list_check)
     e.hmc(THERM, THERM, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10)
     if args.warmup > 0:
         e.hmc(args.warmup, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11)
     e.surrogate_set_params(params)
     x0 = e.positions()
-    steps = args.steps if args.steps_given else 500
+    t_start = e.trajectory()
+    steps = args.steps if args.steps_given else 2000
     out = {}
     for name, surr in (("plain", False), ("surrogate", True)):
         e.set_positions(x0)
-        e.set_trajectory(THERM + args.warmup)
+        e.set_trajectory(t_start)
+        e.hmc(2, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph, surrogate=surr)   # graph warm-up
+        e.set_positions(x0)
+        e.set_trajectory(t_start)
         e.kinetic()
         t0 = time.perf_counter()
         rec, nacc = e.hmc(steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph,
@@ -327,6 +339,7 @@ def surrogate_workload(args):
         tau, _ = stats.tau_int(uu)
         ok = bool(np.isfinite(tau))
         out[name] = {"acceptance": float(nacc[0]) / steps, "ms_per_traj": el / steps * 1e3,
+                     "mean_dh": float(rec["dh"][:, 0].mean()), "median_abs_dh": float(np.median(np.abs(rec["dh"][:, 0]))),
                      "u_per_n": float(uu.mean()), "tau_int_u": tau if ok else None,
                      "ess_per_s": steps / (2 * tau) / el if ok else None, "reliable": stats.reliable(uu),
                      "value": N * NSTEPS * steps / el}
@@ -336,11 +349,13 @@ def surrogate_workload(args):
         "ms_per_step": out["surrogate"]["ms_per_traj"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16 MFMA surrogate MD, f64 LJ accept", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": "c4: surrogate HMC N=32768 rho=0.8442 T=0.722 OMF2(0.1932) 10x0.01; surrogate 32-64-64-1 "
-                               "trained on 5000 N=512 rho=0.8 T=1.5 configurations",
-                   "train_rows": rows, "train_steps": args.train_steps, "train_batch": 4096, "lr": 1e-3,
+                               "trained with energy + force matching on snapshots of the same system's plain-HMC chain",
+                   "train_configs": args.train_configs, "train_steps": args.train_steps, "w_force": C4_W_FORCE, "lr": 1e-3,
                    "collect_seconds": t_collect, "train_seconds": t_train,
-                   "train_mse_first": float(hist[:20].mean()), "train_mse_last": float(hist[-100:].mean())},
+                   "energy_mse_first": float(le[:20].mean()), "energy_mse_last": float(le[-100:].mean()),
+                   "force_mse_first": float(lf[:20].mean()), "force_mse_last": float(lf[-100:].mean())},
         "plain": out["plain"], "surrogate": out["surrogate"],
+        "ms_per_traj_ratio_surrogate_over_plain": out["surrogate"]["ms_per_traj"] / out["plain"]["ms_per_traj"],
         "ess_per_s_ratio_surrogate_over_plain": (out["surrogate"]["ess_per_s"] / out["plain"]["ess_per_s"]
                                                   if out["surrogate"]["ess_per_s"] and out["plain"]["ess_per_s"]
                                                   else None),
@@ -430,8 +445,10 @@ def main():
     ap.add_argument("--skin", type=float, default=None, help="fixed list skin (no skin autotuning)")
     ap.add_argument("--skin-family", default=None,
                     help="fixed per-chain skins BASE/EXP of the density-scaled family (c3, c5; no skin autotuning)")
-    ap.add_argument("--lanes", type=int, default=0, choices=[0, 4, 8, 16],
+    ap.add_argument("--lanes", type=int, default=0, choices=[0, 4, 8, 16, 812, 816],
                     help="fixed list-sweep lanes per atom (0: autotune)")
+    ap.add_argument("--list-check", type=int, default=2, choices=[1, 2, 3],
+                    help="list MD: rebuild checks before every LIST_CHECK-th force evaluation (chmc_set_list_check)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true",
@@ -442,6 +459,7 @@ def main():
                          "times at N=32768 (for kernel traces)")
     ap.add_argument("--reps", type=int, default=20, help="kernels: repetitions of each kernel group")
     ap.add_argument("--train-steps", type=int, default=4000, help="c4, c5s: Adam steps of the surrogate")
+    ap.add_argument("--train-configs", type=int, default=C4_NCONF, help="c4: training snapshots of the N=32768 chain")
     ap.add_argument("--finetune-steps", type=int, default=1000, help="c5s: Adam steps of the fine-tuning")
     ap.add_argument("--cell-md", action="store_true",
                     help="c2 through the 27-cell sweep with a re-binning per evaluation (CHMC_CELL_MD)")
@@ -472,6 +490,7 @@ def main():
     L = (N / RHO) ** (1 / 3)
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T), cell_md=args.cell_md)
     e.init_lattice(1, 0.05)
+    e.set_list_check(args.list_check)
     if args.cell_md:
         cell_md_run(args, e, chmc, stats)
         return
@@ -544,7 +563,7 @@ def main():
         "dtype": "f32 pair forces, f64 x/p/dH", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": "plain HMC N=32768 rho=0.8442 T=0.722 rc=2.5 OMF2(lambda=0.1932) 10x0.01",
                    "N": N, "trajectories": args.steps, "md_steps_per_traj": NSTEPS, "force_evals_per_traj": 2 * NSTEPS + 1,
-                   "thermalisation": THERM, "md_path": "neighbour lists", "skin": skin, "list_lanes": lanes,
+                   "thermalisation": THERM, "md_path": "neighbour lists", "skin": skin, "list_lanes": lanes, "list_check": e.list_check(),
                    "ms_per_traj_by_skin": skins},
         "acceptance": float(nacc[0]) / args.steps,
         "ess_per_s": ess_info["ess_per_s"], "ess": ess_info,

@@ -112,6 +112,8 @@ def _declare(L):
         "chmc_list_stats": ([P, _I, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], _I),
         "chmc_time_list": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
         "chmc_set_list_lanes": ([P, _I], _I),
+        "chmc_set_list_check": ([P, _I], _I),
+        "chmc_get_list_check": ([P, C.POINTER(_I)], _I),
         "chmc_get_list_lanes": ([P, C.POINTER(_I)], _I),
         "chmc_set_trajectory": ([P, _I], _I),
         "chmc_get_trajectory": ([P, C.POINTER(_I)], _I),
@@ -124,6 +126,12 @@ def _declare(L):
         "chmc_surrogate_training_data": ([P, C.c_longlong, _I, C.c_void_p, C.c_void_p], _I),
         "chmc_surrogate_loss_grad": ([P, C.c_longlong, _I, C.c_void_p], _I),
         "chmc_surrogate_train": ([P, _I, _I, C.c_float, _U64, C.c_void_p], _I),
+        "chmc_surrogate_get_adam": ([P, C.c_void_p, C.c_void_p, C.POINTER(_I)], _I),
+        "chmc_surrogate_set_adam": ([P, C.c_void_p, C.c_void_p, _I], _I),
+        "chmc_surrogate_fm_loss_grad": ([P, _D, C.c_void_p, C.POINTER(_D), C.POINTER(_D)], _I),
+        "chmc_surrogate_reserve_configs": ([P, _I], _I),
+        "chmc_surrogate_collect_config": ([P, C.POINTER(_I)], _I),
+        "chmc_surrogate_train_fm": ([P, _I, _D, C.c_float, _U64, C.c_void_p, C.c_void_p], _I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -141,10 +149,13 @@ EXPORTED = (
     "chmc_set_force_tiling",
     "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_set_chain_skins",
     "chmc_get_chain_skin", "chmc_list_stats",
-    "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_trajectory", "chmc_get_trajectory",
+    "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_list_check", "chmc_get_list_check", "chmc_set_trajectory", "chmc_get_trajectory",
     "chmc_surrogate_set_params", "chmc_surrogate_get_params",
     "chmc_surrogate_eval", "chmc_surrogate_features", "chmc_surrogate_reserve", "chmc_surrogate_collect",
     "chmc_surrogate_training_data", "chmc_surrogate_loss_grad", "chmc_surrogate_train",
+    "chmc_surrogate_get_adam", "chmc_surrogate_set_adam",
+    "chmc_surrogate_fm_loss_grad", "chmc_surrogate_reserve_configs", "chmc_surrogate_collect_config",
+    "chmc_surrogate_train_fm",
 )
 
 SURR_NPARAMS, SURR_NGRAD, SURR_NFEAT, SURR_NHID = 6401, 6338, 32, 64
@@ -407,6 +418,45 @@ class Engine:
                  "surrogate_train")
         return h
 
+    def surrogate_adam_state(self):
+        """(m, v, step) of the Adam optimiser: with surrogate_params() a training checkpoint."""
+        m = np.zeros(SURR_NGRAD - 1, np.float32)
+        v = np.zeros(SURR_NGRAD - 1, np.float32)
+        st = _I()
+        self._ck(lib().chmc_surrogate_get_adam(self._h, m.ctypes.data, v.ctypes.data, C.byref(st)), "surrogate_get_adam")
+        return m, v, st.value
+
+    def surrogate_set_adam_state(self, m, v, step):
+        m = np.ascontiguousarray(m, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        self._ck(lib().chmc_surrogate_set_adam(self._h, m.ctypes.data, v.ctypes.data, int(step)), "surrogate_set_adam")
+
+    def surrogate_fm_loss_grad(self, w_force):
+        """Energy + force-matching loss gradient at the current positions (no update): (gradient of the
+        trained parameters, mean energy loss, mean force loss)."""
+        g = np.zeros(SURR_NGRAD - 1, np.float32)
+        le, lf = _D(), _D()
+        self._ck(lib().chmc_surrogate_fm_loss_grad(self._h, float(w_force), g.ctypes.data, C.byref(le), C.byref(lf)),
+                 "surrogate_fm_loss_grad")
+        return g, le.value, lf.value
+
+    def surrogate_reserve_configs(self, nconfigs):
+        self._ck(lib().chmc_surrogate_reserve_configs(self._h, int(nconfigs)), "surrogate_reserve_configs")
+
+    def surrogate_collect_config(self):
+        n = _I()
+        self._ck(lib().chmc_surrogate_collect_config(self._h, C.byref(n)), "surrogate_collect_config")
+        return n.value
+
+    def surrogate_train_fm(self, nsteps, w_force, lr=1e-3, seed=1):
+        """nsteps Adam steps of energy + force matching, one stored configuration each; returns the mean
+        energy and force losses before every step."""
+        le = np.zeros(nsteps, np.float32)
+        lf = np.zeros(nsteps, np.float32)
+        self._ck(lib().chmc_surrogate_train_fm(self._h, int(nsteps), float(w_force), float(lr), int(seed), le.ctypes.data,
+                                               lf.ctypes.data), "surrogate_train_fm")
+        return le, lf
+
     # -- tuning --------------------------------------------------------------------------------
     def set_force_tiling(self, islots):
         self._ck(lib().chmc_set_force_tiling(self._h, int(islots)), "set_force_tiling")
@@ -446,6 +496,15 @@ class Engine:
     def list_lanes(self):
         v = _I()
         self._ck(lib().chmc_get_list_lanes(self._h, C.byref(v)), "get_list_lanes")
+        return v.value
+
+    def set_list_check(self, every):
+        """Rebuild checks of list MD before every `every`-th force evaluation (1 to 4)."""
+        self._ck(lib().chmc_set_list_check(self._h, int(every)), "set_list_check")
+
+    def list_check(self):
+        v = _I()
+        self._ck(lib().chmc_get_list_check(self._h, C.byref(v)), "get_list_check")
         return v.value
 
     def time_list(self, reps=20):

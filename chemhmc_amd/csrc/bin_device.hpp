@@ -9,44 +9,72 @@ namespace chmc {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Exclusive scan of the per-cell counts into the next offsets (and the counts cleared), by one
+// workgroup of 1024 threads: rounds of 8192 counts are loaded into LDS with coalesced reads, each thread
+// scans 8 contiguous ones, a block scan over the threads and a running carry join the rounds, and the
+// offsets go back with coalesced writes.
 __device__ inline void scan_counts_block(const DevView& v)
 {
+    constexpr int T = 1024, PT = 8;
+    __shared__ int buf[T * PT];
     __shared__ int wsum[16];
-    const int nt = blockDim.x, tid = threadIdx.x;
-    const int Ctot = v.Ctot;
+    __shared__ int carry;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int n = v.Ctot;
     int* count = v.count;
     int* offs = offs_next(v);
-    const int per = (Ctot + nt - 1) / nt;
-    const int beg = min(tid * per, Ctot), end = min(beg + per, Ctot);
-    int s = 0;
-    for (int c = beg; c < end; c++) s += count[c];
-    const int lane = tid & 63, wid = tid >> 6, nw = nt >> 6;
-    int incl = s;
+    if (tid == 0) carry = 0;
+    for (int r0 = 0; r0 < n; r0 += T * PT) {
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    if (wid == 0) {
-        int w = lane < nw ? wsum[lane] : 0;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            int t = __shfl_up(w, o, 64);
-            if (lane >= o) w += t;
+        for (int k = 0; k < PT; k++) {
+            const int c = r0 + k * T + tid;
+            buf[k * T + tid] = c < n ? count[c] : 0;
         }
-        if (lane < nw) wsum[lane] = w;
+        __syncthreads();
+        int vals[PT];
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            vals[k] = buf[tid * PT + k];
+            s += vals[k];
+        }
+        int incl = s;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        if (wid == 0) {
+            int w = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                const int t = __shfl_up(w, o, 64);
+                if (lane >= o) w += t;
+            }
+            if (lane < 16) wsum[lane] = w;
+        }
+        __syncthreads();
+        int run = carry + incl - s + (wid > 0 ? wsum[wid - 1] : 0);
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            buf[tid * PT + k] = run;
+            run += vals[k];
+        }
+        __syncthreads();                      // every thread has read carry; buf holds the round's scan
+        if (tid == T - 1) carry = run;
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int c = r0 + k * T + tid;
+            if (c < n) {
+                offs[c] = buf[k * T + tid];
+                count[c] = 0;
+            }
+        }
+        __syncthreads();                      // buf free, carry final
     }
-    __syncthreads();
-    int run = incl - s + (wid > 0 ? wsum[wid - 1] : 0);
-    for (int c = beg; c < end; c++) {
-        const int n = count[c];
-        offs[c] = run;
-        run += n;
-        count[c] = 0;
-    }
-    if (tid == nt - 1) offs[Ctot] = wsum[nw - 1];
+    if (tid == 0) offs[n] = carry;
 }
 
 // cell of a position, and whether it lies outside the 27-cell neighbourhood of cell `old` (periodic)
@@ -85,7 +113,25 @@ __device__ __forceinline__ void count_atom(const DevView& v, const ChainDev& C, 
     v.newcell[prev_slot] = cell;
     if (far_from(C, b, prev_cell)) atomicOr(&v.bflag[v.par ^ 1], 1);
     v.cell_of[a] = cell;
-    v.slot[a] = atomicAdd(&v.count[cell], 1);
+    // one atomic per distinct cell among the wave's active lanes (lanes that run in slot order mostly
+    // share a few cells); after kGroups groups the remaining lanes take an atomic each.  The arrival slot
+    // only has to be unique within the cell: the general placement ranks the cell's entries by key.
+    constexpr int kGroups = 8;
+    const int lane = threadIdx.x & 63;
+    unsigned long long left = __ballot(1);
+    int slot = -1;
+    for (int g = 0; g < kGroups && left; g++) {
+        const int l = __ffsll((long long)left) - 1;
+        const int cl = __shfl(cell, l, 64);
+        const unsigned long long m = __ballot(cell == cl) & left;
+        int base = 0;
+        if (lane == l) base = atomicAdd(&v.count[cell], __popcll(m));
+        base = __shfl(base, l, 64);
+        if (cell == cl) slot = base + __popcll(m & ((1ull << lane) - 1ull));
+        left &= ~m;
+    }
+    if (slot < 0) slot = atomicAdd(&v.count[cell], 1);
+    v.slot[a] = slot;
 }
 
 }  // namespace chmc

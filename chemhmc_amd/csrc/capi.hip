@@ -56,10 +56,16 @@ struct chmc_ctx {
     float* sloss = nullptr;
     int adam_step = 0;
     bool md_surr = false;        // MD kicks use the surrogate's forces (set for the trajectories of an HMC run)
+    bool surr_lists_eval = false; // the last surrogate evaluation ran on the lists (sG, sg in list slot order)
+    // force matching: per-slot residual forces, LJ energies and dL/dg; stored configurations
+    float *fmR = nullptr, *fmu = nullptr, *fmv = nullptr, *fmgrad = nullptr;
+    double* xtrain = nullptr;
+    int xtrain_cap = 0, xtrain_n = 0;
     // neighbour-list MD (allocated on first use)
     double skin = 0.6;           // the skin set for every chain, or the largest per-chain skin
     std::vector<double> chain_skin;  // per-chain skins when set individually (empty: skin for all)
     bool list_ready = false;
+    int list_check = 1;          // rebuild checks before every list_check-th force evaluation of list MD
     int lpar = 0;                // list parity: which FP32 coordinate buffer / rebuild flag is current
     int cpar = 0;                // list count buffer of the next rebuild launch (alternates per launch)
     bool list_valid = false;     // the lists and their rebuild flags account for every position change
@@ -127,6 +133,7 @@ static void destroy_ctx(chmc_ctx* c)
     if (c->Gtrain) hipFree(c->Gtrain);
     if (c->utrain) hipFree(c->utrain);
     if (c->spartial) hipFree(c->spartial);
+    if (c->xtrain) hipFree(c->xtrain);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->ev2) hipEventDestroy(c->ev2);
@@ -705,6 +712,7 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
         }
         const double half = std::max(0.0, 0.5 * sk - 4.0e-7 * lmax);
         cd.half_skin2 = half * half;
+        cd.p2max = 100.0 * cd.mass / cd.beta;      // |p| <= 10 sqrt(m / beta): 10 thermal standard deviations
         cd.rl2_f = (float)(rl * rl * (1.0 + 1e-5));   // a margin over the FP32 rounding of r^2 at the rebuild
         // capacities from a local density bound of max(1.5 / sigma^3, 1.5 x the chain's mean density)
         const double rho = cd.natoms / (cd.L[0] * cd.L[1] * cd.L[2]) * cd.sigma * cd.sigma * cd.sigma;
@@ -861,13 +869,33 @@ static int list_invalidate(chmc_ctx* c)
     return CHMC_OK;
 }
 
-// rel64 (without fuse_drift and storef): the sweep also writes the FP64 positions the energy reads
-static int list_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift, double drift_dt, bool rel64 = false)
+// rel64 (without fuse_drift and storef): the sweep also writes the FP64 positions the energy reads.
+// checked = false skips the rebuild check (the caller's check interval guarantees valid lists)
+static int list_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift, double drift_dt, bool rel64 = false,
+                           bool checked = true)
 {
-    HIPCHK(c, list_rebuild(c));
+    if (checked) HIPCHK(c, list_rebuild(c));
     const bool r64 = rel64 && !fuse_drift && !storef;
     const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0) | (fuse_drift ? LJ_DRIFT : 0) | (r64 ? LJ_REL64 : 0);
-    HIPCHK(c, launch_list_sweep(c->v, mode, c->lpar, dt, fuse_drift ? drift_dt : 0.0, c->stream));
+    HIPCHK(c, launch_list_sweep(c->v, mode, c->lpar, dt, fuse_drift ? drift_dt : 0.0, checked, c->stream));
+    if (fuse_drift) c->lpar ^= 1;
+    c->list_fresh = !fuse_drift;
+    c->rel64_fresh = r64;
+    c->force_kind = (storef && !fuse_drift) ? 1 : 0;
+    c->bin_state = 2;
+    return CHMC_OK;
+}
+
+// the same update with the learned surrogate's forces: descriptors and MLP of the current list state,
+// then the surrogate sweep with the list sweep's kick / drift / rebuild-request tail
+static int list_surr_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift, double drift_dt, bool rel64 = false,
+                                bool checked = true)
+{
+    if (checked) HIPCHK(c, list_rebuild(c));
+    const bool r64 = rel64 && !fuse_drift && !storef;
+    const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0) | (fuse_drift ? LJ_DRIFT : 0) | (r64 ? LJ_REL64 : 0);
+    HIPCHK(c, launch_lsurr_desc_mlp(c->v, c->sv, c->lpar, c->sg, nullptr, nullptr, c->stream));
+    HIPCHK(c, launch_lsurr_sweep(c->v, c->sg, mode, c->lpar, dt, fuse_drift ? drift_dt : 0.0, checked, c->stream));
     if (fuse_drift) c->lpar ^= 1;
     c->list_fresh = !fuse_drift;
     c->rel64_fresh = r64;
@@ -902,7 +930,10 @@ static int list_check_overflow(chmc_ctx* c)
     int f[4] = { 0, 0, 0, 0 };
     HIPCHK(c, hipMemcpyAsync(f, c->v.lflag, sizeof(f), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (f[2]) return fail(c, CHMC_E_STATE, "neighbour list capacity exceeded (cell members or list entries); results invalid");
+    if (f[2] & 3) return fail(c, CHMC_E_STATE, "neighbour list capacity exceeded (cell members or list entries); results invalid");
+    if (f[2] & 4)
+        return fail(c, CHMC_E_STATE, "an atom's momentum exceeded 10 sqrt(m / beta), the bound of the list check interval: "
+                                     "results invalid (chmc_set_list_check(ctx, 1) checks before every force evaluation)");
     return CHMC_OK;
 }
 
@@ -984,25 +1015,40 @@ static int integrate_seq(chmc_ctx* c, int scheme, double dt, int steps, double l
     if (lp_scheme(scheme)) return integrate_lp(c, scheme, dt, steps, lambda, storef, rel64_last);
     std::vector<MdOp> ops;
     if (build_ops(scheme, dt, steps, lambda, ops) != 0) return fail(c, CHMC_E_ARG, "unknown integrator scheme");
-    if (list_md(c) && !c->md_surr) {
-        // neighbour-list MD on the slot-ordered state
+    if (list_md(c)) {
+        // neighbour-list MD on the slot-ordered state (LJ or, in a surrogate-driven trajectory, surrogate forces)
         int rc = flush_drift(c);
         if (rc) return rc;
         if (!c->lstate) return fail(c, CHMC_E_STATE, "internal: list MD without the slot-ordered state");
         if (!c->list_valid && (rc = list_invalidate(c))) return rc;
         c->list_valid = true;
-        for (size_t k = 0; k < ops.size(); k++) {
+        // rebuild checks before every list_check-th force evaluation, and before the last one and every
+        // evaluation without a following drift; the trigger in between is lowered by what the skipped
+        // drifts can add (lcheck_dt)
+        const int m = c->list_check;
+        double max_drift = 0.0;
+        int nevals = 0;
+        for (const MdOp& op : ops) {
+            if (!op.kick) max_drift = std::max(max_drift, std::fabs(op.dt));
+            else nevals++;
+        }
+        c->v.lcheck_dt = m > 1 ? (m - 1) * max_drift : 0.0;
+        int ev = 0;
+        for (size_t k = 0; k < ops.size() && !rc; k++) {
             if (ops[k].kick) {
                 const bool next_x = k + 1 < ops.size() && !ops[k + 1].kick;
-                rc = list_force_kick(c, ops[k].dt, storef, next_x, next_x ? ops[k + 1].dt : 0.0,
-                                     rel64_last && k + 1 == ops.size());
+                const bool checked = m == 1 || ev % m == 0 || !next_x || ev == nevals - 1;
+                rc = (c->md_surr ? list_surr_force_kick : list_force_kick)(c, ops[k].dt, storef, next_x,
+                                                                          next_x ? ops[k + 1].dt : 0.0,
+                                                                          rel64_last && k + 1 == ops.size(), checked);
+                ev++;
                 if (next_x) k++;
             } else {
                 rc = list_drift(c, ops[k].dt);
             }
-            if (rc) return rc;
         }
-        return CHMC_OK;
+        c->v.lcheck_dt = 0.0;
+        return rc;
     }
     c->list_valid = false;
     for (size_t k = 0; k < ops.size(); k++) {
@@ -1162,8 +1208,8 @@ struct Mt64 {
 static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps, int ufirst, int rec_base)
 {
     int rc;
-    if (list_md(c) && !c->md_surr) {
-        // heat bath + K_i; neighbour-list MD; FP64 V_f, W_f, K_f over the lists
+    if (list_md(c)) {
+        // heat bath + K_i; neighbour-list MD (LJ or surrogate forces); FP64 LJ V_f, W_f, K_f over the lists
         if (!c->list_valid) {
             // positions moved outside the lists (a surrogate trajectory): rebuild now and make the
             // rebuilt buffer the start buffer of this trajectory
@@ -1177,8 +1223,8 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
         if (steps == 0 && !c->list_valid && (rc = list_invalidate(c))) return rc;
         c->list_valid = true;
         if ((rc = list_energy(c, false))) return rc;
-        HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, 0, SUM_TILES,
-                                    c->stream));
+        HIPCHK(c, launch_metropolis(c->v, c->d_uniforms, hp.ntraj, hp.therm, ufirst, c->d_rec, rec_base, c->md_surr ? 1 : 0,
+                                    SUM_TILES, c->stream));
         HIPCHK(c, launch_save_restore_slots(c->v, 2, c->lpar, c->stream));
         c->list_fresh = c->rel64_fresh = false;       // a rejected chain returned to its start positions
         c->force_kind = 0;
@@ -1322,7 +1368,7 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     HIPCHK(c, hipMemcpyAsync(c->d_uniforms, u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice, c->stream));
 
     // ---- V_i, W_i of the start configuration (HMC.cpp:64), h_x = x (HMC.cpp:72) ----
-    const bool slot_run = list_md(c) && !hp->surrogate;      // the run starts on the slot-ordered state
+    const bool slot_run = list_md(c);                          // the run starts on the slot-ordered state
     if (slot_run) {
         if ((rc = list_import(c)) || (!c->list_valid && (rc = list_invalidate(c))) || (rc = list_energy(c, true))) return rc;
         c->list_valid = true;
@@ -1353,16 +1399,9 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     std::vector<HmcRecord> wrec;
     for (int t0 = 0; t0 < ntraj && !rc; t0 += window) {
         const int count = std::min(window, ntraj - t0);
-        if (list_md(c)) {
-            // surrogate MD runs on atom order (x, p, x_saved); list MD on the slot order, from a rebuild
-            // and a save of the positions the surrogate trajectories left
-            if (c->md_surr) {
-                rc = list_export(c, true);
-            } else if (!c->lstate) {
-                c->list_valid = false;
-                rc = list_import(c);
-            }
-            if (rc) break;
+        if (list_md(c) && !c->lstate) {          // LJ and surrogate MD both run on the slot-ordered state
+            c->list_valid = false;
+            if ((rc = list_import(c))) break;
         }
         rc = run_segment(c, *hp, t0, count, ufirst, rec_base, rsteps, graphs);
         if (rc || !c->md_surr) continue;
@@ -1605,11 +1644,28 @@ extern "C" int chmc_surrogate_eval(chmc_ctx* c, double* energy, int store_force)
     if (!c->surr_params) return fail(c, CHMC_E_STATE, "surrogate parameters not set");
     int rc = surr_geometry(c);
     if (rc) return rc;
-    rc = surr_descriptors(c, c->sG);
-    if (rc) return rc;
-    HIPCHK(c, launch_surr_mlp(c->v, c->sv, c->sG, c->sg, c->se, c->stream));
-    HIPCHK(c, launch_surr_force(c->v, c->sv, c->sg, 0.0, store_force != 0, c->stream));
-    if ((rc = surr_check_flag(c))) return rc;
+    if (list_md(c)) {
+        // the kernels of surrogate-driven MD: descriptors and MLP over the neighbour lists, then the
+        // surrogate sweep storing the force (no kick); the state returns to atom order afterwards
+        if ((rc = list_import(c))) return rc;
+        if (!c->list_valid && (rc = list_invalidate(c))) return rc;
+        c->list_valid = true;
+        HIPCHK(c, list_rebuild(c));
+        HIPCHK(c, launch_lsurr_desc_mlp(c->v, c->sv, c->lpar, c->sg, c->se, c->sG, c->stream));
+        if (store_force) HIPCHK(c, launch_lsurr_sweep(c->v, c->sg, LJ_FORCE | LJ_STOREF, c->lpar, 0.0, 0.0, true, c->stream));
+        c->list_fresh = store_force != 0;
+        if ((rc = list_export(c, false))) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if ((rc = list_check_overflow(c))) return rc;
+        c->surr_lists_eval = true;
+    } else {
+        rc = surr_descriptors(c, c->sG);
+        if (rc) return rc;
+        HIPCHK(c, launch_surr_mlp(c->v, c->sv, c->sG, c->sg, c->se, c->stream));
+        HIPCHK(c, launch_surr_force(c->v, c->sv, c->sg, 0.0, store_force != 0, c->stream));
+        if ((rc = surr_check_flag(c))) return rc;
+        c->surr_lists_eval = false;
+    }
     if (store_force) c->force_kind = 1;
     if (energy) {
         std::vector<float> e(c->Ntot);
@@ -1624,6 +1680,29 @@ extern "C" int chmc_surrogate_eval(chmc_ctx* c, double* energy, int store_force)
     return CHMC_OK;
 }
 
+// Adam state of the trained parameters (first and second moments, step count): a checkpoint of a
+// training run is these with the parameters
+extern "C" int chmc_surrogate_get_adam(chmc_ctx* c, float* m, float* v, int* step)
+{
+    CHECK_CTX(c);
+    if (!c->surr_params) return fail(c, CHMC_E_STATE, "surrogate parameters not set");
+    if (m) HIPCHK(c, hipMemcpy(m, c->sv.adam_m, sizeof(float) * kSurrNTrain, hipMemcpyDeviceToHost));
+    if (v) HIPCHK(c, hipMemcpy(v, c->sv.adam_v, sizeof(float) * kSurrNTrain, hipMemcpyDeviceToHost));
+    if (step) *step = c->adam_step;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_surrogate_set_adam(chmc_ctx* c, const float* m, const float* v, int step)
+{
+    CHECK_CTX(c);
+    if (!m || !v || step < 0) return fail(c, CHMC_E_ARG, "surrogate_set_adam: m, v required, step >= 0");
+    if (!c->surr_params) return fail(c, CHMC_E_STATE, "surrogate parameters not set");
+    HIPCHK(c, hipMemcpy(c->sv.adam_m, m, sizeof(float) * kSurrNTrain, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->sv.adam_v, v, sizeof(float) * kSurrNTrain, hipMemcpyHostToDevice));
+    c->adam_step = step;
+    return CHMC_OK;
+}
+
 // descriptors G and input gradients g = dE/dG of the last chmc_surrogate_eval, in atom order
 extern "C" int chmc_surrogate_features(chmc_ctx* c, int chain, float* G, float* g)
 {
@@ -1631,8 +1710,15 @@ extern "C" int chmc_surrogate_features(chmc_ctx* c, int chain, float* G, float* 
     if (!c->surr_alloc) return fail(c, CHMC_E_STATE, "surrogate not evaluated");
     const ChainDev& cd = c->hchains[chain];
     std::vector<int> perm(cd.natoms);
-    HIPCHK(c, hipMemcpy(perm.data(), c->v.perm_buf + (size_t)c->v.par * c->Ntot + cd.atom_off, sizeof(int) * cd.natoms,
-                        hipMemcpyDeviceToHost));
+    if (c->surr_lists_eval) {       // slot order of the chain's current list buffer
+        int cur = 0;
+        HIPCHK(c, hipMemcpy(&cur, c->v.lbuf + chain, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(perm.data(), c->v.lperm + (size_t)cur * c->Ntot + cd.atom_off, sizeof(int) * cd.natoms,
+                            hipMemcpyDeviceToHost));
+    } else {                        // order of the current binning
+        HIPCHK(c, hipMemcpy(perm.data(), c->v.perm_buf + (size_t)c->v.par * c->Ntot + cd.atom_off, sizeof(int) * cd.natoms,
+                            hipMemcpyDeviceToHost));
+    }
     std::vector<float> hG((size_t)cd.natoms * kSurrFeat), hg((size_t)cd.natoms * kSurrFeat);
     HIPCHK(c, hipMemcpy(hG.data(), c->sG + (size_t)cd.atom_off * kSurrFeat, sizeof(float) * hG.size(), hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(hg.data(), c->sg + (size_t)cd.atom_off * kSurrFeat, sizeof(float) * hg.size(), hipMemcpyDeviceToHost));
@@ -1737,6 +1823,121 @@ extern "C" int chmc_surrogate_train(chmc_ctx* c, int nsteps, int batch_rows, flo
         for (int t = 0; t < nsteps; t++) loss_history[t] = h[t] / (float)batch_rows;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHMC_OK;
+}
+
+// ---- force matching on whole configurations -----------------------------------------------------
+static int surr_fm_ensure(chmc_ctx* c)
+{
+    const size_t N = (size_t)c->Ntot;
+    if (!c->fmR) {
+        HIPCHK(c, dalloc(c, &c->fmR, 4 * N));
+        HIPCHK(c, dalloc(c, &c->fmu, N));
+        HIPCHK(c, dalloc(c, &c->fmv, N * kSurrFeat));
+        HIPCHK(c, dalloc(c, &c->fmgrad, kSurrNTrain + 2));
+    }
+    const long long need = (long long)surr_fm_nparts(c->Ntot) * (kSurrNTrain + 2);
+    if (need > c->spartial_cap) {
+        if (c->spartial) hipFree(c->spartial);
+        c->spartial = nullptr;
+        HIPCHK(c, hipMalloc(&c->spartial, sizeof(float) * (size_t)need));
+        c->spartial_cap = need;
+    }
+    return CHMC_OK;
+}
+
+// descriptors, g = dE/dG and the force-matching pass at the current positions; grad_out != NULL: the
+// summed gradient and loss sums without an update, else one Adam step with the loss sums in loss_out
+static int surr_fm_pass(chmc_ctx* c, double w_force, float* grad_out, float lr, float* loss_out)
+{
+    int rc = surr_descriptors(c, c->sG);
+    if (rc) return rc;
+    HIPCHK(c, launch_surr_mlp(c->v, c->sv, c->sG, c->sg, nullptr, c->stream));
+    if (!grad_out) c->adam_step++;
+    HIPCHK(c, launch_surr_fm(c->v, c->sv, c->sG, c->sg, reinterpret_cast<float4*>(c->fmR), c->fmu, c->fmv,
+                             (float)w_force, c->spartial, grad_out, lr, c->adam_step, loss_out, c->stream));
+    return CHMC_OK;
+}
+
+extern "C" int chmc_surrogate_fm_loss_grad(chmc_ctx* c, double w_force, float* grad, double* loss_e, double* loss_f)
+{
+    CHECK_CTX(c);
+    if (!grad || w_force < 0.0) return fail(c, CHMC_E_ARG, "surrogate_fm_loss_grad: grad required, w_force >= 0");
+    if (!c->surr_params) return fail(c, CHMC_E_STATE, "surrogate parameters not set");
+    int rc = surr_geometry(c);
+    if (!rc) rc = surr_fm_ensure(c);
+    if (!rc) rc = surr_fm_pass(c, w_force, c->fmgrad, 0.f, nullptr);
+    if (rc) return rc;
+    if ((rc = surr_check_flag(c))) return rc;
+    std::vector<float> h(kSurrNTrain + 2);
+    HIPCHK(c, hipMemcpy(h.data(), c->fmgrad, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+    std::copy(h.begin(), h.begin() + kSurrNTrain, grad);
+    if (loss_e) *loss_e = (double)h[kSurrNTrain] / c->Ntot;
+    if (loss_f) *loss_f = (double)h[kSurrNTrain + 1] / (3.0 * c->Ntot);
+    return CHMC_OK;
+}
+
+extern "C" int chmc_surrogate_reserve_configs(chmc_ctx* c, int nconfigs)
+{
+    CHECK_CTX(c);
+    if (nconfigs < 1) return fail(c, CHMC_E_ARG, "surrogate_reserve_configs: nconfigs >= 1");
+    int rc = surr_ensure(c);
+    if (rc) return rc;
+    if (c->xtrain) hipFree(c->xtrain);
+    c->xtrain = nullptr;
+    c->xtrain_cap = c->xtrain_n = 0;
+    HIPCHK(c, hipMalloc(&c->xtrain, sizeof(double) * 3 * (size_t)c->Ntot * (size_t)nconfigs));
+    c->xtrain_cap = nconfigs;
+    return CHMC_OK;
+}
+
+// appends the current positions of every chain (one snapshot of the context) to the configurations
+extern "C" int chmc_surrogate_collect_config(chmc_ctx* c, int* nconfigs)
+{
+    CHECK_CTX(c);
+    if (!c->xtrain) return fail(c, CHMC_E_STATE, "call chmc_surrogate_reserve_configs first");
+    if (c->xtrain_n >= c->xtrain_cap) return fail(c, CHMC_E_ARG, "configuration store full");
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    const size_t n3 = 3 * (size_t)c->Ntot;
+    HIPCHK(c, hipMemcpyAsync(c->xtrain + (size_t)c->xtrain_n * n3, c->v.x, sizeof(double) * n3, hipMemcpyDeviceToDevice,
+                             c->stream));
+    c->xtrain_n++;
+    if (nconfigs) *nconfigs = c->xtrain_n;
+    return CHMC_OK;
+}
+
+// nsteps Adam steps, each on one stored snapshot drawn from mt19937_64(seed): loss = mean (e - u)^2 +
+// w_force mean (dE/dx - dU/dx)^2 over its atoms.  The context's positions are left at the last snapshot.
+extern "C" int chmc_surrogate_train_fm(chmc_ctx* c, int nsteps, double w_force, float lr, uint64_t seed,
+                                       float* loss_e, float* loss_f)
+{
+    CHECK_CTX(c);
+    if (nsteps < 1 || nsteps > (1 << 15) || w_force < 0.0) return fail(c, CHMC_E_ARG, "surrogate_train_fm: 1 <= nsteps <= 32768, w_force >= 0");
+    if (!c->surr_params) return fail(c, CHMC_E_STATE, "surrogate parameters not set");
+    if (c->xtrain_n < 1) return fail(c, CHMC_E_STATE, "no stored configurations");
+    int rc = surr_geometry(c);
+    if (!rc) rc = surr_fm_ensure(c);
+    if (!rc) rc = flush_drift(c);
+    if (rc) return rc;
+    Mt64 rng(seed);
+    const size_t n3 = 3 * (size_t)c->Ntot;
+    for (int t = 0; t < nsteps; t++) {
+        const int k = (int)(rng.next() % (uint64_t)c->xtrain_n);
+        HIPCHK(c, hipMemcpyAsync(c->v.x, c->xtrain + (size_t)k * n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, c->stream));
+        c->list_valid = false;
+        c->bin_state = 2;
+        c->have_bins = false;
+        c->force_kind = 0;
+        if ((rc = surr_fm_pass(c, w_force, nullptr, lr, c->sloss + 2 * t))) return rc;
+    }
+    if ((rc = surr_check_flag(c))) return rc;
+    std::vector<float> h(2 * (size_t)nsteps);
+    HIPCHK(c, hipMemcpy(h.data(), c->sloss, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+    for (int t = 0; t < nsteps; t++) {
+        if (loss_e) loss_e[t] = h[2 * t] / (float)c->Ntot;
+        if (loss_f) loss_f[t] = h[2 * t + 1] / (3.0f * (float)c->Ntot);
+    }
     return CHMC_OK;
 }
 
@@ -1873,7 +2074,7 @@ extern "C" int chmc_list_stats(chmc_ctx* c, int chain, long long* entries, long 
     if (entries) *entries = tot;
     if (nint) *nint = (long long)std::llround(nintv[chain]);
     if (rebuilds) *rebuilds = f[3];
-    return f[2] ? fail(c, CHMC_E_STATE, "neighbour list capacity exceeded") : CHMC_OK;
+    return f[2] ? list_check_overflow(c) : CHMC_OK;
 }
 
 extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* ms_rebuild);
@@ -1881,8 +2082,8 @@ extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* m
 extern "C" int chmc_set_list_lanes(chmc_ctx* c, int lanes)
 {
     CHECK_CTX(c);
-    if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16)
-        return fail(c, CHMC_E_ARG, "set_list_lanes: 0 (autotune), 4, 8 or 16");
+    if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 812 && lanes != 816)
+        return fail(c, CHMC_E_ARG, "set_list_lanes: 0 (autotune), 4, 8, 16, 812 or 816");
     int rc = list_ensure(c);
     if (rc) return rc;
     if (lanes) {
@@ -1891,13 +2092,28 @@ extern "C" int chmc_set_list_lanes(chmc_ctx* c, int lanes)
     }
     double best = 1e30;
     int pick = 8;
-    for (int lp : { 4, 8, 16 }) {
+    for (int lp : { 4, 8, 16, 812, 816 }) {
         c->v.lanes = lp;
         double ms = 0.0;
         if ((rc = chmc_time_list(c, 20, &ms, nullptr))) return rc;
         if (ms < best) { best = ms; pick = lp; }
     }
     c->v.lanes = pick;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_set_list_check(chmc_ctx* c, int every)
+{
+    CHECK_CTX(c);
+    if (every < 1 || every > 4) return fail(c, CHMC_E_ARG, "set_list_check: 1 to 4 force evaluations");
+    c->list_check = every;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_get_list_check(const chmc_ctx* c, int* every)
+{
+    if (!c || !every) return CHMC_E_ARG;
+    *every = c->list_check;
     return CHMC_OK;
 }
 
@@ -1928,7 +2144,7 @@ extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* m
     // the trajectory's MD step alone: force, kick and drift with dt = 0 (positions and momenta
     // unchanged, so the lists stay valid), `reps` sweeps back to back
     for (int r = 0; r < reps; r++) {
-        HIPCHK(c, launch_list_sweep(c->v, LJ_FORCE | LJ_KICK | LJ_DRIFT, c->lpar, 0.0, 0.0, c->stream));
+        HIPCHK(c, launch_list_sweep(c->v, LJ_FORCE | LJ_KICK | LJ_DRIFT, c->lpar, 0.0, 0.0, true, c->stream));
         c->lpar ^= 1;
     }
     HIPCHK(c, hipEventRecord(c->ev2, c->stream));

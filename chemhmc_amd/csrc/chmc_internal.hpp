@@ -73,6 +73,7 @@ struct ChainDev {
     float  sizebin_f[3], rc2_f, sigma2_f, eps48_f;
     double lsize[3], half_skin2;   // list cell side, (skin / 2)^2
     float  lsize_f[3], rl2_f;      // list radius^2 = (rc + skin)^2
+    double p2max;                  // |p|^2 bound of the lists' check interval: (10 sqrt(m / beta))^2
 };
 
 // per-chain reduction slots
@@ -98,7 +99,7 @@ constexpr int kTile = 32;          // slots per neighbour-list sweep workgroup (
 // leading entries of every list row that the sweep reads unconditionally: the build pads a row past its
 // count up to this span with entries that contribute nothing (the slot itself with the image code
 // kFarCode, which no neighbour has: the sweep's shift table places that image beyond any cut-off)
-constexpr int kListSpan = 112;
+constexpr int kListSpan = 128;
 constexpr int kFarCode = 63;
 constexpr float kFarShift = 1.0e18f;
 constexpr int kScanCells = 3072;   // list cells of a chain whose counts one build workgroup scans in LDS
@@ -121,7 +122,8 @@ struct DevView {
     ChainState* state;    // [nchains]
     int* traj;            // [1] device trajectory counter (1-based index of the last finished one)
     // neighbour-list MD
-    int Cl, ntiles, nbcap, lanes;   // lanes: list-sweep lanes per slot (4, 8, 16)
+    int Cl, ntiles, nbcap, lanes;   // lanes: list-sweep shape (4, 8, 16 lanes per slot; 812, 816: 8 lanes with
+                                    // first batches of 12 / 16 entries)
     const int* lcell_chain;
     const Chunk* tiles;
     int *lcount, *loffs, *lperm, *linv, *nb, *nbn, *lflag, *lreq, *lscan_tmp;
@@ -139,6 +141,8 @@ struct DevView {
     float4* xsl;
     double* tile_part;    // [ntiles * 4]: energy, virial, interacting pairs, kinetic energy
     double lp_scale;      // 10^n_prec inside an lp_leapfrog / lp_omf4 integration, 0 otherwise
+    double lcheck_dt;     // list MD with rebuild checks every m-th force evaluation: (m - 1) times the
+                          // largest drift step of the integration (0: a check before every evaluation)
 };
 
 // The updates of lp_leapfrog / lp_omf4 (code-carsten/integrator.hh:51-79, 156-197) round the updated
@@ -253,7 +257,10 @@ hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s);
 // a standalone drift (q -> q ^ 1) and the FP64 energy / virial / kinetic sums into tile_part
 hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s);
 hipError_t launch_list_pad_init(const DevView& v, hipStream_t s);
-hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, hipStream_t s);
+// checked: this evaluation's rebuild check ran (its parity's requests are consumed); otherwise a pending
+// request is carried to the next parity
+hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, bool checked,
+                             hipStream_t s);
 hipError_t launch_list_drift(const DevView& v, int q, double dt, hipStream_t s);
 // rel64: compute the FP64 relative positions first (false when the last sweep wrote them)
 hipError_t launch_list_energy(const DevView& v, int q, bool rel64, hipStream_t s);

@@ -29,6 +29,7 @@
 //   potential's r < rc test (identical_particles.cpp:432-435), entries in batches of 8 per lane,
 //   per-tile fixed-order partials.
 #include "chmc_internal.hpp"
+#include "surrogate.hpp"
 
 #pragma clang fp contract(off)
 
@@ -515,6 +516,19 @@ hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s)
 // ---- sweep ---------------------------------------------------------------------------------------
 // drift of one atom by dt (FP64, wrap into [0, L)), its FP32 list coordinates of the next parity and
 // the rebuild request when it is beyond skin / 2 from its rebuild position
+// Rebuild trigger: displacement^2 from the rebuild position beyond which a drift requests a rebuild.
+// With a check before every force evaluation it is (skin / 2)^2.  With checks every m-th evaluation the
+// trigger sits lower by the distance m - 1 drifts can add, lcheck_dt * (beta / m) * |p|max with
+// |p|max = sqrt(p2max), so the unchecked evaluations in between still find every pair within rc in
+// the lists; every drift verifies |p|^2 <= p2max and flags a violation (the run then reports an
+// error).  A trigger at or below zero requests a rebuild at every check.
+__device__ __forceinline__ double trigger2(const DevView& v, const ChainDev& C)
+{
+    if (v.lcheck_dt == 0.0) return C.half_skin2;
+    const double h = sqrt(C.half_skin2) - v.lcheck_dt * (C.beta / C.mass) * sqrt(C.p2max);
+    return h > 0.0 ? h * h : -1.0;
+}
+
 __device__ __forceinline__ void list_drift_slot(const DevView& v, const ListBuf& L, const ChainDev& C, int ch,
                                                 int slot, int q, double dt, const double pn[3])
 {
@@ -536,7 +550,67 @@ __device__ __forceinline__ void list_drift_slot(const DevView& v, const ListBuf&
         f3[d] = (float)r;
     }
     L.xsl[(size_t)(q ^ 1) * N + slot] = make_float4(f3[0], f3[1], f3[2], 0.f);
-    if (disp2 > C.half_skin2) request(v, q ^ 1, ch);
+    if (disp2 > trigger2(v, C)) request(v, q ^ 1, ch);
+    if (v.lcheck_dt != 0.0 && pn[0] * pn[0] + pn[1] * pn[1] + pn[2] * pn[2] > C.p2max) atomicOr(&v.lflag[2], 4);
+}
+
+// The per-atom tail of a list sweep, on lanes sub = 0, 1, 2 of the slot (one dimension each; fd is the
+// force component of the lane's dimension): store the force in atom order, kick p in FP64, optionally
+// drift x in FP64 (identical_particles.cpp:651-673), then lane 0 writes the FP32 coordinates of the other
+// parity and raises the chain's rebuild request when the atom is beyond the trigger distance of its
+// rebuild position.  Thread 0 clears the requests of this evaluation's parity when its rebuild check ran
+// (checked), and otherwise carries a pending one over to the next parity.
+template <int MODE, int LP>
+__device__ __forceinline__ void list_tail(const DevView& v, const ListBuf& L, const ChainDev& C, const Chunk& tl, int s,
+                                          bool val, int sub, int d, int atom, float fd, double p_d, double x_d,
+                                          float4 ref, double kick_dt, double drift_dt, int q, int checked)
+{
+    const int N = v.Ntot;
+    double e = 0.0, pn = 0.0;
+    float rel = 0.f;
+    if (val && sub < 3) {
+        if constexpr ((MODE & LJ_STOREF) != 0) v.f32[d * N + atom] = fd;
+        pn = p_d;
+        if constexpr ((MODE & LJ_KICK) != 0) {
+            const double kdt = kick_dt * C.beta;       // dt * coeff_p, coeff_p = beta (:642-644)
+            pn = lp_round(pn - kdt * (double)fd, v.lp_scale);
+            v.pl[d * N + s] = pn;
+        }
+        if constexpr ((MODE & LJ_DRIFT) != 0) {
+            const double coeff_x = C.beta / C.mass;    // compute_coeff_position, identical_particles.cpp:646-648
+            double xv = x_d;
+            xv += drift_dt * coeff_x * pn;
+            xv = lp_round(xv, v.lp_scale);
+            xv -= C.L[d] * floor(xv / C.L[d]);
+            v.xl[d * N + s] = xv;
+            const double r = list_rel(C, ref, d, xv);
+            e = r - (double)ref_coord(ref, d);
+            rel = (float)r;
+        }
+        if constexpr ((MODE & LJ_REL64) != 0) v.rel64[4 * (size_t)s + d] = list_rel(C, ref, d, x_d);
+    }
+    if constexpr ((MODE & LJ_DRIFT) != 0) {
+        const int base = threadIdx.x & ~(LP - 1) & 63;
+        const float ry = __shfl(rel, base + 1, 64), rz = __shfl(rel, base + 2, 64);
+        const double ey = __shfl(e, base + 1, 64), ez = __shfl(e, base + 2, 64);
+        const double py = __shfl(pn, base + 1, 64), pz = __shfl(pn, base + 2, 64);
+        if (val && sub == 0) {
+            L.xsl[(size_t)(q ^ 1) * N + s] = make_float4(rel, ry, rz, 0.f);
+            if (e * e + ey * ey + ez * ez > trigger2(v, C)) request(v, q ^ 1, tl.chain);
+            if (v.lcheck_dt != 0.0 && pn * pn + py * py + pz * pz > C.p2max) atomicOr(&v.lflag[2], 4);
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (tl.begin == C.atom_off) {
+            if (checked) {
+                *req(v, q, tl.chain) = 0;                 // this evaluation's rebuild is done
+            } else if (*req(v, q, tl.chain) != 0) {       // no check ran: the request waits for the next one
+                *req(v, q, tl.chain) = 0;
+                request(v, q ^ 1, tl.chain);
+            }
+        }
+        if (blockIdx.x == 0) *req_any(v, q) = 0;
+    }
 }
 
 // LP lanes per slot, kTile slots per workgroup.  Each lane first loads up to kBatch of its list entries
@@ -545,14 +619,13 @@ __device__ __forceinline__ void list_drift_slot(const DevView& v, const ListBuf&
 // workgroup's list buffer comes from tcur[blockIdx] and the image shifts are decoded from the entries in
 // registers, so nothing delays the first list load but the tile's own two scalar loads (no LDS table,
 // no barrier).
-// entries per lane in the first batch, by lanes per atom (measured at skins 0.5-0.7, N=32768: 14 at 8
-// lanes covers the typical 105 entries at skin 0.6 and beats 12 and 16)
-template <int LP> constexpr int batch_of() { return LP == 4 ? 16 : (LP == 8 ? 14 : 7); }
-
-template <int MODE, int LP>
-__global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q)
+// KB entries per lane in the first batch: KB * LP should cover most rows (at 8 lanes, 14 covers the
+// typical 105 entries at skin 0.6 and N = 32768; 12 the 94 of skin 0.5, 16 the 116 of skin 0.7); the
+// sweep shape (LP, KB) is chosen by the autotuner
+template <int MODE, int LP, int KB>
+__global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q, int checked)
 {
-    constexpr int kBatch = batch_of<LP>();
+    constexpr int kBatch = KB;
     static_assert(kBatch * LP <= kListSpan, "the first batch reads padded entries only");
     __shared__ float4 shift_tab[64];                     // image shift by 6-bit image code
     const int tile = xcd_block(blockIdx.x, v.ntiles);
@@ -631,43 +704,118 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         ay += __shfl_xor(ay, m, 64);
         az += __shfl_xor(az, m, 64);
     }
-    double e = 0.0;
-    float rel = 0.f;
-    if (val && sub < 3) {
-        const float fd = C.eps48_f * (d == 0 ? ax : (d == 1 ? ay : az));
-        if constexpr ((MODE & LJ_STOREF) != 0) v.f32[d * N + atom] = fd;
-        double pn = p_d;
-        if constexpr ((MODE & LJ_KICK) != 0) {
-            const double kdt = kick_dt * C.beta;       // dt * coeff_p, coeff_p = beta (:642-644)
-            pn = lp_round(pn - kdt * (double)fd, v.lp_scale);
-            v.pl[d * N + s] = pn;
+    list_tail<MODE, LP>(v, L, C, tl, s, val, sub, d, atom, C.eps48_f * (d == 0 ? ax : (d == 1 ? ay : az)), p_d, x_d, ref,
+                        kick_dt, drift_dt, q, checked);
+}
+
+// Surrogate forces on the lists (same tiles, lanes, entries and tail as k_lsweep): for every listed pair
+// within rc, dE/dx_i += [sum_k (g_k(i) + g_k(j)) phi'_k(r)] (x_i - x_j) / r, with g = dE/dG of both atoms
+// (slot order, from k_lsurr_desc_mlp), phi_k(r) = exp(-(r - mu_k)^2 / 2w^2) fc(r), fc the cosine cutoff.
+// Two passes, so that no lane idles through the 32-feature work of a pair beyond rc: the LP lanes of a
+// slot first test its list entries (stage_pairs: the first 16 per lane loaded together) and append the
+// pairs within rc to an LDS list, then walk that list densely, each lane holding g(i) and gathering g(j)
+// per pair.  The
+// Gaussians of a pair come from two exponentials per block of 8 centres: with t = r - mu_k,
+// exp(-(t - D)^2 c) = exp(-t^2 c) rho, rho = exp(2 c D t - c D^2), and rho advances by exp(-2 c D^2).
+constexpr int kSurrPairCap = 128;   // pairs within rc per atom staged in LDS (local density <= 1.9 / sigma^3)
+
+template <int MODE, int LP>
+__global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const float* __restrict__ g, double kick_dt,
+                                                            double drift_dt, int q, int checked)
+{
+    __shared__ float4 spd[kTile][kSurrPairCap];          // (dx, dy, dz, r) of the slot's pairs within rc
+    __shared__ int spj[kTile][kSurrPairCap];             // their neighbour slots
+    const int tile = xcd_block(blockIdx.x, v.ntiles);
+    const Chunk tl = v.tiles[tile];
+    const ListBuf L = list_buf(v, v.tcur[tile]);
+    const ChainDev& C = v.chains[tl.chain];
+    const int N = v.Ntot;
+    const int t = threadIdx.x / LP, sub = threadIdx.x % LP;
+    const int s = tl.begin + t;
+    const bool val = s < tl.end;
+    const int d = sub < 3 ? sub : 0;
+    const float4* __restrict__ xq = L.xsl + (size_t)q * N;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    int atom = 0;
+    double p_d = 0.0, x_d = 0.0;
+    float4 ref = make_float4(0.f, 0.f, 0.f, 0.f);
+    int np = 0;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (val) {
+        if constexpr ((MODE & LJ_STOREF) != 0) atom = L.lperm[s];
+        if (sub < 3) {
+            p_d = v.pl[d * N + s];
+            if constexpr ((MODE & (LJ_DRIFT | LJ_REL64)) != 0) {
+                x_d = v.xl[d * N + s];
+                ref = L.lref[s];
+            }
         }
-        if constexpr ((MODE & LJ_DRIFT) != 0) {
-            const double coeff_x = C.beta / C.mass;    // compute_coeff_position, identical_particles.cpp:646-648
-            double xv = x_d;
-            xv += drift_dt * coeff_x * pn;
-            xv = lp_round(xv, v.lp_scale);
-            xv -= C.L[d] * floor(xv / C.L[d]);
-            v.xl[d * N + s] = xv;
-            const double r = list_rel(C, ref, d, xv);
-            e = r - (double)ref_coord(ref, d);
-            rel = (float)r;
-        }
-        if constexpr ((MODE & LJ_REL64) != 0) v.rel64[4 * (size_t)s + d] = list_rel(C, ref, d, x_d);
-    }
-    if constexpr ((MODE & LJ_DRIFT) != 0) {
-        const int base = threadIdx.x & ~(LP - 1) & 63;
-        const float ry = __shfl(rel, base + 1, 64), rz = __shfl(rel, base + 2, 64);
-        const double ey = __shfl(e, base + 1, 64), ez = __shfl(e, base + 2, 64);
-        if (val && sub == 0) {
-            L.xsl[(size_t)(q ^ 1) * N + s] = make_float4(rel, ry, rz, 0.f);
-            if (e * e + ey * ey + ez * ez > C.half_skin2) request(v, q ^ 1, tl.chain);
+        xi = xq[s];
+        const int n = L.nbn[s];
+        const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
+        // pass 1: the pairs within rc, in entry order
+        np = stage_pairs<LP, 16, true>(C, xq, nbs, n, xi, sub, &spd[t][0], &spj[t][0], nullptr, kSurrPairCap);
+        if (np > kSurrPairCap) {
+            if (sub == 0) atomicOr(&v.lflag[2], 8);
+            np = kSurrPairCap;
         }
     }
-    if (threadIdx.x == 0) {                               // this evaluation's rebuild is done
-        if (tl.begin == C.atom_off) *req(v, q, tl.chain) = 0;
-        if (blockIdx.x == 0) *req_any(v, q) = 0;
+    __syncthreads();
+    if (val) {
+        float gi[kSurrFeat];
+        const float4* gis = reinterpret_cast<const float4*>(g + (size_t)s * kSurrFeat);
+#pragma unroll
+        for (int b = 0; b < kSurrFeat / 4; b++) {
+            const float4 q4 = gis[b];
+            gi[4 * b] = q4.x; gi[4 * b + 1] = q4.y; gi[4 * b + 2] = q4.z; gi[4 * b + 3] = q4.w;
+        }
+        const float pirc = 3.14159265358979f / sqrtf(C.rc2_f);
+        const float c2 = 1.0f / (2.0f * kSurrWidth * kSurrWidth), iw2 = 1.0f / (kSurrWidth * kSurrWidth);
+        constexpr float kD = (kSurrMu1 - kSurrMu0) / (float)(kSurrFeat - 1);
+        const float cq = __expf(-2.0f * c2 * kD * kD);
+        // pass 2: the staged pairs, densely
+        for (int k = sub; k < np; k += LP) {
+            const float4 pd = spd[t][k];
+            const int j = spj[t][k];
+            const float r = pd.w;
+            const float fc = 0.5f * (__cosf(pirc * r) + 1.0f), dfc = -0.5f * pirc * __sinf(pirc * r);
+            const float4* gjs = reinterpret_cast<const float4*>(g + (size_t)j * kSurrFeat);
+            float gj[kSurrFeat];
+#pragma unroll
+            for (int b = 0; b < kSurrFeat / 4; b++) {
+                const float4 q4 = gjs[b];
+                gj[4 * b] = q4.x; gj[4 * b + 1] = q4.y; gj[4 * b + 2] = q4.z; gj[4 * b + 3] = q4.w;
+            }
+            float acc = 0.f;
+#pragma unroll
+            for (int b = 0; b < kSurrFeat / 8; b++) {
+                const float tb = r - mu_k(8 * b);
+                float ex = __expf(-tb * tb * c2);
+                float rho = __expf(c2 * kD * (2.0f * tb - kD));
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int f = 8 * b + u;
+                    const float tt = r - mu_k(f);
+                    const float dphi = ex * __builtin_fmaf(-tt * iw2, fc, dfc);
+                    acc = __builtin_fmaf(gi[f] + gj[f], dphi, acc);
+                    ex *= rho;
+                    rho *= cq;
+                }
+            }
+            const float coef = acc / r;
+            ax = __builtin_fmaf(coef, pd.x, ax);
+            ay = __builtin_fmaf(coef, pd.y, ay);
+            az = __builtin_fmaf(coef, pd.z, az);
+        }
     }
+#pragma unroll
+    for (int m = 1; m < LP; m <<= 1) {
+        ax += __shfl_xor(ax, m, 64);
+        ay += __shfl_xor(ay, m, 64);
+        az += __shfl_xor(az, m, 64);
+    }
+    list_tail<MODE, LP>(v, L, C, tl, s, val, sub, d, atom, d == 0 ? ax : (d == 1 ? ay : az), p_d, x_d, ref, kick_dt,
+                        drift_dt, q, checked);
 }
 
 __global__ __launch_bounds__(256) void k_ldrift(DevView v, double dt, int q)
@@ -850,31 +998,67 @@ hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s)
 }
 
 // ---- launchers -----------------------------------------------------------------------------------
+// sweep shape: 4, 8, 16 lanes per slot with first batches of 16, 14, 7 entries, or 812 / 816 (8 lanes with
+// batches of 12 / 16)
 template <int MODE>
-static hipError_t sweep_lp(const DevView& v, int lp, int q, double kick_dt, double drift_dt, hipStream_t s)
+static hipError_t sweep_lp(const DevView& v, int shape, int q, double kick_dt, double drift_dt, int ck, hipStream_t s)
 {
-    switch (lp) {
-    case 4: hipLaunchKernelGGL((k_lsweep<MODE, 4>), dim3(v.ntiles), dim3(kTile * 4), 0, s, v, kick_dt, drift_dt, q); break;
-    case 8: hipLaunchKernelGGL((k_lsweep<MODE, 8>), dim3(v.ntiles), dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q); break;
-    case 16: hipLaunchKernelGGL((k_lsweep<MODE, 16>), dim3(v.ntiles), dim3(kTile * 16), 0, s, v, kick_dt, drift_dt, q); break;
+    const dim3 g(v.ntiles);
+    switch (shape) {
+    case 4: hipLaunchKernelGGL((k_lsweep<MODE, 4, 16>), g, dim3(kTile * 4), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 8: hipLaunchKernelGGL((k_lsweep<MODE, 8, 14>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 16: hipLaunchKernelGGL((k_lsweep<MODE, 16, 7>), g, dim3(kTile * 16), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 812: hipLaunchKernelGGL((k_lsweep<MODE, 8, 12>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 816: hipLaunchKernelGGL((k_lsweep<MODE, 8, 16>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, hipStream_t s)
+hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, bool checked,
+                             hipStream_t s)
 {
     const int lp = v.lanes;
+    const int ck = checked ? 1 : 0;
     switch (mode) {
-    case LJ_FORCE | LJ_KICK: return sweep_lp<LJ_FORCE | LJ_KICK>(v, lp, q, kick_dt, drift_dt, s);
-    case LJ_FORCE | LJ_KICK | LJ_STOREF: return sweep_lp<LJ_FORCE | LJ_KICK | LJ_STOREF>(v, lp, q, kick_dt, drift_dt, s);
-    case LJ_FORCE | LJ_KICK | LJ_DRIFT: return sweep_lp<LJ_FORCE | LJ_KICK | LJ_DRIFT>(v, lp, q, kick_dt, drift_dt, s);
+    case LJ_FORCE | LJ_KICK: return sweep_lp<LJ_FORCE | LJ_KICK>(v, lp, q, kick_dt, drift_dt, ck, s);
+    case LJ_FORCE | LJ_KICK | LJ_STOREF: return sweep_lp<LJ_FORCE | LJ_KICK | LJ_STOREF>(v, lp, q, kick_dt, drift_dt, ck, s);
+    case LJ_FORCE | LJ_KICK | LJ_DRIFT: return sweep_lp<LJ_FORCE | LJ_KICK | LJ_DRIFT>(v, lp, q, kick_dt, drift_dt, ck, s);
     case LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT:
-        return sweep_lp<LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT>(v, lp, q, kick_dt, drift_dt, s);
-    case LJ_FORCE | LJ_STOREF: return sweep_lp<LJ_FORCE | LJ_STOREF>(v, lp, q, kick_dt, drift_dt, s);
-    case LJ_FORCE | LJ_KICK | LJ_REL64: return sweep_lp<LJ_FORCE | LJ_KICK | LJ_REL64>(v, lp, q, kick_dt, drift_dt, s);
+        return sweep_lp<LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT>(v, lp, q, kick_dt, drift_dt, ck, s);
+    case LJ_FORCE | LJ_STOREF: return sweep_lp<LJ_FORCE | LJ_STOREF>(v, lp, q, kick_dt, drift_dt, ck, s);
+    case LJ_FORCE | LJ_KICK | LJ_REL64: return sweep_lp<LJ_FORCE | LJ_KICK | LJ_REL64>(v, lp, q, kick_dt, drift_dt, ck, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+// the surrogate sweep runs 8 lanes per slot whatever the LJ sweep's tuning
+hipError_t launch_lsurr_sweep(const DevView& v, const float* g, int mode, int q, double kick_dt, double drift_dt,
+                              bool checked, hipStream_t s)
+{
+    const dim3 grid(v.ntiles), block(kTile * 8);
+    const int ck = checked ? 1 : 0;
+    switch (mode) {
+    case LJ_FORCE | LJ_KICK:
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK, 8>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck); break;
+    case LJ_FORCE | LJ_KICK | LJ_STOREF:
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_STOREF, 8>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck);
+        break;
+    case LJ_FORCE | LJ_KICK | LJ_DRIFT:
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_DRIFT, 8>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck);
+        break;
+    case LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT:
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT, 8>), grid, block, 0, s, v, g, kick_dt,
+                           drift_dt, q, ck);
+        break;
+    case LJ_FORCE | LJ_STOREF:
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_STOREF, 8>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck); break;
+    case LJ_FORCE | LJ_KICK | LJ_REL64:
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_REL64, 8>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_list_drift(const DevView& v, int q, double dt, hipStream_t s)

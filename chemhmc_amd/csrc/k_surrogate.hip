@@ -35,7 +35,6 @@ __device__ __forceinline__ float dsilu_f(float a)
     const float s = 1.0f / (1.0f + __expf(-a));
     return s * (1.0f + a * (1.0f - s));
 }
-__device__ __forceinline__ float mu_k(int k) { return kSurrMu0 + (kSurrMu1 - kSurrMu0) * (float)k / (float)(kSurrFeat - 1); }
 
 // --------------------------------------------------------------------------------------------------
 // pair list
@@ -541,6 +540,145 @@ __global__ __launch_bounds__(256) void k_surr_targets(DevView v, SurrView s, flo
     u_out[a] = (float)(2.0 * C.eps * acc);
 }
 
+// D layout -> bf16 A-layout transpose buffer of this wave, then the two K = 32 fragments of a 64-wide row
+__device__ __forceinline__ void act_store(MlpSmem& sm, const f32x4 x[4])
+{
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) sm.act[w][4 * (l >> 4) + i][16 * t + (l & 15)] = (__bf16)x[t][i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// x[16 rows][64] (act buffer) times a 64 x 64 bf16 matrix given by rows B[n][k] (B = W2: x W2^T; B = W2T: x W2)
+__device__ __forceinline__ void act_times(MlpSmem& sm, const __bf16* B, f32x4 out[4])
+{
+    const int w = threadIdx.x >> 6;
+    const bf16x8 a0 = frag_rows(&sm.act[w][0][0], kSurrHid, 0);
+    const bf16x8 a1 = frag_rows(&sm.act[w][0][0], kSurrHid, 32);
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, frag_rows(B + 16 * t * kSurrHid, kSurrHid, 0), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, frag_rows(B + 16 * t * kSurrHid, kSurrHid, 32), acc, 0, 0, 0);
+        out[t] = acc;
+    }
+}
+
+// --------------------------------------------------------------------------------------------------
+// surrogate on the neighbour lists: descriptors + MLP in one launch
+// --------------------------------------------------------------------------------------------------
+// 32 slots per workgroup, 8 lanes per slot.  Pass 1 (stage_pairs): the lanes test the slot's list
+// entries (the pairs within rc + skin of the last rebuild, the first 16 per lane loaded together) and
+// append the distances within rc to an LDS row in entry order; pass 2: they walk that row densely and
+// accumulate the 32 Gaussian features in registers (two exponentials per block of 8 centres, the
+// recurrence of the surrogate sweep).  The eight partial descriptors meet in a butterfly and go to LDS;
+// waves 0 and 1 then run the MLP forward pass and its input gradient on 16 slots each (as k_surr_mlp).
+// Slots follow the current list buffer of their chain; groups of 32 slots are mapped XCD-contiguously.
+constexpr int kDescPairCap = 128;   // pairs within rc per atom staged in LDS
+__global__ __launch_bounds__(256) void k_lsurr_desc_mlp(DevView v, SurrView s, int q, float* __restrict__ g_out,
+                                                        float* __restrict__ e_out, float* __restrict__ G_out)
+{
+    __shared__ MlpSmem sm;
+    __shared__ float sG[32][kSurrFeat];
+    __shared__ float srd[32][kDescPairCap];
+    load_weights(sm, s);
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ngroups = (v.Ntot + 31) / 32;
+    const int s0 = xcd_block(blockIdx.x, ngroups) * 32;               // first slot of this workgroup
+    const int a = threadIdx.x >> 3, sub = threadIdx.x & 7;             // slot within the workgroup
+    const int slot = s0 + a;
+    const int N = v.Ntot;
+    float G[kSurrFeat];
+#pragma unroll
+    for (int f = 0; f < kSurrFeat; f++) G[f] = 0.f;
+    if (slot < N) {
+        const int ch = v.atom_chain[slot];
+        const ChainDev& C = v.chains[ch];
+        const ListBuf L = list_buf(v, *list_cur(v, ch));
+        const float4* __restrict__ xq = L.xsl + (size_t)q * N;
+        const int n = L.nbn[slot];
+        int np = stage_pairs<8, 16, false>(C, xq, L.nb + (size_t)slot * v.nbcap, n, xq[slot], sub, nullptr, nullptr,
+                                           &srd[a][0], kDescPairCap);
+        if (np > kDescPairCap) {
+            if (sub == 0) atomicOr(s.flag, 1);
+            np = kDescPairCap;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const float pirc = 3.14159265358979f / sqrtf(C.rc2_f);
+        const float c2 = 1.0f / (2.0f * kSurrWidth * kSurrWidth);
+        constexpr float kD = (kSurrMu1 - kSurrMu0) / (float)(kSurrFeat - 1);
+        const float cq = __expf(-2.0f * c2 * kD * kD);
+        for (int k = sub; k < np; k += 8) {
+            const float r = srd[a][k];
+            const float fc = 0.5f * (__cosf(pirc * r) + 1.0f);
+#pragma unroll
+            for (int b = 0; b < kSurrFeat / 8; b++) {
+                const float tb = r - mu_k(8 * b);
+                float ex = __expf(-tb * tb * c2) * fc;
+                float rho = __expf(c2 * kD * (2.0f * tb - kD));
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    G[8 * b + u] += ex;
+                    ex *= rho;
+                    rho *= cq;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < kSurrFeat; f++) {
+        G[f] += __shfl_xor(G[f], 1, 64);
+        G[f] += __shfl_xor(G[f], 2, 64);
+        G[f] += __shfl_xor(G[f], 4, 64);
+    }
+    if (sub == 0) {
+#pragma unroll
+        for (int f = 0; f < kSurrFeat; f++) sG[a][f] = G[f];
+        if (G_out && slot < N) {
+#pragma unroll
+            for (int f = 0; f < kSurrFeat; f++) G_out[(size_t)slot * kSurrFeat + f] = G[f];
+        }
+    }
+    __syncthreads();                                   // weights and descriptors in LDS
+    const int m0 = s0 + 16 * w;                        // first slot of this wave's MLP rows
+    if (w >= 2 || m0 >= N) return;                     // wave-uniform; no block barrier follows
+    const int nval = min(16, N - m0);
+    WaveOut o;
+    mlp_forward_wave(sm, &sG[16 * w][0], 0, nval, o);
+    if (e_out && (l & 15) == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int m = 4 * (l >> 4) + i;
+            if (m < nval) e_out[m0 + m] = o.e[i];
+        }
+    }
+    const float one[4] = { 1.f, 1.f, 1.f, 1.f };
+    f32x4 d2[4], d1[4];
+    mlp_backward_wave(sm, o, one, d2, d1);
+    act_store(sm, d1);
+    const bf16x8 d1a0 = frag_rows(&sm.act[w][0][0], kSurrHid, 0);
+    const bf16x8 d1a1 = frag_rows(&sm.act[w][0][0], kSurrHid, 32);
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(d1a0, frag_rows(&sm.W1T[16 * t][0], kSurrHid, 0), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(d1a1, frag_rows(&sm.W1T[16 * t][0], kSurrHid, 32), acc, 0, 0, 0);
+        const int n = 16 * t + (l & 15);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int m = 4 * (l >> 4) + i;
+            if (m < nval) g_out[(size_t)(m0 + m) * kSurrFeat + n] = acc[i] * sm.scale[n];
+        }
+    }
+}
+
 // --------------------------------------------------------------------------------------------------
 // force matching: loss = mean_i (e_i - u_i)^2 + w_f mean_{i,d} (dE/dx_id - dU/dx_id)^2 over whole
 // configurations (all atoms of every chain at the current positions)
@@ -642,35 +780,6 @@ struct FmSmem {
     float de[kFmRows];
     float lsum[2][2];                                        // per wave: sum (e - u)^2, sum |R|^2
 };
-
-// D layout -> bf16 A-layout transpose buffer of this wave, then the two K = 32 fragments of a 64-wide row
-__device__ __forceinline__ void act_store(MlpSmem& sm, const f32x4 x[4])
-{
-    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) sm.act[w][4 * (l >> 4) + i][16 * t + (l & 15)] = (__bf16)x[t][i];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// x[16 rows][64] (act buffer) times a 64 x 64 bf16 matrix given by rows B[n][k] (B = W2: x W2^T; B = W2T: x W2)
-__device__ __forceinline__ void act_times(MlpSmem& sm, const __bf16* B, f32x4 out[4])
-{
-    const int w = threadIdx.x >> 6;
-    const bf16x8 a0 = frag_rows(&sm.act[w][0][0], kSurrHid, 0);
-    const bf16x8 a1 = frag_rows(&sm.act[w][0][0], kSurrHid, 32);
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, frag_rows(B + 16 * t * kSurrHid, kSurrHid, 0), acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, frag_rows(B + 16 * t * kSurrHid, kSurrHid, 32), acc, 0, 0, 0);
-        out[t] = acc;
-    }
-}
 
 // Gradient of the energy + force-matching loss over rows [0, nrows) (sorted slots of the current
 // configuration).  With d = dL/dg = v (from k_surr_fm_v) the force term is differentiated by a forward
@@ -964,6 +1073,12 @@ hipError_t launch_surr_fm(const DevView& v, const SurrView& s, const float* G, c
     else
         hipLaunchKernelGGL(k_surr_adam, dim3((ng + 255) / 256), dim3(256), 0, st, s, partial, nb, ng, lr, 0.9f, 0.999f,
                            1e-8f, step, loss_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_lsurr_desc_mlp(const DevView& v, const SurrView& s, int q, float* g, float* e, float* G, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lsurr_desc_mlp, dim3((v.Ntot + 31) / 32), dim3(256), 0, st, v, s, q, g, e, G);
     return hipGetLastError();
 }
 

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include "chmc_internal.hpp"
+
 namespace chmc {
 
 constexpr int kSurrFeat = 32, kSurrHid = 64, kSurrMaxNb = 128;
@@ -12,6 +14,9 @@ constexpr float kSurrMu0 = 0.8f, kSurrMu1 = 2.5f, kSurrWidth = 0.1f;
 constexpr int kOffW1 = 0, kOffB1 = 2048, kOffW2 = 2112, kOffB2 = 6208, kOffW3 = 6272, kOffB3 = 6336;
 constexpr int kSurrNTrain = 6337, kOffShift = 6337, kOffScale = 6369, kSurrNParams = 6401;
 constexpr int kSurrNGrad = kSurrNTrain + 1;     // + the loss slot of the training partials
+
+// centre of descriptor k: uniform on [kSurrMu0, kSurrMu1]
+__device__ __forceinline__ float mu_k(int k) { return kSurrMu0 + (kSurrMu1 - kSurrMu0) * (float)k / (float)(kSurrFeat - 1); }
 
 struct SurrView {
     float* params;          // [kSurrNParams] FP32 master copy
@@ -25,7 +30,56 @@ struct SurrView {
     int* flag;              // bit 0: a list overflowed kSurrMaxNb; bit 1: a j-list overflowed the LDS stage
 };
 
-struct DevView;
+// The pairs within rc of one slot, from its neighbour-list row (surrogate kernels): the LP lanes of the
+// slot read KB entries each without a count test (rows are padded past their count to kListSpan
+// entries), gather the neighbours' coordinates together, then test them; a row longer than KB * LP
+// entries continues in a loop.  The pairs within rc are appended in entry order to the slot's LDS row
+// by ballot prefix counts: with GEOM, pd = (dx, dy, dz, r) and pj = the neighbour slot, otherwise r
+// only (rd).  Every lane of the slot returns the same count (which may exceed cap: the caller flags it).
+template <int LP, int KB, bool GEOM>
+__device__ __forceinline__ int stage_pairs(const ChainDev& C, const float4* __restrict__ xq, const int* __restrict__ nbs,
+                                           int n, float4 xi, int sub, float4* pd, int* pj, float* rd, int cap)
+{
+    static_assert(KB * LP <= kListSpan, "the first batch reads padded entries only");
+    const int gbase = (threadIdx.x & 63) & ~(LP - 1);
+    const float rc2 = C.rc2_f;
+    int np = 0;
+    auto test = [&](int e, float4 q, bool live) {
+        const int code = e & 63;
+        const float dx = xi.x - (q.x + image_shift(code, 0, C.lsize_f[0]));
+        const float dy = xi.y - (q.y + image_shift(code, 1, C.lsize_f[1]));
+        const float dz = xi.z - (q.z + image_shift(code, 2, C.lsize_f[2]));
+        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const bool hit = live && r2 < rc2;
+        const unsigned grp = (unsigned)((__ballot(hit) >> gbase) & ((1ull << LP) - 1ull));
+        const int pos = np + __popc(grp & ((1u << sub) - 1u));
+        if (hit && pos < cap) {
+            if constexpr (GEOM) {
+                pd[pos] = make_float4(dx, dy, dz, sqrtf(r2));
+                pj[pos] = (int)((unsigned)e >> kImageBits);
+            } else {
+                rd[pos] = sqrtf(r2);
+            }
+        }
+        np += __popc(grp);
+    };
+    int ent[KB];
+    float4 q[KB];
+#pragma unroll
+    for (int u = 0; u < KB; u++) ent[u] = nbs[sub + u * LP];
+#pragma unroll
+    for (int u = 0; u < KB; u++) q[u] = xq[(unsigned)ent[u] >> kImageBits];
+#pragma unroll
+    for (int u = 0; u < KB; u++) test(ent[u], q[u], sub + u * LP < n);
+    for (int k0 = KB * LP; k0 < n; k0 += LP) {
+        const int k = k0 + sub;
+        const bool live = k < n;
+        const int e = nbs[live ? k : k0];
+        test(e, xq[(unsigned)e >> kImageBits], live);
+    }
+    return np;
+}
+
 hipError_t launch_surr_pairs(const DevView& v, const SurrView& s, hipStream_t st);
 hipError_t launch_surr_desc(const DevView& v, const SurrView& s, float* G, hipStream_t st);
 hipError_t launch_surr_mlp(const DevView& v, const SurrView& s, const float* G, float* g, float* e, hipStream_t st);
@@ -36,6 +90,13 @@ hipError_t launch_surr_train_step(const SurrView& s, const float* G, const float
 hipError_t launch_surr_grad_only(const SurrView& s, const float* G, const float* u, long long row0, int nrows,
                                  float* partial, float* grad_out, hipStream_t st);
 hipError_t launch_surr_refresh(const SurrView& s, hipStream_t st);
+// surrogate MD on the neighbour lists (slot-ordered state of the current list buffer, FP32 coordinates of
+// parity q): descriptors of the pairs within rc from each atom's list and the MLP with its input
+// gradient in one launch (g[slot][32], optional e[slot] and G[slot][32]); then the force from g and the lists with the
+// list sweep's kick / drift / rebuild-request tail (mode: LJ_* flags of launch_list_sweep)
+hipError_t launch_lsurr_desc_mlp(const DevView& v, const SurrView& s, int q, float* g, float* e, float* G, hipStream_t st);
+hipError_t launch_lsurr_sweep(const DevView& v, const float* g, int mode, int q, double kick_dt, double drift_dt,
+                              bool checked, hipStream_t st);
 int surr_fm_nparts(int nrows);
 hipError_t launch_surr_fm(const DevView& v, const SurrView& s, const float* G, const float* g, float4* R, float* u,
                           float* vbuf, float w_force, float* partial, float* grad_out, float lr, int step,

@@ -203,12 +203,19 @@ int chmc_potential_energy_open_noalloc(const double* xyz_dev, int n_particles, i
  * CHMC_SURR_NPARAMS floats: W1[64][32], b1[64], W2[64][64], b2[64], w3[64], b3, shift[32], scale[32]
  * (input z = (G - shift) * scale); the last 64 are set by the caller, not trained. */
 #define CHMC_SURR_NPARAMS 6401
+#define CHMC_SURR_NTRAIN 6337    /* the trained parameters: W1, b1, W2, b2, w3, b3 */
 #define CHMC_SURR_NGRAD 6338     /* the trained parameters + one slot for the sum of squared residuals */
 #define CHMC_SURR_NFEAT 32
 int chmc_surrogate_set_params(chmc_ctx* ctx, const float* params, int n);
 int chmc_surrogate_get_params(chmc_ctx* ctx, float* params, int n);
+/* Adam moments m[CHMC_SURR_NTRAIN], v[CHMC_SURR_NTRAIN] and step count of the training (either
+ * pointer of get may be NULL); with the parameters they checkpoint a training run (set_params resets
+ * them to zero) */
+int chmc_surrogate_get_adam(chmc_ctx* ctx, float* m, float* v, int* step);
+int chmc_surrogate_set_adam(chmc_ctx* ctx, const float* m, const float* v, int step);
 /* E of every chain (energy[nchains], may be NULL) at the current positions; store_force keeps +grad E
- * readable through chmc_get_force */
+ * readable through chmc_get_force.  FP32 neighbour-list contexts evaluate with the kernels of
+ * surrogate-driven MD (descriptors from the skin lists), cell-MD contexts with the 27-cell pair list. */
 int chmc_surrogate_eval(chmc_ctx* ctx, double* energy, int store_force);
 /* descriptors G[natoms][32] and dE/dG g[natoms][32] of the last chmc_surrogate_eval, atom order */
 int chmc_surrogate_features(chmc_ctx* ctx, int chain, float* G, float* g);
@@ -223,6 +230,18 @@ int chmc_surrogate_loss_grad(chmc_ctx* ctx, long long row0, int nrows, float* gr
 /* nsteps Adam steps (lr, beta1 0.9, beta2 0.999, eps 1e-8) on batches of batch_rows consecutive rows
  * at offsets drawn from mt19937_64(seed); loss_history[nsteps] (may be NULL) gets each batch's MSE */
 int chmc_surrogate_train(chmc_ctx* ctx, int nsteps, int batch_rows, float lr, uint64_t seed, float* loss_history);
+/* Energy and force matching on whole configurations: loss = mean_i (e_i - u_i)^2 + w_force *
+ * mean_{i,d} (dE/dx_id - dU/dx_id)^2 over every atom of every chain (U the truncated LJ energy, u_i its
+ * per-atom share, dU/dx the LJ gradient).  fm_loss_grad: at the current positions, the gradient w.r.t.
+ * the trained parameters grad[CHMC_SURR_NTRAIN] and both mean losses (no update).  reserve_configs /
+ * collect_config keep snapshots of the context's positions on the device; train_fm runs nsteps Adam
+ * steps (lr, 0.9, 0.999, 1e-8), each on one snapshot drawn by mt19937_64(seed), and writes the mean
+ * losses before each step (loss_e, loss_f may be NULL); the positions are left at the last snapshot. */
+int chmc_surrogate_fm_loss_grad(chmc_ctx* ctx, double w_force, float* grad, double* loss_e, double* loss_f);
+int chmc_surrogate_reserve_configs(chmc_ctx* ctx, int nconfigs);
+int chmc_surrogate_collect_config(chmc_ctx* ctx, int* nconfigs);
+int chmc_surrogate_train_fm(chmc_ctx* ctx, int nsteps, double w_force, float lr, uint64_t seed, float* loss_e,
+                            float* loss_f);
 
 /* ---- tuning ----------------------------------------------------------------------------------- */
 /* tiling of the FP32 LJ sweep, encoded i-slots per wavefront * 10 + wavefronts per workgroup, one of
@@ -252,10 +271,18 @@ int chmc_list_stats(chmc_ctx* ctx, int chain, long long* entries, long long* nin
 /* average device time (ms) of the list sweep with kick + drift (the MD step of the trajectory) and of
  * one rebuild, over reps, at the current positions (the positions do not change: dt = 0) */
 int chmc_time_list(chmc_ctx* ctx, int reps, double* ms_sweep, double* ms_rebuild);
-/* lanes per atom of the list sweep: 4, 8 or 16; 0 runs the autotuner (times each at the current
- * positions and keeps the fastest) */
+/* shape of the list sweep: 4, 8 or 16 lanes per atom (first unconditional batches of 16, 14 and 7 list
+ * entries per lane), or 812 / 816 (8 lanes, batches of 12 / 16 entries); 0 runs the autotuner (times
+ * each at the current positions and keeps the fastest) */
 int chmc_set_list_lanes(chmc_ctx* ctx, int lanes);
 int chmc_get_list_lanes(const chmc_ctx* ctx, int* lanes);
+/* rebuild checks of list MD before every `every`-th force evaluation of an integration (1 to 4; default
+ * 1), and before its last one.  In between, the rebuild trigger sits lower than skin / 2 by the
+ * distance every - 1 drifts can add for momenta |p| <= 10 sqrt(m / beta), so the lists still hold every
+ * pair within the cutoff; each drift verifies that bound, and a violation makes the call that reads
+ * the flags fail with CHMC_E_STATE.  1: the check before every evaluation.  No reference counterpart. */
+int chmc_set_list_check(chmc_ctx* ctx, int every);
+int chmc_get_list_check(const chmc_ctx* ctx, int* every);
 
 #ifdef __cplusplus
 }

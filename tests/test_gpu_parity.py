@@ -312,10 +312,10 @@ def test_list_md_vs_cell_md_at_1m_atoms(chmc):
 
 
 def test_list_sweep_lanes_agree(chmc):
-    """The list sweep at 4, 8 and 16 lanes per atom (first batches of 64, 112 and 112 entries read
-    without a count test from rows padded to a fixed span) gives the cell sweep's forces, also when the
-    lanes change after the lists were built (the padding does not depend on them).  The lane counts
-    differ only in the FP32 summation order."""
+    """The list sweep at every shape, 4, 8 and 16 lanes per atom and 8 lanes with first batches of 12 and
+    16 entries (64, 112, 112, 96 and 128 entries read without a count test from rows padded to a fixed
+    span), gives the cell sweep's forces, also when the shape changes after the lists were built (the
+    padding does not depend on it).  The shapes differ only in the FP32 summation order."""
     N, T = 4000, 0.722
     e, L = make(chmc, N, 0.8442, T=T)
     ec, _ = make(chmc, N, 0.8442, T=T, cell_md=True)
@@ -327,7 +327,7 @@ def test_list_sweep_lanes_agree(chmc):
     frms = np.sqrt(np.mean(fc ** 2))
     fcut = abs(48.0 * RC ** -6 * (RC ** -6 - 0.5) / RC)
     forces = {}
-    for lanes in (8, 4, 16, 8):
+    for lanes in (8, 4, 16, 812, 816, 8):
         e.set_list_lanes(lanes)
         assert e.list_lanes() == lanes
         e.integrate(6, 0.0, 1, 0.1932)               # dt = 0: one stored force of the list path
@@ -520,7 +520,7 @@ def test_skin_tuner_on_the_engine(chmc):
     assert np.allclose(skins, tune.density_scaled_skins(chains, b, ex))
     assert np.allclose([e.chain_skin(k) for k in range(2)], skins)
     lanes = e.list_lanes()
-    assert lanes in (4, 8, 16)
+    assert lanes in (4, 8, 16, 812, 816)
     r1, _ = e.hmc(3, 0, chmc.LEAPFROG, 0.005, 20, seed=9)
     e2 = chmc.Engine(chains)
     for k in range(2):
@@ -597,3 +597,51 @@ def test_error_paths(chmc):
         e.integrate(7, 0.01, 5)                                     # unknown scheme
     with pytest.raises(chmc.ChmcError):
         e.integrate(0, 0.01, 0)                                     # steps < 1 (reference loops forever)
+
+
+@pytest.mark.parametrize("every", [1, 2, 3])
+def test_list_check_interval_vs_oracle(chmc, orc, every):
+    """Rebuild checks before every `every`-th force evaluation (the trigger lowered by what the skipped
+    drifts can add) keep the list MD within the FP32 bars of the FP64 oracle over 10 OMF2 steps at
+    N = 32768 with a small skin (several rebuilds inside the run), and the pair counts equal."""
+    N, rho, T = 32768, 0.8442, 0.722
+    e, L = make(chmc, N, rho, T=T)
+    e.set_skin(0.3)
+    e.set_list_check(every)
+    assert e.list_check() == every
+    e.heatbath(9, 1)
+    x, p = e.positions(), e.momenta()
+    s = orc.Sim(N, [L] * 3, cutoff=RC, beta=1 / T)
+    s.x[:] = x
+    s.p[:] = p
+    _, _, rb0 = e.list_stats()
+    e.integrate(6, 0.01, 10, 0.1932)
+    s.integrate(6, 0.01, 10, 0.1932)
+    entries, nint, rb1 = e.list_stats()
+    dx = np.abs(min_image(e.positions() - s.x, L))
+    print(f"check every {every}: rebuilds {rb1 - rb0 - 1}, rms {np.sqrt(np.mean(dx ** 2)):.2e} max {dx.max():.2e}")
+    assert np.sqrt(np.mean(dx ** 2)) < 1e-6 and dx.max() < 2e-4
+    assert nint == e.pair_counts(0)[1]
+    e.close()
+
+
+def test_list_check_momentum_bound_is_enforced(chmc):
+    """With checks every other evaluation, an atom whose momentum exceeds 10 sqrt(m / beta) (set from the
+    host) makes the integration report invalid results; with a check before every evaluation the same
+    run is accepted."""
+    N, rho, T = 4000, 0.8442, 0.722
+    for every, fails in ((2, True), (1, False)):
+        e, L = make(chmc, N, rho, T=T)
+        e.set_list_check(every)
+        e.heatbath(9, 1)
+        p = e.momenta()
+        p[17] = [12.0 * np.sqrt(T), 0.0, 0.0]
+        e.set_momenta(p)
+        if fails:
+            with pytest.raises(chmc.ChmcError, match="momentum"):
+                e.integrate(6, 0.001, 2, 0.1932)
+                e.list_stats()
+        else:
+            e.integrate(6, 0.001, 2, 0.1932)
+            e.list_stats()
+        e.close()

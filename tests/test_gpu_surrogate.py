@@ -197,15 +197,16 @@ def test_training_reduces_loss(trained):
 
 
 def test_surrogate_hmc_graph_matches_direct(chmc, trained):
-    """Bit-identical records from graph replays and direct launches.  The engine orders atoms in a cell
-    by id, so the second run does not inherit a different binning history (and FP32 summation order)
-    from the first."""
+    """Bit-identical records from graph replays and direct launches.  Both runs start from the same
+    positions set through the API, so both rebuild their neighbour lists there (the list contents, and
+    with them the FP32 summation order, depend on where the lists were last built)."""
     t, _ = trained
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T), atom_id_order=True)
     e.init_lattice(13, 0.05)
     e.hmc(30, 31, chmc.LEAPFROG, DT, STEPS, seed=24)
     e.surrogate_set_params(t.surrogate_params())
     x0, t0 = e.positions(), e.trajectory()
+    e.set_positions(x0)
     r1, _ = e.hmc(6, 0, chmc.LEAPFROG, DT, STEPS, seed=77, use_graph=True, surrogate=True)
     e.set_positions(x0)
     e.set_trajectory(t0)                        # same heat-bath counters
@@ -246,3 +247,61 @@ def test_surrogate_chain_samples_lj_and_reports_ess(chmc, trained):
     z = stats.zscore(out["lj"], out["surrogate"])
     print(f"z(<U/N>) = {z:.2f}")
     assert abs(z) < 4
+
+
+def test_force_matching_gradient(chmc, system, osur):
+    """Energy + force matching loss gradient of the training kernels (residual forces, dL/dg, forward
+    tangent and reverse pass over whole configurations) against the oracle's (pinned to torch double
+    backward), for the force weight alone and mixed with the energy term."""
+    e, x, p = system
+    for w_force in (1.0, 0.3):
+        grad, le, lf = e.surrogate_fm_loss_grad(w_force)
+        me, mf, go = osur.force_loss_grad(p, [x], L, RC, w_force, bf=True)
+        print(f"w_force {w_force}: loss_e gpu {le:.5e} oracle {me:.5e}; loss_f gpu {lf:.5e} oracle {mf:.5e}")
+        assert abs(le - me) <= 2e-3 * me and abs(lf - mf) <= 2e-3 * mf
+        gd = chmc.unpack_surrogate(grad, trained_only=True)
+        for k, v in go.items():
+            err = np.abs(gd[k] - v).max()
+            print(f"  d loss / d {k}: max err {err:.3e} of max {np.abs(v).max():.3e}")
+            assert err <= 3e-3 * np.abs(v).max()
+
+
+def test_force_matching_training_reduces_both_losses(chmc, osur):
+    e = _engine(chmc, seed=17, burn=60)
+    e.surrogate_set_params(_params(osur, 3))
+    e.surrogate_eval()
+    G, _ = e.surrogate_features()
+    e.surrogate_set_params(_params(osur, 3, G.astype(np.float64)))
+    e.surrogate_reserve_configs(30)
+    for k in range(30):
+        e.hmc(3, 3, chmc.LEAPFROG, DT, STEPS, seed=700 + k)
+        assert e.surrogate_collect_config() == k + 1
+    le, lf = e.surrogate_train_fm(1500, 1.0, lr=2e-3, seed=8)
+    e.close()
+    print(f"energy MSE {le[:20].mean():.4e} -> {le[-50:].mean():.4e}; force MSE {lf[:20].mean():.4e} -> {lf[-50:].mean():.4e}")
+    assert np.all(np.isfinite(le)) and np.all(np.isfinite(lf))
+    assert le[-50:].mean() < 0.3 * le[:20].mean() and lf[-50:].mean() < 0.5 * lf[:20].mean()
+
+
+def test_training_checkpoint_resumes_bitwise(chmc, osur):
+    """Parameters + Adam moments + step count are a complete checkpoint: 40 Adam steps in one run equal
+    20 steps, a checkpoint into a fresh engine, and 20 more (same rows: one batch of all rows, offset 0)."""
+    def eng():
+        e = _engine(chmc, seed=19, burn=20)
+        e.surrogate_set_params(_params(osur, 5))
+        e.surrogate_reserve(1)
+        e.surrogate_collect()
+        return e
+    a = eng()
+    a.surrogate_train(40, N, lr=1e-3, seed=3)
+    b = eng()
+    b.surrogate_train(20, N, lr=1e-3, seed=3)
+    p, (m, v, st) = b.surrogate_params(), b.surrogate_adam_state()
+    assert st == 20
+    c = eng()
+    c.surrogate_set_params(p)
+    c.surrogate_set_adam_state(m, v, st)
+    c.surrogate_train(20, N, lr=1e-3, seed=3)
+    assert np.array_equal(c.surrogate_params(), a.surrogate_params())
+    for e in (a, b, c):
+        e.close()
