@@ -150,12 +150,13 @@ def other_workload(args):
     elif args.skin is None:
         def run(n, sd):
             e.hmc(n, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=sd, use_graph=not args.no_graph)
+        checks = None if args.list_check_given else (1, 2)
         if len(w.chains) > 1:
-            _, skins = tune.tune_skins(e, run, lanes=args.lanes)
+            _, skins = tune.tune_skins(e, run, lanes=args.lanes, checks=checks)
         else:
             _, skins = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,),
-                                       lanes=args.lanes)
-        family = min(skins, key=skins.get)
+                                       lanes=args.lanes, checks=checks)
+        family = "/".join(min(skins, key=skins.get).split("/")[:2])
     else:
         e.set_skin(args.skin)
         e.set_list_lanes(args.lanes)
@@ -432,6 +433,12 @@ def kernels_workload(args):
         e.surrogate_eval(store_force=True)
     out["surrogate_eval_ms_host"] = (time.perf_counter() - t0) / args.reps * 1e3
     out["surrogate_mlp_flops_fwd"] = 2.0 * (32 * 64 + 64 * 64 + 64) * N
+    # device-to-device copies of a known size (the positions, 3 N doubles each): the calibration of the
+    # FETCH_SIZE / WRITE_SIZE counters against a plain copy
+    e.surrogate_reserve_configs(4)
+    for _ in range(4):
+        e.surrogate_collect_config()
+    out["calibration_copy_bytes"] = 3 * N * 8
     e.close()
     print(json.dumps(out))
 
@@ -448,7 +455,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, choices=[0, 4, 8, 16, 812, 816],
                     help="fixed list-sweep lanes per atom (0: autotune)")
     ap.add_argument("--list-check", type=int, default=2, choices=[1, 2, 3],
-                    help="list MD: rebuild checks before every LIST_CHECK-th force evaluation (chmc_set_list_check)")
+                    help="list MD: rebuild checks before every LIST_CHECK-th force evaluation (chmc_set_list_check); "
+                         "tuned with the skin (1 or 2) unless given; 2 where the skin is fixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true",
@@ -466,6 +474,7 @@ def main():
     args = ap.parse_args()
     args.steps_given = "--steps" in sys.argv
     args.warmup_given = "--warmup" in sys.argv
+    args.list_check_given = "--list-check" in sys.argv
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
@@ -507,8 +516,8 @@ def main():
         def run(n, sd):
             e.hmc(n, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=sd, use_graph=not args.no_graph)
         _, table = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), reps=args.tune,
-                                   lanes=args.lanes)
-        skins = {float(k.split("/")[0]): v for k, v in table.items()}
+                                   lanes=args.lanes, checks=None if args.list_check_given else (1, 2))
+        skins = {k.split("/")[0] + ("/" + k.split("/")[2] if k.count("/") == 2 else ""): v for k, v in table.items()}
     else:
         e.set_skin(args.skin)
         e.set_list_lanes(args.lanes)

@@ -1557,6 +1557,8 @@ static int surr_ensure(chmc_ctx* c)
     HIPCHK(c, dalloc(c, &c->sv.params, kSurrNParams));
     HIPCHK(c, dalloc(c, &c->sv.w1b, kSurrHid * kSurrFeat));
     HIPCHK(c, dalloc(c, &c->sv.w2b, kSurrHid * kSurrHid));
+    HIPCHK(c, dalloc(c, &c->sv.w1tb, kSurrHid * kSurrFeat));
+    HIPCHK(c, dalloc(c, &c->sv.w2tb, kSurrHid * kSurrHid));
     HIPCHK(c, dalloc(c, &c->sv.adam_m, kSurrNTrain));
     HIPCHK(c, dalloc(c, &c->sv.adam_v, kSurrNTrain));
     HIPCHK(c, dalloc(c, &c->sv.nb_n, N));

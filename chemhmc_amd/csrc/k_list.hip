@@ -717,14 +717,13 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
 // per pair.  The
 // Gaussians of a pair come from two exponentials per block of 8 centres: with t = r - mu_k,
 // exp(-(t - D)^2 c) = exp(-t^2 c) rho, rho = exp(2 c D t - c D^2), and rho advances by exp(-2 c D^2).
-constexpr int kSurrPairCap = 128;   // pairs within rc per atom staged in LDS (local density <= 1.9 / sigma^3)
+constexpr int kSurrPairCap = 104;   // pairs within rc per atom staged in LDS (3 workgroups per CU)
 
 template <int MODE, int LP>
 __global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const float* __restrict__ g, double kick_dt,
                                                             double drift_dt, int q, int checked)
 {
-    __shared__ float4 spd[kTile][kSurrPairCap];          // (dx, dy, dz, r) of the slot's pairs within rc
-    __shared__ int spj[kTile][kSurrPairCap];             // their neighbour slots
+    __shared__ float4 spd[kTile][kSurrPairCap];          // (dx, dy, dz, neighbour slot bits) of the pairs within rc
     const int tile = xcd_block(blockIdx.x, v.ntiles);
     const Chunk tl = v.tiles[tile];
     const ListBuf L = list_buf(v, v.tcur[tile]);
@@ -740,7 +739,6 @@ __global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const flo
     double p_d = 0.0, x_d = 0.0;
     float4 ref = make_float4(0.f, 0.f, 0.f, 0.f);
     int np = 0;
-    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
     if (val) {
         if constexpr ((MODE & LJ_STOREF) != 0) atom = L.lperm[s];
         if (sub < 3) {
@@ -750,11 +748,9 @@ __global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const flo
                 ref = L.lref[s];
             }
         }
-        xi = xq[s];
-        const int n = L.nbn[s];
-        const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
         // pass 1: the pairs within rc, in entry order
-        np = stage_pairs<LP, 16, true>(C, xq, nbs, n, xi, sub, &spd[t][0], &spj[t][0], nullptr, kSurrPairCap);
+        np = stage_pairs<LP, 16, true>(C, xq, L.nb + (size_t)s * v.nbcap, L.nbn[s], xq[s], sub, &spd[t][0], nullptr,
+                                       nullptr, kSurrPairCap);
         if (np > kSurrPairCap) {
             if (sub == 0) atomicOr(&v.lflag[2], 8);
             np = kSurrPairCap;
@@ -773,19 +769,17 @@ __global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const flo
         const float c2 = 1.0f / (2.0f * kSurrWidth * kSurrWidth), iw2 = 1.0f / (kSurrWidth * kSurrWidth);
         constexpr float kD = (kSurrMu1 - kSurrMu0) / (float)(kSurrFeat - 1);
         const float cq = __expf(-2.0f * c2 * kD * kD);
-        // pass 2: the staged pairs, densely
-        for (int k = sub; k < np; k += LP) {
-            const float4 pd = spd[t][k];
-            const int j = spj[t][k];
-            const float r = pd.w;
-            const float fc = 0.5f * (__cosf(pirc * r) + 1.0f), dfc = -0.5f * pirc * __sinf(pirc * r);
-            const float4* gjs = reinterpret_cast<const float4*>(g + (size_t)j * kSurrFeat);
-            float gj[kSurrFeat];
+        auto load_g = [&](const float4 pd, float gj[kSurrFeat]) {
+            const float4* gjs = reinterpret_cast<const float4*>(g + (size_t)__float_as_int(pd.w) * kSurrFeat);
 #pragma unroll
             for (int b = 0; b < kSurrFeat / 4; b++) {
                 const float4 q4 = gjs[b];
                 gj[4 * b] = q4.x; gj[4 * b + 1] = q4.y; gj[4 * b + 2] = q4.z; gj[4 * b + 3] = q4.w;
             }
+        };
+        auto pair_force = [&](const float4 pd, const float gj[kSurrFeat]) {
+            const float r = sqrtf(__builtin_fmaf(pd.z, pd.z, __builtin_fmaf(pd.y, pd.y, pd.x * pd.x)));
+            const float fc = 0.5f * (__cosf(pirc * r) + 1.0f), dfc = -0.5f * pirc * __sinf(pirc * r);
             float acc = 0.f;
 #pragma unroll
             for (int b = 0; b < kSurrFeat / 8; b++) {
@@ -806,6 +800,17 @@ __global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const flo
             ax = __builtin_fmaf(coef, pd.x, ax);
             ay = __builtin_fmaf(coef, pd.y, ay);
             az = __builtin_fmaf(coef, pd.z, az);
+        };
+        // pass 2: the staged pairs, densely, two per step with both neighbours' g loaded together
+        for (int k = sub; k < np; k += 2 * LP) {
+            const bool two = k + LP < np;
+            const float4 pd0 = spd[t][k];
+            const float4 pd1 = two ? spd[t][k + LP] : pd0;
+            float g0[kSurrFeat], g1[kSurrFeat];
+            load_g(pd0, g0);
+            load_g(pd1, g1);
+            pair_force(pd0, g0);
+            if (two) pair_force(pd1, g1);
         }
     }
 #pragma unroll

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void k_surr_desc(DevView v, SurrView s, float*
 // --------------------------------------------------------------------------------------------------
 // MLP forward + input gradient on the MFMA units
 // --------------------------------------------------------------------------------------------------
-struct MlpSmem {
+struct alignas(16) MlpSmem {
     __bf16 W1[kSurrHid][kSurrFeat];      // [out][in]
     __bf16 W1T[kSurrFeat][kSurrHid];
     __bf16 W2[kSurrHid][kSurrHid];
@@ -169,16 +169,16 @@ struct MlpSmem {
 
 __device__ void load_weights(MlpSmem& sm, const SurrView& s)
 {
-    for (int t = threadIdx.x; t < kSurrHid * kSurrFeat; t += blockDim.x) {
-        const int o = t / kSurrFeat, i = t % kSurrFeat;
-        sm.W1[o][i] = s.w1b[t];
-        sm.W1T[i][o] = s.w1b[t];
-    }
-    for (int t = threadIdx.x; t < kSurrHid * kSurrHid; t += blockDim.x) {
-        const int o = t / kSurrHid, i = t % kSurrHid;
-        sm.W2[o][i] = s.w2b[t];
-        sm.W2T[i][o] = s.w2b[t];
-    }
+    // the four bf16 matrices (both orientations are kept in global memory) in 16-byte pieces
+    auto copy16 = [&](void* dst, const void* src, int bytes) {
+        const uint4* q = static_cast<const uint4*>(src);
+        uint4* d = static_cast<uint4*>(dst);
+        for (int t = threadIdx.x; t < bytes / 16; t += blockDim.x) d[t] = q[t];
+    };
+    copy16(&sm.W1[0][0], s.w1b, kSurrHid * kSurrFeat * 2);
+    copy16(&sm.W1T[0][0], s.w1tb, kSurrHid * kSurrFeat * 2);
+    copy16(&sm.W2[0][0], s.w2b, kSurrHid * kSurrHid * 2);
+    copy16(&sm.W2T[0][0], s.w2tb, kSurrHid * kSurrHid * 2);
     const float* P = s.params;
     for (int t = threadIdx.x; t < kSurrHid; t += blockDim.x) {
         sm.b1[t] = P[kOffB1 + t];
@@ -491,6 +491,20 @@ __global__ __launch_bounds__(256) void k_surr_sum(const float* __restrict__ part
     out[p] = gsum;
 }
 
+// the bfloat16 copies (both orientations) of W1 / W2 entry p of the parameter blob
+__device__ __forceinline__ void store_bf16_weight(const SurrView& s, int p, float val)
+{
+    const __bf16 b = (__bf16)val;
+    if (p < kOffB1) {
+        s.w1b[p] = b;
+        s.w1tb[(p % kSurrFeat) * kSurrHid + p / kSurrFeat] = b;
+    } else if (p >= kOffW2 && p < kOffB2) {
+        const int q = p - kOffW2;
+        s.w2b[q] = b;
+        s.w2tb[(q % kSurrHid) * kSurrHid + q / kSurrHid] = b;
+    }
+}
+
 // one thread per trainable parameter: fixed-order sum over workgroup partials, then Adam
 __global__ __launch_bounds__(256) void k_surr_adam(SurrView s, const float* __restrict__ partial, int nparts, int ngrad,
                                                    float lr, float beta1, float beta2, float eps, int step,
@@ -510,15 +524,13 @@ __global__ __launch_bounds__(256) void k_surr_adam(SurrView s, const float* __re
     const float vh = vv / (1.f - powf(beta2, (float)step));
     const float np = s.params[p] - lr * mh / (sqrtf(vh) + eps);
     s.params[p] = np;
-    if (p < kOffB1) s.w1b[p] = (__bf16)np;
-    else if (p >= kOffW2 && p < kOffB2) s.w2b[p - kOffW2] = (__bf16)np;
+    store_bf16_weight(s, p, np);
 }
 
 __global__ void k_surr_refresh(SurrView s)
 {
     const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p < kOffB1) s.w1b[p] = (__bf16)s.params[p];
-    else if (p >= kOffW2 && p < kOffB2) s.w2b[p - kOffW2] = (__bf16)s.params[p];
+    if (p < kOffB2) store_bf16_weight(s, p, s.params[p]);
 }
 
 // per-atom LJ energies u_i = 2 eps sum_j sr6 (sr6 - 1) in sorted order, FP64 pairs from the surrogate's

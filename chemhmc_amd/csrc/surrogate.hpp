@@ -22,6 +22,8 @@ struct SurrView {
     float* params;          // [kSurrNParams] FP32 master copy
     __bf16* w1b;            // [64*32] bfloat16 copy of W1
     __bf16* w2b;            // [64*64] bfloat16 copy of W2
+    __bf16* w1tb;           // [32*64] bfloat16 copy of W1 transposed
+    __bf16* w2tb;           // [64*64] bfloat16 copy of W2 transposed
     float *adam_m, *adam_v; // [kSurrNTrain]
     int* nb_n;              // [Ntot] pairs per atom (by sorted slot)
     int* nb_j;              // [Ntot * kSurrMaxNb] sorted slot of the neighbour
@@ -34,11 +36,11 @@ struct SurrView {
 // slot read KB entries each without a count test (rows are padded past their count to kListSpan
 // entries), gather the neighbours' coordinates together, then test them; a row longer than KB * LP
 // entries continues in a loop.  The pairs within rc are appended in entry order to the slot's LDS row
-// by ballot prefix counts: with GEOM, pd = (dx, dy, dz, r) and pj = the neighbour slot, otherwise r
-// only (rd).  Every lane of the slot returns the same count (which may exceed cap: the caller flags it).
+// by ballot prefix counts: with GEOM, pd = (dx, dy, dz, neighbour slot as float bits), otherwise r only
+// (rd).  Every lane of the slot returns the same count (which may exceed cap: the caller flags it).
 template <int LP, int KB, bool GEOM>
 __device__ __forceinline__ int stage_pairs(const ChainDev& C, const float4* __restrict__ xq, const int* __restrict__ nbs,
-                                           int n, float4 xi, int sub, float4* pd, int* pj, float* rd, int cap)
+                                           int n, float4 xi, int sub, float4* pd, int* /*unused*/, float* rd, int cap)
 {
     static_assert(KB * LP <= kListSpan, "the first batch reads padded entries only");
     const int gbase = (threadIdx.x & 63) & ~(LP - 1);
@@ -55,8 +57,7 @@ __device__ __forceinline__ int stage_pairs(const ChainDev& C, const float4* __re
         const int pos = np + __popc(grp & ((1u << sub) - 1u));
         if (hit && pos < cap) {
             if constexpr (GEOM) {
-                pd[pos] = make_float4(dx, dy, dz, sqrtf(r2));
-                pj[pos] = (int)((unsigned)e >> kImageBits);
+                pd[pos] = make_float4(dx, dy, dz, __int_as_float((int)((unsigned)e >> kImageBits)));
             } else {
                 rd[pos] = sqrtf(r2);
             }

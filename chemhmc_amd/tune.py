@@ -41,16 +41,19 @@ def restore(engine, snap):
 
 
 def tune_skins(engine, run, bases=(0.35, 0.4, 0.45, 0.5, 0.6), exponents=(0.0, 1.0, 1.5), warm=0, reps=24,
-               seed=1000, lanes=0):
+               seed=1000, lanes=0, checks=None):
     """Time `run(ntraj, seed)` (trajectories of the engine's workload) for every (base, exponent) as a
     paired comparison: each candidate starts from the same positions and trajectory counter and runs the
     same random streams, warm untimed trajectories and then reps timed ones in one call, as a timed run
     that follows the tuner starts (fresh lists, then a rebuild whenever an atom has moved skin / 2; a
     window that skips the first rebuild interval misranks the skins).  The lanes per atom of the list
     sweep are set for every candidate (`lanes`, 0: autotuned at that candidate's lists), since the best
-    lanes change with the list length and a skin timed at the wrong lanes is misjudged.  The engine is
-    returned to the starting state with the fastest skins and their lanes.  Returns (skins,
-    {"base/exponent": ms per trajectory})."""
+    lanes change with the list length and a skin timed at the wrong lanes is misjudged.  `checks`
+    (optional) also searches the rebuild-check interval of list MD (chmc_set_list_check) with every
+    skin: a longer interval saves launches but lowers the rebuild trigger, which pays where rebuilds are
+    cheap.  The engine is returned to the starting state with the fastest skins, their lanes and check
+    interval.  Returns (skins, {"base/exponent": ms per trajectory}, with "/m" appended when checks are
+    searched)."""
     snap = snapshot(engine)
     table, best = {}, None
     for ex in exponents:
@@ -60,20 +63,25 @@ def tune_skins(engine, run, bases=(0.35, 0.4, 0.45, 0.5, 0.6), exponents=(0.0, 1
                 engine.set_chain_skins(skins)
             except ChmcError:
                 continue
-            restore(engine, snap)
-            engine.set_list_lanes(lanes)
-            lp = engine.list_lanes()
-            if warm > 0:
-                run(warm, seed)
-            t0 = time.perf_counter()
-            run(reps, seed + 1)
-            ms = (time.perf_counter() - t0) / reps * 1e3
-            table[f"{b}/{ex}"] = ms
-            if best is None or ms < best[0]:
-                best = (ms, skins, lp)
+            for m in (checks if checks else (None,)):
+                if m is not None:
+                    engine.set_list_check(m)
+                restore(engine, snap)
+                engine.set_list_lanes(lanes)
+                lp = engine.list_lanes()
+                if warm > 0:
+                    run(warm, seed)
+                t0 = time.perf_counter()
+                run(reps, seed + 1)
+                ms = (time.perf_counter() - t0) / reps * 1e3
+                table[f"{b}/{ex}" if m is None else f"{b}/{ex}/{m}"] = ms
+                if best is None or ms < best[0]:
+                    best = (ms, skins, lp, m)
     if best is None:
         raise ChmcError("tune_skins: no candidate skin fits the list MD")
     engine.set_chain_skins(best[1])
+    if best[3] is not None:
+        engine.set_list_check(best[3])
     restore(engine, snap)
     engine.set_list_lanes(best[2])
     return best[1], table
