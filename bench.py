@@ -47,7 +47,8 @@ SWEEP_BYTES_PER_ATOM = 148.0
 # --pmc WRITE_SIZE (medians over 2850 dispatches, KiB; a plain copy reads 1.00 for WRITE_SIZE) at
 # N=32768, skins 0.6 and 0.7, 8 lanes per atom (profiles/lj_list_sweep_N32768.md): reported as the roofline's
 # traffic when the run's tuned skin and lanes are those, null otherwise
-SWEEP_TRAFFIC_PROFILED = {(0.6, 8): (9976.0 + 2048.0) * 1024.0, (0.7, 8): (9970.0 + 2048.0) * 1024.0}
+SWEEP_TRAFFIC_PROFILED = {(0.6, 8): (9976.0 + 2048.0) * 1024.0, (0.7, 8): (9970.0 + 2048.0) * 1024.0,
+                          (0.7, 816): (10086.8 + 2048.0) * 1024.0}
 # the same at N=1,048,576 (workload c5), skin 0.5, 8 lanes (profiles/lj_list_sweep_N1048576.md)
 SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8): (322982.0 + 65536.0) * 1024.0}
 # the same for the 256-chain batch (workload c3) at the per-chain skins base 0.4, exponent 1.5 and 4
@@ -408,7 +409,7 @@ def kernels_workload(args):
     L = (N / RHO) ** (1 / 3)
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
     e.init_lattice(1, 0.05)
-    e.hmc(THERM, THERM, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10)
+    e.hmc(args.therm, args.therm, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10)
     out = {"workload": "kernels at N=32768 rho=0.8442 T=0.722", "N": N}
     ms_bin, ms_force = e.time_force(reps=args.reps)
     ncand, nint = e.pair_counts(0)
@@ -457,6 +458,10 @@ def main():
     ap.add_argument("--list-check", type=int, default=2, choices=[1, 2, 3],
                     help="list MD: rebuild checks before every LIST_CHECK-th force evaluation (chmc_set_list_check); "
                          "tuned with the skin (1 or 2) unless given; 2 where the skin is fixed")
+    ap.add_argument("--therm", type=int, default=THERM,
+                    help="c2, kernels: accept-all burn-in trajectories off the lattice (shorter for counter runs)")
+    ap.add_argument("--no-ess", action="store_true",
+                    help="c2: ESS from the timed run itself, no separate long run (counter and trace runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true",
@@ -504,7 +509,7 @@ def main():
         cell_md_run(args, e, chmc, stats)
         return
     # accept-all thermalisation away from the lattice (thermalization_steps), then the untimed warm-up
-    e.hmc(THERM, THERM, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10, use_graph=not args.no_graph)
+    e.hmc(args.therm, args.therm, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10, use_graph=not args.no_graph)
     if args.warmup > 0:
         e.hmc(args.warmup, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
     # skin autotuner: wall time of a short run per skin from the equilibrated state (paired: every skin
@@ -534,7 +539,9 @@ def main():
     rebuilds_per_traj = (rebuilds1 - rebuilds0 - 1) / args.steps       # list_stats itself rebuilds once
 
     # ESS/s of U/N on a run long enough for tau_int (the timed run itself when it is)
-    if args.steps >= ESS_TRAJ:
+    if args.no_ess:
+        ess_rec, ess_acc, ess_el = rec, nacc, el
+    elif args.steps >= ESS_TRAJ:
         ess_rec, ess_acc, ess_el = rec, nacc, el
     else:
         t0 = time.perf_counter()
@@ -572,7 +579,7 @@ def main():
         "dtype": "f32 pair forces, f64 x/p/dH", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": "plain HMC N=32768 rho=0.8442 T=0.722 rc=2.5 OMF2(lambda=0.1932) 10x0.01",
                    "N": N, "trajectories": args.steps, "md_steps_per_traj": NSTEPS, "force_evals_per_traj": 2 * NSTEPS + 1,
-                   "thermalisation": THERM, "md_path": "neighbour lists", "skin": skin, "list_lanes": lanes, "list_check": e.list_check(),
+                   "thermalisation": args.therm, "md_path": "neighbour lists", "skin": skin, "list_lanes": lanes, "list_check": e.list_check(),
                    "ms_per_traj_by_skin": skins},
         "acceptance": float(nacc[0]) / args.steps,
         "ess_per_s": ess_info["ess_per_s"], "ess": ess_info,
