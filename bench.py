@@ -45,10 +45,11 @@ PEAK_HBM_GBS = 8000.0
 SWEEP_BYTES_PER_ATOM = 148.0
 # HBM traffic of one list sweep (force + kick + drift) measured with rocprofv3 --pmc FETCH_SIZE and
 # --pmc WRITE_SIZE (medians over 2850 dispatches, KiB; a plain copy reads 1.00 for WRITE_SIZE) at
-# N=32768, skins 0.6 and 0.7, 8 lanes per atom (profiles/lj_list_sweep_N32768.md): reported as the roofline's
-# traffic when the run's tuned skin and lanes are those, null otherwise
-SWEEP_TRAFFIC_PROFILED = {(0.6, 8): (9976.0 + 2048.0) * 1024.0, (0.7, 8): (9970.0 + 2048.0) * 1024.0,
-                          (0.7, 816): (10086.8 + 2048.0) * 1024.0}
+# N=32768, skins 0.6 and 0.7, 8 lanes per atom (profiles/lj_list_sweep_N32768.md), keyed by (skin, lanes,
+# speculative copies in the launch): reported as the roofline's traffic when the run's tuned skin, lanes
+# and copies are those, null otherwise
+SWEEP_TRAFFIC_PROFILED = {(0.6, 8, 1): (9976.0 + 2048.0) * 1024.0, (0.7, 8, 1): (9970.0 + 2048.0) * 1024.0,
+                          (0.7, 816, 1): (10086.8 + 2048.0) * 1024.0}
 # the same at N=1,048,576 (workload c5), skin 0.5, 8 lanes (profiles/lj_list_sweep_N1048576.md)
 SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8): (322982.0 + 65536.0) * 1024.0}
 # the same for the 256-chain batch (workload c3) at the per-chain skins base 0.4, exponent 1.5 and 4
@@ -458,6 +459,9 @@ def main():
     ap.add_argument("--list-check", type=int, default=2, choices=[1, 2, 3],
                     help="list MD: rebuild checks before every LIST_CHECK-th force evaluation (chmc_set_list_check); "
                          "tuned with the skin (1 or 2) unless given; 2 where the skin is fixed")
+    ap.add_argument("--spec", type=int, default=8, choices=[1, 2, 4, 8, 16],
+                    help="c2: speculative copies of the chain (chmc_set_speculation; 1: the sequential loop).  The "
+                         "chain, its records and its acceptance are the sequential run's, bit for bit")
     ap.add_argument("--therm", type=int, default=THERM,
                     help="c2, kernels: accept-all burn-in trajectories off the lattice (shorter for counter runs)")
     ap.add_argument("--no-ess", action="store_true",
@@ -502,7 +506,8 @@ def main():
     from chemhmc_amd import stats, tune
 
     L = (N / RHO) ** (1 / 3)
-    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T), cell_md=args.cell_md)
+    copies = 1 if args.cell_md else args.spec
+    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T), cell_md=args.cell_md, speculate=copies)
     e.init_lattice(1, 0.05)
     e.set_list_check(args.list_check)
     if args.cell_md:
@@ -536,7 +541,8 @@ def main():
     ms = el / args.steps * 1e3
     value = N * NSTEPS * args.steps / el
     entries, nint, rebuilds1 = e.list_stats()
-    rebuilds_per_traj = (rebuilds1 - rebuilds0 - 1) / args.steps       # list_stats itself rebuilds once
+    # list_stats itself rebuilds every copy once; with speculative copies every copy's rebuilds count
+    rebuilds_per_traj = (rebuilds1 - rebuilds0 - copies) / args.steps
 
     # ESS/s of U/N on a run long enough for tau_int (the timed run itself when it is)
     if args.no_ess:
@@ -564,13 +570,14 @@ def main():
     # stencil + 13 per interacting pair, per atom-evaluation; candidates and interacting pairs counted on
     # the device).  Its HBM side: 4 B per list entry + SWEEP_BYTES_PER_ATOM per atom algorithmic, and the
     # rocprofv3-measured traffic when the tuned skin and lanes are the profiled ones.
+    # (a launch sweeps every speculative copy: copies x the one chain's work)
     ms_sweep, ms_rebuild = e.time_list(reps=50)
     ncand_cell, nint_cell = e.pair_counts(0)
-    flops_alg = 8.0 * ncand_cell + 13.0 * nint_cell
-    flops_exec = 8.0 * entries + 13.0 * nint
-    nbytes = 4.0 * entries + SWEEP_BYTES_PER_ATOM * N
+    flops_alg = copies * (8.0 * ncand_cell + 13.0 * nint_cell)
+    flops_exec = copies * (8.0 * entries + 13.0 * nint)
+    nbytes = copies * (4.0 * entries + SWEEP_BYTES_PER_ATOM * N)
     tflops = flops_alg / (ms_sweep * 1e-3) / 1e12
-    traffic = SWEEP_TRAFFIC_PROFILED.get((round(float(skin), 2), int(lanes)))
+    traffic = SWEEP_TRAFFIC_PROFILED.get((round(float(skin), 2), int(lanes), copies))
 
     out = {
         "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N",
@@ -579,15 +586,17 @@ def main():
         "dtype": "f32 pair forces, f64 x/p/dH", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": "plain HMC N=32768 rho=0.8442 T=0.722 rc=2.5 OMF2(lambda=0.1932) 10x0.01",
                    "N": N, "trajectories": args.steps, "md_steps_per_traj": NSTEPS, "force_evals_per_traj": 2 * NSTEPS + 1,
-                   "thermalisation": args.therm, "md_path": "neighbour lists", "skin": skin, "list_lanes": lanes, "list_check": e.list_check(),
+                   "thermalisation": args.therm, "md_path": "neighbour lists", "speculative_copies": copies,
+                   "skin": skin, "list_lanes": lanes, "list_check": e.list_check(),
                    "ms_per_traj_by_skin": skins},
         "acceptance": float(nacc[0]) / args.steps,
         "ess_per_s": ess_info["ess_per_s"], "ess": ess_info,
-        "list_rebuilds_per_traj": rebuilds_per_traj,
+        "list_rebuilds_per_traj": rebuilds_per_traj,   # over all copies when speculative
         "roofline": {"bound": "valu", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": tflops,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFLOPS,
                      "traffic": traffic, "flops_per_launch": flops_alg, "ms_per_launch": ms_sweep,
                      "work": "8 * N_cand + 13 * N_int per atom-evaluation, N_cand of the 27-cell stencil",
+                     "atoms_per_launch": copies * N,
                      "cell_candidates_per_atom": ncand_cell / N, "n_int_per_atom": nint_cell / N,
                      "executed_flops_per_launch": flops_exec, "list_entries_per_atom": entries / N,
                      "hbm": {"bytes_per_launch": nbytes, "achieved_gbs": nbytes / (ms_sweep * 1e-3) / 1e9,

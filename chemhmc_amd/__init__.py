@@ -114,6 +114,8 @@ def _declare(L):
         "chmc_set_list_lanes": ([P, _I], _I),
         "chmc_set_list_check": ([P, _I], _I),
         "chmc_get_list_check": ([P, C.POINTER(_I)], _I),
+        "chmc_set_speculation": ([P, _I], _I),
+        "chmc_get_speculation": ([P, C.POINTER(_I)], _I),
         "chmc_get_list_lanes": ([P, C.POINTER(_I)], _I),
         "chmc_set_trajectory": ([P, _I], _I),
         "chmc_get_trajectory": ([P, C.POINTER(_I)], _I),
@@ -149,7 +151,7 @@ EXPORTED = (
     "chmc_set_force_tiling",
     "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_set_chain_skins",
     "chmc_get_chain_skin", "chmc_list_stats",
-    "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_list_check", "chmc_get_list_check", "chmc_set_trajectory", "chmc_get_trajectory",
+    "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_list_check", "chmc_get_list_check", "chmc_set_speculation", "chmc_get_speculation", "chmc_set_trajectory", "chmc_get_trajectory",
     "chmc_surrogate_set_params", "chmc_surrogate_get_params",
     "chmc_surrogate_eval", "chmc_surrogate_features", "chmc_surrogate_reserve", "chmc_surrogate_collect",
     "chmc_surrogate_training_data", "chmc_surrogate_loss_grad", "chmc_surrogate_train",
@@ -205,12 +207,21 @@ def _c(a, dtype=np.float64):
 
 
 class Engine:
-    """A context of one or more chains on one GPU (all calls act on every chain at once)."""
+    """A context of one or more chains on one GPU (all calls act on every chain at once).
+
+    speculate=K (K > 1, one chain given): the context holds K copies of the chain and hmc() runs them as
+    speculative trajectories of that one chain (chmc_set_speculation): the records and counts are the
+    single chain's, identical to a run without speculation."""
 
     def __init__(self, chains: Sequence[Chain] | Chain, device: int = -1, fp64_forces: bool = False,
-                 atom_id_order: bool = False, cell_md: bool = False):
+                 atom_id_order: bool = False, cell_md: bool = False, speculate: int = 0):
         if isinstance(chains, Chain):
             chains = [chains]
+        if speculate > 1:
+            if len(chains) != 1:
+                raise ValueError("speculate: give the one chain to copy")
+            chains = list(chains) * int(speculate)
+        self._spec = 0
         self.chains = list(chains)
         arr = (ChainDesc * len(self.chains))(*[c.to_c() for c in self.chains])
         flags = ((FORCE_FP64 if fp64_forces else 0) | (ORDER_ATOM_ID if atom_id_order else 0)
@@ -219,6 +230,8 @@ class Engine:
         rc = lib().chmc_create(C.byref(self._h), len(self.chains), arr, int(device), flags)
         if rc != 0:
             raise ChmcError(f"chmc_create failed ({rc}): {lib().chmc_last_error(None).decode()}")
+        if speculate > 1:
+            self.set_speculation(int(speculate))
 
     # -- plumbing ----------------------------------------------------------------------------
     def _ck(self, rc, what):
@@ -362,6 +375,8 @@ class Engine:
         rec = np.zeros(ntraj * self.nchains, HMC_RECORD_DTYPE)
         nacc = np.zeros(self.nchains, np.int32)
         self._ck(lib().chmc_hmc_run(self._h, C.byref(hp), rec.ctypes.data, nacc.ctypes.data), "hmc_run")
+        if self._spec > 1:                     # speculative copies of one chain: its records only
+            return rec.reshape(ntraj, self.nchains)[:, :1].copy(), nacc[:1].copy()
         return rec.reshape(ntraj, self.nchains), nacc
 
     def rdf(self, nbins, size_bin, chain=0):
@@ -505,6 +520,17 @@ class Engine:
     def list_check(self):
         v = _I()
         self._ck(lib().chmc_get_list_check(self._h, C.byref(v)), "get_list_check")
+        return v.value
+
+    def set_speculation(self, copies):
+        """Run hmc() as speculative trajectories of chain 0 over the context's `copies` identical chains
+        (copies <= 1: off)."""
+        self._ck(lib().chmc_set_speculation(self._h, int(copies)), "set_speculation")
+        self._spec = self.speculation()
+
+    def speculation(self):
+        v = _I()
+        self._ck(lib().chmc_get_speculation(self._h, C.byref(v)), "get_speculation")
         return v.value
 
     def time_list(self, reps=20):

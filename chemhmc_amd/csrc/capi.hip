@@ -37,6 +37,7 @@ struct chmc_ctx {
     double pending_drift = 0.0;  // drift deferred into the next binning (integrator sequencing)
     int lp_exponent = 16;        // n_prec of the lp_* integrators (lp_leapfrog() : n_prec(16))
     int traj_done = 0;           // trajectories completed on the device counter
+    int spec = 0;                // > 1: the chains are that many copies of one chain, run speculatively
     std::string err;
     double* d_uniforms = nullptr;
     size_t uniforms_cap = 0;
@@ -1249,44 +1250,58 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
     return CHMC_OK;
 }
 
+// One speculative batch: the chains are copies of one chain at its state after trajectory t0 (the
+// device counter); copy k runs trajectory t0 + 1 + k (heat bath of that trajectory, MD, FP64 energy),
+// the decisions consume them in order up to the first accepted one (or the run's last trajectory tend),
+// and the accepted copy's state goes to every copy.  The chain is the one the sequential loop of
+// enqueue_trajectory produces, trajectory for trajectory: a rejected trajectory leaves the state
+// unchanged, so every trajectory up to the first accepted one starts from the same state.
+static int enqueue_spec_batch(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst, int rec_base, int tend)
+{
+    int rc;
+    HIPCHK(c, launch_heatbath_spec(c->v, hp.seed, c->stream));
+    if ((rc = integrate_seq(c, hp.scheme, hp.dt, hp.steps, hp.lambda, false, true))) return rc;
+    c->list_valid = true;
+    if ((rc = list_energy(c, false))) return rc;
+    HIPCHK(c, launch_reduce(c->v, SUM_TILES | SUM_CHUNKS_KI, c->stream));
+    HIPCHK(c, launch_spec_restore(c->v, c->d_uniforms, hp.therm, ufirst, c->d_rec, rec_base, tend, c->lpar, c->stream));
+    HIPCHK(c, launch_spec_broadcast(c->v, -1, true, c->stream));
+    c->list_fresh = c->rel64_fresh = false;
+    c->force_kind = 0;
+    c->bin_state = 2;
+    return CHMC_OK;
+}
+
 // a captured trajectory sequence: valid for replays that start at binning parity `par`, list parity
-// `lpar` and list count buffer `cpar` in MD mode `surr`
+// `lpar` and list count buffer `cpar` in MD mode `surr` (spec: speculative batches)
 struct TrajGraph {
     hipGraphExec_t exec;
     int par, lpar, cpar, per_graph;
-    bool surr;
+    bool surr, spec;
 };
 
-// trajectories t0 .. t0+count-1 of a run in the context's current MD mode, by graph replays where
-// allowed and directly otherwise
-static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count, int ufirst, int rec_base,
-                       const std::vector<int>& rsteps, std::vector<TrajGraph>& graphs)
+// count units (trajectories or speculative batches) enqueued by enq(), by graph replays
+template <typename Enq>
+static int run_units(chmc_ctx* c, int count, bool spec, std::vector<TrajGraph>& graphs, Enq enq)
 {
     int rc;
-    if (!hp.use_graph || hp.randomize) {
-        for (int t = t0; t < t0 + count; t++) {
-            const int steps = hp.randomize ? rsteps[t] : hp.steps;
-            if ((rc = enqueue_trajectory(c, hp, steps, ufirst, rec_base))) return rc;
-        }
-        return CHMC_OK;
-    }
     int done = 0;
     const TrajGraph* g = nullptr;
     for (const TrajGraph& k : graphs)
-        if (k.par == c->v.par && k.lpar == c->lpar && k.cpar == c->cpar && k.surr == c->md_surr) g = &k;
+        if (k.par == c->v.par && k.lpar == c->lpar && k.cpar == c->cpar && k.surr == c->md_surr && k.spec == spec) g = &k;
     if (!g) {
-        // one trajectory per graph when the parities baked into the captured launches (binning, list,
+        // one unit per graph when the parities baked into the captured launches (binning, list,
         // list count buffer) end where they started, two otherwise.  The host state after the capture
-        // is that of per_graph trajectories, which every replay reproduces.
+        // is that of per_graph units, which every replay reproduces.
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         HIPCHK(c, hipStreamSynchronize(c->stream));
         const int par0 = c->v.par, lpar0 = c->lpar, cpar0 = c->cpar;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base);
+        rc = enq();
         int per_graph = 1;
         if (!rc && (c->v.par != par0 || c->lpar != lpar0 || c->cpar != cpar0)) {
-            rc = enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base);
+            rc = enq();
             per_graph = 2;
         }
         hipError_t ec = hipStreamEndCapture(c->stream, &graph);
@@ -1295,7 +1310,7 @@ static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count
         hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         HIPCHK(c, ei);
-        graphs.push_back(TrajGraph{ exec, par0, lpar0, cpar0, per_graph, c->md_surr });
+        graphs.push_back(TrajGraph{ exec, par0, lpar0, cpar0, per_graph, c->md_surr, spec });
         g = &graphs.back();
         if (count < per_graph) {
             // nothing of the capture ran; the host state is restored by running the leftover directly
@@ -1310,7 +1325,49 @@ static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count
     }
     for (; done + g->per_graph <= count; done += g->per_graph) HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
     for (; done < count; done++)
-        if ((rc = enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base))) return rc;
+        if ((rc = enq())) return rc;
+    return CHMC_OK;
+}
+
+// trajectories t0 .. t0+count-1 of a run in the context's current MD mode, by graph replays where
+// allowed and directly otherwise
+static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count, int ufirst, int rec_base,
+                       const std::vector<int>& rsteps, std::vector<TrajGraph>& graphs)
+{
+    int rc;
+    if (!hp.use_graph || hp.randomize) {
+        for (int t = t0; t < t0 + count; t++) {
+            const int steps = hp.randomize ? rsteps[t] : hp.steps;
+            if ((rc = enqueue_trajectory(c, hp, steps, ufirst, rec_base))) return rc;
+        }
+        return CHMC_OK;
+    }
+    return run_units(c, count, false, graphs, [&]() { return enqueue_trajectory(c, hp, hp.steps, ufirst, rec_base); });
+}
+
+// a speculative run: batches of nchains trajectories until the device counter reaches tend.  A batch
+// consumes between one trajectory and nchains, so the host enqueues as many batches as the remaining
+// trajectories need when none is accepted, reads the counter and repeats.
+static int run_spec(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst, int rec_base, int tend,
+                    std::vector<TrajGraph>& graphs)
+{
+    const int K = c->nchains;
+    int done = rec_base, rc;
+    while (done < tend) {
+        const int nb = (tend - done + K - 1) / K;
+        auto enq = [&]() { return enqueue_spec_batch(c, hp, ufirst, rec_base, tend); };
+        if (hp.use_graph) {
+            if ((rc = run_units(c, nb, true, graphs, enq))) return rc;
+        } else {
+            for (int b = 0; b < nb; b++)
+                if ((rc = enq())) return rc;
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const int before = done;
+        HIPCHK(c, hipMemcpy(&done, c->v.traj, sizeof(int), hipMemcpyDeviceToHost));
+        if (done <= before || done > tend)     // every batch consumes at least one trajectory
+            return fail(c, CHMC_E_STATE, "hmc_run: speculative batches did not advance the trajectory counter");
+    }
     return CHMC_OK;
 }
 
@@ -1321,6 +1378,13 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     if (hp->ntraj < 1 || hp->steps < 1 || !scheme_ok(hp->scheme) || hp->therm < 0)
         return fail(c, CHMC_E_ARG, "hmc_run: ntraj >= 1, steps >= 1, therm >= 0 and a known scheme are required");
     if (hp->surrogate && !c->surr_params) return fail(c, CHMC_E_STATE, "hmc_run: surrogate parameters not set");
+    const bool spec = c->spec > 1;
+    if (spec) {
+        if (hp->randomize || hp->surrogate)
+            return fail(c, CHMC_E_ARG, "hmc_run: speculative trajectories need fixed-length trajectories with LJ forces");
+        for (double sk : c->chain_skin)
+            if (sk != c->chain_skin[0]) return fail(c, CHMC_E_ARG, "hmc_run: speculative copies need one skin");
+    }
     if (list_md(c) && (rc = list_ensure(c))) return rc;          // allocations stay outside graph capture
     if (hp->surrogate && (rc = surr_geometry(c))) return rc;
     const int nch = c->nchains, ntraj = hp->ntraj;
@@ -1370,7 +1434,10 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     // ---- V_i, W_i of the start configuration (HMC.cpp:64), h_x = x (HMC.cpp:72) ----
     const bool slot_run = list_md(c);                          // the run starts on the slot-ordered state
     if (slot_run) {
-        if ((rc = list_import(c)) || (!c->list_valid && (rc = list_invalidate(c))) || (rc = list_energy(c, true))) return rc;
+        if ((rc = list_import(c)) || (!c->list_valid && (rc = list_invalidate(c)))) return rc;
+        // speculative copies start from chain 0's state (positions, lists, rebuild requests)
+        if (spec) HIPCHK(c, launch_spec_broadcast(c->v, 0, false, c->stream));
+        if ((rc = list_energy(c, true))) return rc;
         c->list_valid = true;
     } else if ((rc = do_energy(c, true))) {
         return rc;
@@ -1397,7 +1464,12 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     c->md_surr = hp->surrogate != 0;
     const int window = (c->md_surr && hp->fallback_window > 0) ? hp->fallback_window : ntraj;
     std::vector<HmcRecord> wrec;
-    for (int t0 = 0; t0 < ntraj && !rc; t0 += window) {
+    if (spec) {
+        // records of the chain in column 0, the other columns zero
+        HIPCHK(c, hipMemsetAsync(c->d_rec, 0, sizeof(HmcRecord) * nrec, c->stream));
+        rc = run_spec(c, *hp, ufirst, rec_base, rec_base + ntraj, graphs);
+    }
+    for (int t0 = 0; t0 < ntraj && !rc && !spec; t0 += window) {
         const int count = std::min(window, ntraj - t0);
         if (list_md(c) && !c->lstate) {          // LJ and surrogate MD both run on the slot-ordered state
             c->list_valid = false;
@@ -1420,6 +1492,20 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     if (rc) return rc;
     if ((rc = list_export(c, true))) return rc;
     c->saved_lists = false;
+    if (spec) {
+        // the chain's momenta after the run are the post-MD momenta of its last trajectory, which
+        // copy spec_ctl[2] ran (the positions are the same in every copy)
+        int ctl[4];
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(ctl, c->v.spec_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
+        const int last = ctl[2];
+        if (last > 0 && last < nch) {
+            const size_t Nt = (size_t)c->Ntot, n = (size_t)c->hchains[0].natoms;
+            for (int d = 0; d < 3; d++)
+                HIPCHK(c, hipMemcpyAsync(c->v.p + d * Nt + c->hchains[0].atom_off, c->v.p + d * Nt + c->hchains[last].atom_off,
+                                         sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        }
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (list_md(c) && (rc = list_check_overflow(c))) return rc;
     c->traj_done += ntraj;
@@ -2101,6 +2187,40 @@ extern "C" int chmc_set_list_lanes(chmc_ctx* c, int lanes)
         if (ms < best) { best = ms; pick = lp; }
     }
     c->v.lanes = pick;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_set_speculation(chmc_ctx* c, int copies)
+{
+    CHECK_CTX(c);
+    if (copies <= 1) {
+        c->spec = 0;
+        return CHMC_OK;
+    }
+    if (copies != c->nchains) return fail(c, CHMC_E_ARG, "set_speculation: the context must hold exactly `copies` chains");
+    if (!list_md(c))
+        return fail(c, CHMC_E_STATE, "set_speculation: speculative trajectories run on the FP32 neighbour-list MD only");
+    const chmc_chain_desc& d0 = c->desc[0];
+    for (int k = 1; k < c->nchains; k++) {
+        const chmc_chain_desc& d = c->desc[k];
+        if (d.N != d0.N || d.L[0] != d0.L[0] || d.L[1] != d0.L[1] || d.L[2] != d0.L[2] || d.cutoff != d0.cutoff ||
+            d.sigma != d0.sigma || d.eps != d0.eps || d.mass != d0.mass || d.beta != d0.beta)
+            return fail(c, CHMC_E_ARG, "set_speculation: the chains must be identical copies of one chain");
+    }
+    if (!c->v.spec_ctl) {
+        HIPCHK(c, dalloc(c, &c->v.spec_ctl, 4));
+        HIPCHK(c, dalloc(c, &c->v.spec_st, 1));
+        HIPCHK(c, hipMemset(c->v.spec_ctl, 0, 4 * sizeof(int)));
+        HIPCHK(c, hipMemset(c->v.spec_st, 0, sizeof(ChainState)));
+    }
+    c->spec = copies;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_get_speculation(const chmc_ctx* c, int* copies)
+{
+    if (!c || !copies) return CHMC_E_ARG;
+    *copies = c->spec > 1 ? c->spec : 1;
     return CHMC_OK;
 }
 

@@ -143,6 +143,11 @@ struct DevView {
     double lp_scale;      // 10^n_prec inside an lp_leapfrog / lp_omf4 integration, 0 otherwise
     double lcheck_dt;     // list MD with rebuild checks every m-th force evaluation: (m - 1) times the
                           // largest drift step of the integration (0: a check before every evaluation)
+    // speculative trajectories (the chains are copies of one chain): [0] accepted copy of the last
+    // batch (-1: none), [1] trajectories it consumed, [2] copy of its last consumed trajectory; the
+    // chain state after the batch, committed by the broadcast
+    int* spec_ctl;
+    ChainState* spec_st;
 };
 
 // The updates of lp_leapfrog / lp_omf4 (code-carsten/integrator.hh:51-79, 156-197) round the updated
@@ -241,12 +246,22 @@ hipError_t launch_lj(const DevView& v, int mode, bool fp64, int tiling, double k
 hipError_t launch_kinetic(const DevView& v, hipStream_t s);
 // slots: write the momenta in the current list buffer's slot order (pl) instead of atom order (p)
 hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, bool slots, hipStream_t s);
+// speculative batch: chain k draws the momenta of trajectory (device counter + 1 + k) with chain 0's key
+hipError_t launch_heatbath_spec(const DevView& v, uint64_t seed, hipStream_t s);
 hipError_t launch_metropolis(const DevView& v, const double* uniforms, int ustride, int therm, int ufirst,
                              HmcRecord* rec, int rec_base, int surr, int what, hipStream_t s);
 // list_q >= 0: neighbour-list MD at that list parity; saves and restores also switch the list buffers
 hipError_t launch_save_restore(const DevView& v, int mode, int list_q, hipStream_t s);
 // the same on the slot-ordered state (xl, xl_saved) of list MD, at list parity list_q
 hipError_t launch_save_restore_slots(const DevView& v, int mode, int list_q, hipStream_t s);
+// speculative batch (the chains are copies of one chain, each ran the next trajectory in turn): the
+// Metropolis decisions in copy order up to the first accepted one or trajectory tend (records of the
+// consumed trajectories in column 0), save / restore per copy; then the commit of the counter and the
+// chain state and the copy of the accepted copy's state into the others (src >= 0: copy chain src
+// without a commit)
+hipError_t launch_spec_restore(const DevView& v, const double* uniforms, int therm, int ufirst, HmcRecord* rec,
+                               int rec_base, int tend, int list_q, hipStream_t s);
+hipError_t launch_spec_broadcast(const DevView& v, int src, bool commit, hipStream_t s);
 // x, p (and x_saved from xl_saved) <-> xl, pl in the current list buffer's slot order
 hipError_t launch_list_import(const DevView& v, hipStream_t s);
 hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s);

@@ -47,13 +47,17 @@ __global__ __launch_bounds__(256) void k_kinetic(DevView v)
 
 // heat bath p ~ N(0, m/beta) (identical_particles.cpp:287-296), K_i partials of the fresh momenta and,
 // optionally, the count phase of the binning of the unchanged positions.  With slots the chunk runs
-// over list slots: the draw of the slot's atom (the same Philox key) goes to pl in slot order.
-__global__ __launch_bounds__(256) void k_heatbath(DevView v, uint64_t seed, long long traj_explicit, int count, int slots)
+// over list slots: the draw of the slot's atom (the same Philox key) goes to pl in slot order.  With
+// spec the chains are copies of chain 0 running its next trajectories: copy k draws the momenta of
+// trajectory (counter + 1 + k) with chain 0's key.
+__global__ __launch_bounds__(256) void k_heatbath(DevView v, uint64_t seed, long long traj_explicit, int count, int slots,
+                                                  int spec)
 {
     const Chunk ck = v.chunks[blockIdx.x];
     const int ch = ck.chain;
     const ChainDev& C = v.chains[ch];
-    const uint64_t traj = traj_explicit >= 0 ? (uint64_t)traj_explicit : (uint64_t)(*v.traj + 1);
+    const uint64_t traj = traj_explicit >= 0 ? (uint64_t)traj_explicit : (uint64_t)(*v.traj + 1 + (spec ? ch : 0));
+    const uint64_t key_chain = spec ? 0 : (uint64_t)ch;
     const double sd = sqrt(C.mass / C.beta);
     const double twopi = 6.283185307179586;
     const int N = v.Ntot;
@@ -63,7 +67,7 @@ __global__ __launch_bounds__(256) void k_heatbath(DevView v, uint64_t seed, long
     if (slots && ck.begin == C.atom_off && threadIdx.x == 0) *list_pord(v, ch) = -1;   // pl in the current order
     for (int i = ck.begin + (int)threadIdx.x; i < ck.end; i += 256) {
         const int a = slots ? lperm[i] : i;       // i: the atom, or the slot of atom a
-        const u64x4 r = philox4x64_10((uint64_t)(a - C.atom_off), traj, (uint64_t)ch, 0, seed, kKeyHeatbath);
+        const u64x4 r = philox4x64_10((uint64_t)(a - C.atom_off), traj, key_chain, 0, seed, kKeyHeatbath);
         const double r0 = sqrt(-2.0 * log(u01_oo(r.v[0])));
         const double a0 = twopi * u01_co(r.v[1]);
         const double r1 = sqrt(-2.0 * log(u01_oo(r.v[2])));
@@ -211,15 +215,9 @@ __global__ __launch_bounds__(256) void k_save_restore(DevView v, int mode, int l
 // The same on the slot-ordered state of list MD: xl_saved <- xl on save (the current list buffer becomes
 // the start buffer), xl <- xl_saved on restore (the start buffer becomes current again, and its FP32
 // coordinates of parity list_q are recomputed for the restored positions)
-__global__ __launch_bounds__(256) void k_save_restore_slots(DevView v, int mode, int list_q)
+__device__ __forceinline__ void slot_save_restore(const DevView& v, int s, int ch, bool save, int list_q)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (mode == 2 && s == 0) *v.traj = *v.traj + 1;
-    if (s >= v.Ntot) return;
-    const int ch = v.atom_chain[s];               // a chain's slots span the same range as its atoms
     const ChainDev& C = v.chains[ch];
-    int save = mode == 0;
-    if (mode == 2) save = v.state[ch].accepted;
     const int N = v.Ntot;
     if (save) {
 #pragma unroll
@@ -248,6 +246,165 @@ __global__ __launch_bounds__(256) void k_save_restore_slots(DevView v, int mode,
     L.xsl[(size_t)list_q * N + s] = make_float4(f3[0], f3[1], f3[2], 0.f);
 }
 
+__global__ __launch_bounds__(256) void k_save_restore_slots(DevView v, int mode, int list_q)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == 2 && s == 0) *v.traj = *v.traj + 1;
+    if (s >= v.Ntot) return;
+    const int ch = v.atom_chain[s];               // a chain's slots span the same range as its atoms
+    const bool save = mode == 2 ? v.state[ch].accepted != 0 : mode == 0;
+    slot_save_restore(v, s, ch, save, list_q);
+}
+
+// ---- speculative trajectories: the context's chains are K identical copies of one chain ----------
+// Copy k runs trajectory t0 + 1 + k (t0 = the device counter) from the chain's state after t0.  The
+// sequential chain would run trajectory t0 + 1 + k from that same state exactly when trajectories
+// t0 + 1 .. t0 + k were all rejected, so the copies are tested in order with the Metropolis step of
+// k_metropolis against the state after t0; the first accepted copy closes the batch (the copies after
+// it started from a state the chain no longer holds and are discarded), and so does the run's last
+// trajectory tend.  Every workgroup evaluates the same decision from the same inputs; workgroup 0
+// writes the records, the control words and the chain state after the batch, which the broadcast
+// commits (the kernel that reads the counter and the state never writes them).
+//   spec_ctl[0]: copy whose end state the chain takes (-1: none accepted; every copy restores the
+//                start state), [1]: trajectories the batch consumed, [2]: copy of the last consumed
+//                trajectory (its post-MD momenta are the chain's at the end of a run)
+struct SpecDecision {
+    int j, consumed;
+    ChainState st;
+};
+
+__device__ SpecDecision spec_decide(const DevView& v, const double* __restrict__ uniforms, int therm, int ufirst, int tend,
+                                    HmcRecord* __restrict__ rec, int rec_base)
+{
+    SpecDecision d;
+    d.j = -1;
+    d.consumed = 0;
+    ChainState st = v.state[0];
+    const int t0 = *v.traj;
+    for (int k = 0; k < v.nchains && t0 + 1 + k <= tend; k++) {
+        const int i = t0 + 1 + k;
+        const ChainDev& C = v.chains[k];
+        const double* r = &v.red[k * RED_NSLOT];
+        const double Ki = r[RED_KI], Kf = r[RED_K], Vf = r[RED_V], Wf = r[RED_W];
+        const double dh = C.beta * (Kf + Vf - Ki - st.Vi);
+        int acc;
+        if (i < therm) {
+            acc = 1;
+        } else {
+            const double u = uniforms[i - ufirst];
+            acc = (u < exp(-dh)) ? 1 : 0;
+            st.nacc += acc;
+        }
+        if (acc) { st.Vi = Vf; st.Wi = Wf; }
+        st.Ki = Ki;
+        st.accepted = acc;
+        d.consumed = k + 1;
+        if (rec) {
+            HmcRecord h;
+            h.dh = dh; h.u = st.Vi; h.k = acc ? Kf : Ki; h.w = st.Wi; h.acc = acc; h.surr = 0;
+            rec[(size_t)(i - 1 - rec_base) * v.nchains] = h;
+        }
+        if (acc) {
+            d.j = k;
+            break;
+        }
+    }
+    d.st = st;
+    return d;
+}
+
+// the decision (the per-chain sums in red[] are those of k_reduce over the energy's tile partials and
+// the heat bath's chunk partials), then per slot: the accepted copy keeps its end state as its start
+// state, every other copy restores its start state
+__global__ __launch_bounds__(256) void k_spec_restore(DevView v, const double* __restrict__ uniforms, int therm, int ufirst,
+                                                      HmcRecord* __restrict__ rec, int rec_base, int tend, int list_q)
+{
+    __shared__ int sj;
+    if (threadIdx.x == 0) {
+        const bool lead = blockIdx.x == 0;
+        const SpecDecision d = spec_decide(v, uniforms, therm, ufirst, tend, lead ? rec : nullptr, rec_base);
+        sj = d.j;
+        if (lead) {
+            v.spec_ctl[0] = d.j;
+            v.spec_ctl[1] = d.consumed;
+            if (d.consumed > 0) v.spec_ctl[2] = d.consumed - 1;
+            *v.spec_st = d.st;
+        }
+    }
+    __syncthreads();
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.Ntot) return;
+    const int ch = v.atom_chain[s];
+    slot_save_restore(v, s, ch, ch == sj, list_q);
+}
+
+// commit: advance the counter by the consumed trajectories and give every copy the chain state after
+// the batch; then, when a copy j was accepted (or src >= 0 names one), copy its state into every other
+// copy: positions and start positions by slot and its current list buffer (rows, counts, slot <-> atom
+// maps, rebuild references, FP32 coordinates of both parities, cell offsets, rebuild requests), with
+// slot and atom numbers moved to the target copy's range, so that every copy continues exactly as
+// copy j would.  The momenta are not copied: the next heat bath draws them.
+__global__ __launch_bounds__(256) void k_spec_broadcast(DevView v, int src, int commit)
+{
+    if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
+        *v.traj += v.spec_ctl[1];
+        const ChainState st = *v.spec_st;
+        for (int k = 0; k < v.nchains; k++) v.state[k] = st;
+    }
+    const int j = src >= 0 ? src : v.spec_ctl[0];
+    if (j < 0) return;
+    const ChainDev& Cj = v.chains[j];
+    const int b = *list_cur(v, j);
+    const ListBuf L = list_buf(v, b);
+    const int Nt = v.Ntot;
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    int* __restrict__ lreq = v.lreq;
+    for (size_t s = gid; s < (size_t)Nt; s += stride) {
+        const int k = v.atom_chain[s];
+        if (k == j) continue;
+        const ChainDev& Ck = v.chains[k];
+        const int r = (int)s - Ck.atom_off, sj = Cj.atom_off + r, delta = Ck.atom_off - Cj.atom_off;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            v.xl[(size_t)d * Nt + s] = v.xl[(size_t)d * Nt + sj];
+            v.xl_saved[(size_t)d * Nt + s] = v.xl_saved[(size_t)d * Nt + sj];
+        }
+        L.nbn[s] = L.nbn[sj];
+        L.lperm[s] = L.lperm[sj] + delta;
+        L.linv[s] = L.linv[sj] + delta;          // atom Ck.atom_off + r: a chain's atoms and slots share a range
+        L.lref[s] = L.lref[sj];
+        L.xsl[s] = L.xsl[sj];
+        L.xsl[(size_t)Nt + s] = L.xsl[(size_t)Nt + sj];
+        tile_cur_mark(v, Ck, (int)s, b);
+        if (r == 0) {
+            *list_cur(v, k) = b;
+            *list_start(v, k) = b;
+            *list_pord(v, k) = -1;
+            for (int q = 0; q < 2; q++) lreq[q * (v.nchains + 1) + 1 + k] = lreq[q * (v.nchains + 1) + 1 + j];
+        }
+    }
+    // list rows, four entries per thread, up to the padded span or the count
+    const int q4 = v.nbcap / 4, span = min(kListSpan, v.nbcap);
+    for (size_t u = gid; u < (size_t)Nt * q4; u += stride) {
+        const int s = (int)(u / q4), c4 = (int)(u % q4);
+        const int k = v.atom_chain[s];
+        if (k == j) continue;
+        const ChainDev& Ck = v.chains[k];
+        const int sj = Cj.atom_off + (s - Ck.atom_off);
+        if (4 * c4 >= max(L.nbn[sj], span)) continue;
+        const int shift = (Ck.atom_off - Cj.atom_off) * (1 << kImageBits);
+        int4 e = reinterpret_cast<const int4*>(L.nb + (size_t)sj * v.nbcap)[c4];
+        e.x += shift; e.y += shift; e.z += shift; e.w += shift;
+        reinterpret_cast<int4*>(L.nb + (size_t)s * v.nbcap)[c4] = e;
+    }
+    for (size_t c = gid; c < (size_t)v.Cl; c += stride) {
+        const int k = v.lcell_chain[c];
+        if (k == j) continue;
+        const ChainDev& Ck = v.chains[k];
+        L.loffs[c] = L.loffs[c - Ck.lcell_off + Cj.lcell_off] + (Ck.atom_off - Cj.atom_off);
+    }
+}
+
 // Standalone kick with the stored force: p -= (dt*beta) f  (identical_particles.cpp:686-688)
 __global__ __launch_bounds__(256) void k_kick(DevView v, double dt, int fp64)
 {
@@ -265,7 +422,13 @@ __global__ __launch_bounds__(256) void k_kick(DevView v, double dt, int fp64)
 
 hipError_t launch_heatbath(const DevView& v, uint64_t seed, long long traj_or_neg, bool count, bool slots, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_heatbath, dim3(v.nchunks), dim3(256), 0, s, v, seed, traj_or_neg, count ? 1 : 0, slots ? 1 : 0);
+    hipLaunchKernelGGL(k_heatbath, dim3(v.nchunks), dim3(256), 0, s, v, seed, traj_or_neg, count ? 1 : 0, slots ? 1 : 0, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_heatbath_spec(const DevView& v, uint64_t seed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_heatbath, dim3(v.nchunks), dim3(256), 0, s, v, seed, -1LL, 0, 1, 1);
     return hipGetLastError();
 }
 
@@ -298,6 +461,20 @@ hipError_t launch_save_restore(const DevView& v, int mode, int list_q, hipStream
 hipError_t launch_save_restore_slots(const DevView& v, int mode, int list_q, hipStream_t s)
 {
     hipLaunchKernelGGL(k_save_restore_slots, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, mode, list_q);
+    return hipGetLastError();
+}
+
+hipError_t launch_spec_restore(const DevView& v, const double* uniforms, int therm, int ufirst, HmcRecord* rec,
+                               int rec_base, int tend, int list_q, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spec_restore, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, uniforms, therm, ufirst, rec,
+                       rec_base, tend, list_q);
+    return hipGetLastError();
+}
+
+hipError_t launch_spec_broadcast(const DevView& v, int src, bool commit, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spec_broadcast, dim3(1024), dim3(256), 0, s, v, src, commit ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -283,6 +283,19 @@ int chmc_get_list_lanes(const chmc_ctx* ctx, int* lanes);
  * the flags fail with CHMC_E_STATE.  1: the check before every evaluation.  No reference counterpart. */
 int chmc_set_list_check(chmc_ctx* ctx, int every);
 int chmc_get_list_check(const chmc_ctx* ctx, int* every);
+/* speculative trajectories: a context created with `copies` identical chains (same descriptor, one
+ * skin, FP32 neighbour-list MD) runs chmc_hmc_run as ONE chain, chain 0.  Each batch runs the chain's
+ * next `copies` trajectories at once, all from its current state (copy k: trajectory t + 1 + k, heat
+ * bath and Metropolis uniform of that trajectory); the Metropolis tests then take them in order up to
+ * the first accepted one, whose end state every copy takes over, and the later ones are run again in
+ * the next batch.  A trajectory of the sequential loop starts from the same state exactly when the
+ * ones before it were rejected, so records, positions, momenta and acceptance equal the single-chain
+ * run's bit for bit; where acceptance is low, nearly every batch keeps all `copies` trajectories.
+ * chmc_hmc_run then writes the chain's records in column 0 (the other columns zero) and its count in
+ * nacc[0]; randomize and surrogate runs are refused.  copies <= 1 switches it off.  No reference
+ * counterpart (HMC.cpp:77-117 is the sequential loop this reproduces). */
+int chmc_set_speculation(chmc_ctx* ctx, int copies);
+int chmc_get_speculation(const chmc_ctx* ctx, int* copies);
 
 #ifdef __cplusplus
 }
