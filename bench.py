@@ -43,18 +43,23 @@ PEAK_HBM_GBS = 8000.0
 # list sweep bytes per atom besides its list row: own FP32 coordinates 16, entry count 4, p 24 + 24 and
 # x 24 + 24 (read and written, slot order), rebuild reference 16, next FP32 coordinates 16
 SWEEP_BYTES_PER_ATOM = 148.0
-# HBM traffic of one list sweep (force + kick + drift) measured with rocprofv3 --pmc FETCH_SIZE and
-# --pmc WRITE_SIZE (medians over 2850 dispatches, KiB; a plain copy reads 1.00 for WRITE_SIZE) at
-# N=32768, skins 0.6 and 0.7, 8 lanes per atom (profiles/lj_list_sweep_N32768.md), keyed by (skin, lanes,
-# speculative copies in the launch): reported as the roofline's traffic when the run's tuned skin, lanes
-# and copies are those, null otherwise
-SWEEP_TRAFFIC_PROFILED = {(0.6, 8, 1): (9976.0 + 2048.0) * 1024.0, (0.7, 8, 1): (9970.0 + 2048.0) * 1024.0,
-                          (0.7, 816, 1): (10086.8 + 2048.0) * 1024.0}
+# HBM traffic of one list sweep (force + kick + drift) from rocprofv3 --pmc FETCH_SIZE and --pmc
+# WRITE_SIZE (medians over the profiled dispatches, KiB).  Calibration (--workload calib, cold 1 GiB
+# device copies): FETCH_SIZE reads 524,300 KiB for 1,048,576 KiB read, WRITE_SIZE 1,048,576 KiB for as
+# many written, so FETCH_SIZE is scaled by FETCH_SCALE and WRITE_SIZE taken as it is
+# (profiles/hbm_counter_calibration.md).  Keyed by (skin, lanes, speculative copies in the launch) at
+# N=32768 (profiles/lj_list_sweep_N32768.md, profiles/lj_list_sweep_spec8_N32768.md): reported as the
+# roofline's traffic when the run's tuned skin, lanes and copies are those, null otherwise
+FETCH_SCALE = 2.0
+SWEEP_TRAFFIC_PROFILED = {(0.6, 8, 1): (FETCH_SCALE * 9976.0 + 2048.0) * 1024.0,
+                          (0.7, 8, 1): (FETCH_SCALE * 9970.0 + 2048.0) * 1024.0,
+                          (0.7, 816, 1): (FETCH_SCALE * 10086.8 + 2048.0) * 1024.0,
+                          (0.4, 812, 8): (FETCH_SCALE * 60498.2 + 16384.2) * 1024.0}
 # the same at N=1,048,576 (workload c5), skin 0.5, 8 lanes (profiles/lj_list_sweep_N1048576.md)
-SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8): (322982.0 + 65536.0) * 1024.0}
+SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8): (FETCH_SCALE * 322982.0 + 65536.0) * 1024.0}
 # the same for the 256-chain batch (workload c3) at the per-chain skins base 0.4, exponent 1.5 and 4
 # lanes per atom (profiles/lj_list_sweep_c3_batch.md), keyed by the skin family the run uses
-SWEEP_TRAFFIC_PROFILED_C3 = {("0.4/1.5", 4): (115006.0 + 38233.0) * 1024.0}
+SWEEP_TRAFFIC_PROFILED_C3 = {("0.4/1.5", 4): (FETCH_SCALE * 115006.0 + 38233.0) * 1024.0}
 
 
 def _omp_threads(n):
@@ -445,6 +450,20 @@ def kernels_workload(args):
     print(json.dumps(out))
 
 
+def calib_workload(args):
+    """Calibration of the HBM counters: four device-to-device copies of 1 GiB, each source written long
+    before (the other buffers evict it from the L2 and the MALL), so every byte is read from HBM once and
+    written once; profiled with rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE."""
+    import torch
+    n = 1 << 28                                    # FP32 elements: 1 GiB per buffer
+    bufs = [torch.full((n,), float(i + 1), device="cuda") for i in range(4)]
+    torch.cuda.synchronize()
+    for src, dst in ((0, 1), (2, 3), (1, 0), (3, 2)):
+        bufs[dst].copy_(bufs[src])
+        torch.cuda.synchronize()
+    print(json.dumps({"workload": "calib", "copies": 4, "bytes_per_copy": 4 * n}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -470,10 +489,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every trajectory directly instead of replaying the captured graph")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c5s", "kernels"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c5s", "kernels", "calib"],
                     help="c2 (default, the headline line), c3 batched chains, c4 surrogate vs plain, c5 N=1M, "
                          "c5s the surrogate fine-tuned and run at N=1M, kernels: every hot-path kernel a few "
-                         "times at N=32768 (for kernel traces)")
+                         "times at N=32768 (for kernel traces), calib: 1 GiB device copies (HBM counter calibration)")
     ap.add_argument("--reps", type=int, default=20, help="kernels: repetitions of each kernel group")
     ap.add_argument("--train-steps", type=int, default=4000, help="c4, c5s: Adam steps of the surrogate")
     ap.add_argument("--train-configs", type=int, default=C4_NCONF, help="c4: training snapshots of the N=32768 chain")
@@ -491,6 +510,9 @@ def main():
 
     if args.workload == "kernels":
         kernels_workload(args)
+        return
+    if args.workload == "calib":
+        calib_workload(args)
         return
     if args.workload == "c4":
         surrogate_workload(args)
