@@ -179,6 +179,7 @@ extern "C" int chmc_create(chmc_ctx** out, int nchains, const chmc_chain_desc* c
         cd.rc = d.cutoff; cd.rc2 = d.cutoff * d.cutoff;
         cd.sigma = d.sigma; cd.sigma2 = d.sigma * d.sigma;
         cd.eps = d.eps; cd.mass = d.mass; cd.beta = d.beta;
+        cd.coeff_x = cd.beta / cd.mass;
         cd.rc2_f = (float)cd.rc2; cd.sigma2_f = (float)cd.sigma2; cd.eps48_f = (float)(48 * d.eps);
         cd.chunk_first = (int)c->hchunks.size();
         for (int b = 0; b < d.N; b += kRedChunk) c->hchunks.push_back(Chunk{ k, cd.atom_off + b, cd.atom_off + std::min(d.N, b + kRedChunk), 0 });
@@ -714,6 +715,8 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
         const double half = std::max(0.0, 0.5 * sk - 4.0e-7 * lmax);
         cd.half_skin2 = half * half;
         cd.p2max = 100.0 * cd.mass / cd.beta;      // |p| <= 10 sqrt(m / beta): 10 thermal standard deviations
+        cd.half_skin = std::sqrt(cd.half_skin2);
+        cd.vmax = cd.coeff_x * std::sqrt(cd.p2max);
         cd.rl2_f = (float)(rl * rl * (1.0 + 1e-5));   // a margin over the FP32 rounding of r^2 at the rebuild
         // capacities from a local density bound of max(1.5 / sigma^3, 1.5 x the chain's mean density)
         const double rho = cd.natoms / (cd.L[0] * cd.L[1] * cd.L[2]) * cd.sigma * cd.sigma * cd.sigma;

@@ -58,6 +58,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace chmc {
 
@@ -74,6 +75,8 @@ struct ChainDev {
     double lsize[3], half_skin2;   // list cell side, (skin / 2)^2
     float  lsize_f[3], rl2_f;      // list radius^2 = (rc + skin)^2
     double p2max;                  // |p|^2 bound of the lists' check interval: (10 sqrt(m / beta))^2
+    double coeff_x;                // beta / m, the drift coefficient (compute_coeff_position)
+    double half_skin, vmax;        // sqrt(half_skin2); coeff_x * sqrt(p2max), the largest drift speed
 };
 
 // per-chain reduction slots
@@ -155,6 +158,52 @@ struct DevView {
 // bit-reversible scheme of Levesque & Verlet 1993); positions are rounded before the wrap.  s == 0:
 // the plain update.
 __device__ __forceinline__ double lp_round(double val, double s) { return s != 0.0 ? round(val * s) / s : val; }
+
+// Lane exchanges inside a row of 16 lanes through DPP (a VALU move, no LDS round trip): quad_perm
+// controls (two bits per lane of a quad), half-row mirror (lane i <- 7 - i) and row mirror (i <- 15 - i)
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+constexpr int kDppFrom1 = 0xE5, kDppFrom2 = 0xE6;     // lane 0 of a quad <- lane 1 / lane 2 (others keep their own)
+
+// Sum over aligned groups of G lanes (G = 4, 8, 16), in every lane of the group: the xor butterfly
+// (steps 1, 2, 4, 8), whose third and fourth steps may take the mirrored lane instead because after the
+// previous steps every lane of a quad (half-row) holds the same bits -- additions commute exactly
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v)
+{
+    auto x = [](T a, auto ctrl) {
+        constexpr int c = decltype(ctrl)::value;
+        if constexpr (sizeof(T) == 8) return dpp_d<c>(a);
+        else return dpp_f<c>(a);
+    };
+    static_assert(G == 4 || G == 8 || G == 16, "group of 4, 8 or 16 lanes");
+    v += x(v, std::integral_constant<int, kDppXor1>{});
+    v += x(v, std::integral_constant<int, kDppXor2>{});
+    if constexpr (G >= 8) v += x(v, std::integral_constant<int, kDppHalfMirror>{});
+    if constexpr (G >= 16) v += x(v, std::integral_constant<int, kDppMirror>{});
+    return v;
+}
+
+// the periodic wrap x - L floor(x / L) of update_positions (identical_particles.cpp:662-666), with the
+// division skipped where it cannot change x: for 0 < x < L the exact quotient is at most 1 - ulp(L) / L,
+// more than half a unit below 1, so it rounds below 1, floor gives 0 and the formula returns x itself
+__device__ __forceinline__ double wrap_box(double x, double L)
+{
+#pragma clang fp contract(off)
+    return (x > 0.0 && x < L) ? x : x - L * floor(x / L);
+}
 
 // LJ_REL64 (list sweep without drift): also the FP64 cell-relative positions of the energy (rel64)
 enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8, LJ_DRIFT = 16, LJ_REL64 = 32 };

@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void k_count(DevView v, double drift_dt, int d
     if (a < v.Ntot) {
         const ChainDev& C = v.chains[v.atom_chain[a]];
         const int N = v.Ntot;
-        const double coeff_x = C.beta / C.mass;       // compute_coeff_position, identical_particles.cpp:646-648
+        const double coeff_x = C.coeff_x;              // compute_coeff_position, identical_particles.cpp:646-648
         double x3[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_count(DevView v, double drift_dt, int d
             if (do_drift) {
                 xv += drift_dt * coeff_x * v.p[d * N + a];
                 xv = lp_round(xv, v.lp_scale);
-                xv -= C.L[d] * floor(xv / C.L[d]);
+                xv = wrap_box(xv, C.L[d]);
                 v.x[d * N + a] = xv;
             }
             x3[d] = xv;
@@ -62,13 +62,13 @@ __global__ __launch_bounds__(256) void k_drift(DevView v, double dt)
     if (a >= v.Ntot) return;
     const ChainDev& C = v.chains[v.atom_chain[a]];
     const int N = v.Ntot;
-    double coeff_x = C.beta / C.mass;
+    double coeff_x = C.coeff_x;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         double xv = v.x[d * N + a];
         xv += dt * coeff_x * v.p[d * N + a];
         xv = lp_round(xv, v.lp_scale);
-        xv -= C.L[d] * floor(xv / C.L[d]);
+        xv = wrap_box(xv, C.L[d]);
         v.x[d * N + a] = xv;
     }
 }

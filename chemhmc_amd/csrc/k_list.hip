@@ -525,7 +525,7 @@ hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s)
 __device__ __forceinline__ double trigger2(const DevView& v, const ChainDev& C)
 {
     if (v.lcheck_dt == 0.0) return C.half_skin2;
-    const double h = sqrt(C.half_skin2) - v.lcheck_dt * (C.beta / C.mass) * sqrt(C.p2max);
+    const double h = C.half_skin - v.lcheck_dt * C.vmax;
     return h > 0.0 ? h * h : -1.0;
 }
 
@@ -533,7 +533,7 @@ __device__ __forceinline__ void list_drift_slot(const DevView& v, const ListBuf&
                                                 int slot, int q, double dt, const double pn[3])
 {
     const int N = v.Ntot;
-    const double coeff_x = C.beta / C.mass;       // compute_coeff_position, identical_particles.cpp:646-648
+    const double coeff_x = C.coeff_x;              // compute_coeff_position, identical_particles.cpp:646-648
     float f3[3];
     double disp2 = 0.0;
     const float4 ref = L.lref[slot];
@@ -542,7 +542,7 @@ __device__ __forceinline__ void list_drift_slot(const DevView& v, const ListBuf&
         double xv = v.xl[d * N + slot];
         xv += dt * coeff_x * pn[d];
         xv = lp_round(xv, v.lp_scale);
-        xv -= C.L[d] * floor(xv / C.L[d]);
+        xv = wrap_box(xv, C.L[d]);
         v.xl[d * N + slot] = xv;
         const double r = list_rel(C, ref, d, xv);
         const double e = r - (double)ref_coord(ref, d);
@@ -577,11 +577,11 @@ __device__ __forceinline__ void list_tail(const DevView& v, const ListBuf& L, co
             v.pl[d * N + s] = pn;
         }
         if constexpr ((MODE & LJ_DRIFT) != 0) {
-            const double coeff_x = C.beta / C.mass;    // compute_coeff_position, identical_particles.cpp:646-648
+            const double coeff_x = C.coeff_x;          // compute_coeff_position, identical_particles.cpp:646-648
             double xv = x_d;
             xv += drift_dt * coeff_x * pn;
             xv = lp_round(xv, v.lp_scale);
-            xv -= C.L[d] * floor(xv / C.L[d]);
+            xv = wrap_box(xv, C.L[d]);
             v.xl[d * N + s] = xv;
             const double r = list_rel(C, ref, d, xv);
             e = r - (double)ref_coord(ref, d);
@@ -590,14 +590,14 @@ __device__ __forceinline__ void list_tail(const DevView& v, const ListBuf& L, co
         if constexpr ((MODE & LJ_REL64) != 0) v.rel64[4 * (size_t)s + d] = list_rel(C, ref, d, x_d);
     }
     if constexpr ((MODE & LJ_DRIFT) != 0) {
-        const int base = threadIdx.x & ~(LP - 1) & 63;
-        const float ry = __shfl(rel, base + 1, 64), rz = __shfl(rel, base + 2, 64);
-        const double ey = __shfl(e, base + 1, 64), ez = __shfl(e, base + 2, 64);
-        const double py = __shfl(pn, base + 1, 64), pz = __shfl(pn, base + 2, 64);
+        // lane 0 of the slot gathers the y, z coordinates and the squared displacement and momentum over
+        // lanes 0-2 (lane 3 adds zeros: (ex^2 + ey^2) + (ez^2 + 0) is ex^2 + ey^2 + ez^2 to the bit)
+        const float ry = dpp_f<kDppFrom1>(rel), rz = dpp_f<kDppFrom2>(rel);
+        const double e2 = group_sum<4>(e * e), p2 = group_sum<4>(pn * pn);
         if (val && sub == 0) {
             L.xsl[(size_t)(q ^ 1) * N + s] = make_float4(rel, ry, rz, 0.f);
-            if (e * e + ey * ey + ez * ez > trigger2(v, C)) request(v, q ^ 1, tl.chain);
-            if (v.lcheck_dt != 0.0 && pn * pn + py * py + pz * pz > C.p2max) atomicOr(&v.lflag[2], 4);
+            if (e2 > trigger2(v, C)) request(v, q ^ 1, tl.chain);
+            if (v.lcheck_dt != 0.0 && p2 > C.p2max) atomicOr(&v.lflag[2], 4);
         }
     }
     if (threadIdx.x == 0) {
@@ -698,12 +698,9 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
             pair(xq[(unsigned)ee >> kImageBits], ee & 63);
         }
     }
-#pragma unroll
-    for (int m = 1; m < LP; m <<= 1) {
-        ax += __shfl_xor(ax, m, 64);
-        ay += __shfl_xor(ay, m, 64);
-        az += __shfl_xor(az, m, 64);
-    }
+    ax = group_sum<LP>(ax);
+    ay = group_sum<LP>(ay);
+    az = group_sum<LP>(az);
     list_tail<MODE, LP>(v, L, C, tl, s, val, sub, d, atom, C.eps48_f * (d == 0 ? ax : (d == 1 ? ay : az)), p_d, x_d, ref,
                         kick_dt, drift_dt, q, checked);
 }
@@ -813,12 +810,9 @@ __global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const flo
             if (two) pair_force(pd1, g1);
         }
     }
-#pragma unroll
-    for (int m = 1; m < LP; m <<= 1) {
-        ax += __shfl_xor(ax, m, 64);
-        ay += __shfl_xor(ay, m, 64);
-        az += __shfl_xor(az, m, 64);
-    }
+    ax = group_sum<LP>(ax);
+    ay = group_sum<LP>(ay);
+    az = group_sum<LP>(az);
     list_tail<MODE, LP>(v, L, C, tl, s, val, sub, d, atom, d == 0 ? ax : (d == 1 ? ay : az), p_d, x_d, ref, kick_dt,
                         drift_dt, q, checked);
 }

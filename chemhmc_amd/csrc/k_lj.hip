@@ -246,14 +246,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_lj(DevView v, double kick_dt, d
                     if constexpr ((MODE & LJ_DRIFT) != 0) {
                         // the following update_positions (identical_particles.cpp:651-669) of this atom,
                         // then the count phase of the next binning at its new position
-                        const double coeff_x = C.beta / C.mass;
+                        const double coeff_x = C.coeff_x;
                         double x3[3];
 #pragma unroll
                         for (int d = 0; d < 3; d++) {
                             double xv = v.x[d * N + atom];
                             xv += drift_dt * coeff_x * pn[d];
                             xv = lp_round(xv, v.lp_scale);
-                            xv -= C.L[d] * floor(xv / C.L[d]);
+                            xv = wrap_box(xv, C.L[d]);
                             v.x[d * N + atom] = xv;
                             x3[d] = xv;
                         }
