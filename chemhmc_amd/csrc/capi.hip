@@ -38,6 +38,8 @@ struct chmc_ctx {
     int lp_exponent = 16;        // n_prec of the lp_* integrators (lp_leapfrog() : n_prec(16))
     int traj_done = 0;           // trajectories completed on the device counter
     int spec = 0;                // > 1: the chains are that many copies of one chain, run speculatively
+    bool skip_next_check = false; // the next checked list evaluation's rebuild kernels already ran (a
+                                  // trajectory's start check, before its save)
     std::string err;
     double* d_uniforms = nullptr;
     size_t uniforms_cap = 0;
@@ -875,10 +877,22 @@ static int list_invalidate(chmc_ctx* c)
 
 // rel64 (without fuse_drift and storef): the sweep also writes the FP64 positions the energy reads.
 // checked = false skips the rebuild check (the caller's check interval guarantees valid lists)
+// the rebuild kernels of a checked evaluation, unless a trajectory's start check just ran them
+static int list_check_rebuild(chmc_ctx* c)
+{
+    if (c->skip_next_check) {
+        c->skip_next_check = false;
+        return CHMC_OK;
+    }
+    HIPCHK(c, list_rebuild(c));
+    return CHMC_OK;
+}
+
 static int list_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift, double drift_dt, bool rel64 = false,
                            bool checked = true)
 {
-    if (checked) HIPCHK(c, list_rebuild(c));
+    int rc;
+    if (checked && (rc = list_check_rebuild(c))) return rc;
     const bool r64 = rel64 && !fuse_drift && !storef;
     const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0) | (fuse_drift ? LJ_DRIFT : 0) | (r64 ? LJ_REL64 : 0);
     HIPCHK(c, launch_list_sweep(c->v, mode, c->lpar, dt, fuse_drift ? drift_dt : 0.0, checked, c->stream));
@@ -895,7 +909,8 @@ static int list_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift,
 static int list_surr_force_kick(chmc_ctx* c, double dt, bool storef, bool fuse_drift, double drift_dt, bool rel64 = false,
                                 bool checked = true)
 {
-    if (checked) HIPCHK(c, list_rebuild(c));
+    int rc;
+    if (checked && (rc = list_check_rebuild(c))) return rc;
     const bool r64 = rel64 && !fuse_drift && !storef;
     const int mode = LJ_FORCE | LJ_KICK | (storef ? LJ_STOREF : 0) | (fuse_drift ? LJ_DRIFT : 0) | (r64 ? LJ_REL64 : 0);
     HIPCHK(c, launch_lsurr_desc_mlp(c->v, c->sv, c->lpar, c->sg, nullptr, nullptr, c->stream));
@@ -1121,8 +1136,16 @@ extern "C" int chmc_save_positions(chmc_ctx* c)
     CHECK_CTX(c);
     int rc = flush_drift(c);
     if (rc) return rc;
-    // with valid lists the save also keeps their buffer, so a restore needs no rebuild
+    // with valid lists the save also keeps their buffer, so a restore needs no rebuild; the lists are
+    // first rebuilt at the saved positions (as a trajectory's start check does after an accepted
+    // trajectory), so that every trajectory from them keeps the whole skin
     c->saved_lists = list_md(c) && c->list_ready && c->list_valid;
+    if (c->saved_lists) {
+        if ((rc = list_import(c)) || (rc = list_invalidate(c))) return rc;
+        HIPCHK(c, list_rebuild(c));
+        HIPCHK(c, launch_fill_int(c->v.lreq + (size_t)c->lpar * (c->nchains + 1), 0, c->nchains + 1, c->stream));
+        if ((rc = list_export(c, false))) return rc;
+    }
     HIPCHK(c, launch_save_restore(c->v, 0, c->saved_lists ? c->lpar : -1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHMC_OK;
@@ -1207,23 +1230,34 @@ struct Mt64 {
     }
 };
 
+// The start check of a list-MD trajectory: the rebuild kernels (a chain rebuilds when a request is
+// pending: after an accepted trajectory, whose end positions its lists were not built at, or after
+// positions changed outside the lists), then the save of the start state, so that the trajectory and
+// every rejected one after it start from lists built at the start positions and keep the whole skin
+// for their motion.  The first force evaluation then consumes the requests without running the
+// rebuild kernels again.
+static int list_start_check(chmc_ctx* c)
+{
+    HIPCHK(c, list_rebuild(c));
+    HIPCHK(c, launch_save_restore_slots(c->v, 0, c->lpar, c->stream));
+    c->skip_next_check = true;
+    return CHMC_OK;
+}
+
 // one trajectory after the heat bath's counter is known: everything between hb() and the copy of
 // HMC.cpp:103/111 (HMC.cpp:77-117), enqueued on the context stream
 static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps, int ufirst, int rec_base)
 {
     int rc;
     if (list_md(c)) {
-        // heat bath + K_i; neighbour-list MD (LJ or surrogate forces); FP64 LJ V_f, W_f, K_f over the lists
-        if (!c->list_valid) {
-            // positions moved outside the lists (a surrogate trajectory): rebuild now and make the
-            // rebuilt buffer the start buffer of this trajectory
-            if ((rc = list_invalidate(c))) return rc;
-            HIPCHK(c, list_rebuild(c));
-            HIPCHK(c, launch_save_restore_slots(c->v, 0, c->lpar, c->stream));
-            c->list_valid = true;
-        }
+        // the start check; heat bath + K_i (momenta and K_i partials in the start lists' slot order);
+        // neighbour-list MD (LJ or surrogate forces); FP64 LJ V_f, W_f, K_f over the lists
+        if (!c->list_valid && (rc = list_invalidate(c))) return rc;      // positions moved outside the lists
+        c->list_valid = true;
+        if ((rc = list_start_check(c))) return rc;
         HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, false, true, c->stream));
         if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false, true))) return rc;
+        c->skip_next_check = false;
         if (steps == 0 && !c->list_valid && (rc = list_invalidate(c))) return rc;
         c->list_valid = true;
         if ((rc = list_energy(c, false))) return rc;
@@ -1262,8 +1296,10 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
 static int enqueue_spec_batch(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst, int rec_base, int tend)
 {
     int rc;
+    if ((rc = list_start_check(c))) return rc;
     HIPCHK(c, launch_heatbath_spec(c->v, hp.seed, c->stream));
     if ((rc = integrate_seq(c, hp.scheme, hp.dt, hp.steps, hp.lambda, false, true))) return rc;
+    c->skip_next_check = false;
     c->list_valid = true;
     if ((rc = list_energy(c, false))) return rc;
     HIPCHK(c, launch_reduce(c->v, SUM_TILES | SUM_CHUNKS_KI, c->stream));
@@ -1437,7 +1473,8 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     // ---- V_i, W_i of the start configuration (HMC.cpp:64), h_x = x (HMC.cpp:72) ----
     const bool slot_run = list_md(c);                          // the run starts on the slot-ordered state
     if (slot_run) {
-        if ((rc = list_import(c)) || (!c->list_valid && (rc = list_invalidate(c)))) return rc;
+        // the run starts from lists built at its start positions (as after an accepted trajectory)
+        if ((rc = list_import(c)) || (rc = list_invalidate(c))) return rc;
         // speculative copies start from chain 0's state (positions, lists, rebuild requests)
         if (spec) HIPCHK(c, launch_spec_broadcast(c->v, 0, false, c->stream));
         if ((rc = list_energy(c, true))) return rc;

@@ -246,6 +246,15 @@ __device__ __forceinline__ void slot_save_restore(const DevView& v, int s, int c
     L.xsl[(size_t)list_q * N + s] = make_float4(f3[0], f3[1], f3[2], 0.f);
 }
 
+// An accepted trajectory's end positions start the next trajectory, while its chain's lists were built
+// at some earlier point: a rebuild request for the next trajectory's start check (list parity q), after
+// which that trajectory and every rejected one from the same positions keep the whole skin.
+__device__ __forceinline__ void request_start_rebuild(const DevView& v, int q, int ch)
+{
+    atomicOr(v.lreq + q * (v.nchains + 1) + 1 + ch, 1);
+    atomicOr(v.lreq + q * (v.nchains + 1), 1);
+}
+
 __global__ __launch_bounds__(256) void k_save_restore_slots(DevView v, int mode, int list_q)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -254,6 +263,7 @@ __global__ __launch_bounds__(256) void k_save_restore_slots(DevView v, int mode,
     const int ch = v.atom_chain[s];               // a chain's slots span the same range as its atoms
     const bool save = mode == 2 ? v.state[ch].accepted != 0 : mode == 0;
     slot_save_restore(v, s, ch, save, list_q);
+    if (mode == 2 && save && s == v.chains[ch].atom_off) request_start_rebuild(v, list_q, ch);
 }
 
 // ---- speculative trajectories: the context's chains are K identical copies of one chain ----------
@@ -336,6 +346,7 @@ __global__ __launch_bounds__(256) void k_spec_restore(DevView v, const double* _
     if (s >= v.Ntot) return;
     const int ch = v.atom_chain[s];
     slot_save_restore(v, s, ch, ch == sj, list_q);
+    if (ch == sj && s == v.chains[ch].atom_off) request_start_rebuild(v, list_q, ch);   // copied to all by the broadcast
 }
 
 // commit: advance the counter by the consumed trajectories and give every copy the chain state after
