@@ -481,6 +481,9 @@ def main():
     ap.add_argument("--spec", type=int, default=8, choices=[1, 2, 4, 8, 16],
                     help="c2: speculative copies of the chain (chmc_set_speculation; 1: the sequential loop).  The "
                          "chain, its records and its acceptance are the sequential run's, bit for bit")
+    ap.add_argument("--start-rebuild", type=int, default=1, choices=[0, 1],
+                    help="c2: rebuild the lists at the start of the trajectory after an accepted one "
+                         "(chmc_set_list_start_rebuild; pays at low acceptance)")
     ap.add_argument("--therm", type=int, default=THERM,
                     help="c2, kernels: accept-all burn-in trajectories off the lattice (shorter for counter runs)")
     ap.add_argument("--no-ess", action="store_true",
@@ -532,6 +535,7 @@ def main():
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T), cell_md=args.cell_md, speculate=copies)
     e.init_lattice(1, 0.05)
     e.set_list_check(args.list_check)
+    e.set_list_start_rebuild(bool(args.start_rebuild))
     if args.cell_md:
         cell_md_run(args, e, chmc, stats)
         return
@@ -610,6 +614,7 @@ def main():
                    "N": N, "trajectories": args.steps, "md_steps_per_traj": NSTEPS, "force_evals_per_traj": 2 * NSTEPS + 1,
                    "thermalisation": args.therm, "md_path": "neighbour lists", "speculative_copies": copies,
                    "skin": skin, "list_lanes": lanes, "list_check": e.list_check(),
+                   "start_rebuild": e.list_start_rebuild(),
                    "ms_per_traj_by_skin": skins},
         "acceptance": float(nacc[0]) / args.steps,
         "ess_per_s": ess_info["ess_per_s"], "ess": ess_info,

@@ -114,6 +114,8 @@ def _declare(L):
         "chmc_set_list_lanes": ([P, _I], _I),
         "chmc_set_list_check": ([P, _I], _I),
         "chmc_get_list_check": ([P, C.POINTER(_I)], _I),
+        "chmc_set_list_start_rebuild": ([P, _I], _I),
+        "chmc_get_list_start_rebuild": ([P, C.POINTER(_I)], _I),
         "chmc_set_speculation": ([P, _I], _I),
         "chmc_get_speculation": ([P, C.POINTER(_I)], _I),
         "chmc_get_list_lanes": ([P, C.POINTER(_I)], _I),
@@ -151,7 +153,7 @@ EXPORTED = (
     "chmc_set_force_tiling",
     "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_set_chain_skins",
     "chmc_get_chain_skin", "chmc_list_stats",
-    "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_list_check", "chmc_get_list_check", "chmc_set_speculation", "chmc_get_speculation", "chmc_set_trajectory", "chmc_get_trajectory",
+    "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_list_check", "chmc_get_list_check", "chmc_set_speculation", "chmc_get_speculation", "chmc_set_list_start_rebuild", "chmc_get_list_start_rebuild", "chmc_set_trajectory", "chmc_get_trajectory",
     "chmc_surrogate_set_params", "chmc_surrogate_get_params",
     "chmc_surrogate_eval", "chmc_surrogate_features", "chmc_surrogate_reserve", "chmc_surrogate_collect",
     "chmc_surrogate_training_data", "chmc_surrogate_loss_grad", "chmc_surrogate_train",
@@ -521,6 +523,15 @@ class Engine:
         v = _I()
         self._ck(lib().chmc_get_list_check(self._h, C.byref(v)), "get_list_check")
         return v.value
+
+    def set_list_start_rebuild(self, on):
+        """Rebuild a chain's lists at the start of the trajectory after an accepted one (list MD)."""
+        self._ck(lib().chmc_set_list_start_rebuild(self._h, int(bool(on))), "set_list_start_rebuild")
+
+    def list_start_rebuild(self):
+        v = _I()
+        self._ck(lib().chmc_get_list_start_rebuild(self._h, C.byref(v)), "get_list_start_rebuild")
+        return bool(v.value)
 
     def set_speculation(self, copies):
         """Run hmc() as speculative trajectories of chain 0 over the context's `copies` identical chains

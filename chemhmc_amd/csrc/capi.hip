@@ -1136,11 +1136,11 @@ extern "C" int chmc_save_positions(chmc_ctx* c)
     CHECK_CTX(c);
     int rc = flush_drift(c);
     if (rc) return rc;
-    // with valid lists the save also keeps their buffer, so a restore needs no rebuild; the lists are
-    // first rebuilt at the saved positions (as a trajectory's start check does after an accepted
-    // trajectory), so that every trajectory from them keeps the whole skin
+    // with valid lists the save also keeps their buffer, so a restore needs no rebuild; with start
+    // rebuilds the lists are first rebuilt at the saved positions (as a trajectory's start check does
+    // after an accepted trajectory), so that every trajectory from them keeps the whole skin
     c->saved_lists = list_md(c) && c->list_ready && c->list_valid;
-    if (c->saved_lists) {
+    if (c->saved_lists && c->v.start_rebuild) {
         if ((rc = list_import(c)) || (rc = list_invalidate(c))) return rc;
         HIPCHK(c, list_rebuild(c));
         HIPCHK(c, launch_fill_int(c->v.lreq + (size_t)c->lpar * (c->nchains + 1), 0, c->nchains + 1, c->stream));
@@ -1230,12 +1230,14 @@ struct Mt64 {
     }
 };
 
-// The start check of a list-MD trajectory: the rebuild kernels (a chain rebuilds when a request is
-// pending: after an accepted trajectory, whose end positions its lists were not built at, or after
-// positions changed outside the lists), then the save of the start state, so that the trajectory and
-// every rejected one after it start from lists built at the start positions and keep the whole skin
-// for their motion.  The first force evaluation then consumes the requests without running the
-// rebuild kernels again.
+// The start check of a list-MD trajectory (with start rebuilds, chmc_set_list_start_rebuild): the
+// rebuild kernels (a chain rebuilds when a request is pending: after an accepted trajectory, whose end
+// positions its lists were not built at, or after positions changed outside the lists), then the save
+// of the start state, so that the trajectory and every rejected one after it start from lists built at
+// the start positions and keep the whole skin for their motion.  The first force evaluation then
+// consumes the requests without running the rebuild kernels again.  It pays where most trajectories
+// are rejected (each start serves many); at high acceptance the extra rebuild per accepted trajectory
+// costs more than it saves.
 static int list_start_check(chmc_ctx* c)
 {
     HIPCHK(c, list_rebuild(c));
@@ -1254,7 +1256,7 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
         // neighbour-list MD (LJ or surrogate forces); FP64 LJ V_f, W_f, K_f over the lists
         if (!c->list_valid && (rc = list_invalidate(c))) return rc;      // positions moved outside the lists
         c->list_valid = true;
-        if ((rc = list_start_check(c))) return rc;
+        if (c->v.start_rebuild && (rc = list_start_check(c))) return rc;
         HIPCHK(c, launch_heatbath(c->v, hp.seed, -1, false, true, c->stream));
         if (steps > 0 && (rc = integrate_seq(c, hp.scheme, hp.dt, steps, hp.lambda, false, true))) return rc;
         c->skip_next_check = false;
@@ -1296,7 +1298,7 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
 static int enqueue_spec_batch(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst, int rec_base, int tend)
 {
     int rc;
-    if ((rc = list_start_check(c))) return rc;
+    if (c->v.start_rebuild && (rc = list_start_check(c))) return rc;
     HIPCHK(c, launch_heatbath_spec(c->v, hp.seed, c->stream));
     if ((rc = integrate_seq(c, hp.scheme, hp.dt, hp.steps, hp.lambda, false, true))) return rc;
     c->skip_next_check = false;
@@ -1473,8 +1475,9 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     // ---- V_i, W_i of the start configuration (HMC.cpp:64), h_x = x (HMC.cpp:72) ----
     const bool slot_run = list_md(c);                          // the run starts on the slot-ordered state
     if (slot_run) {
-        // the run starts from lists built at its start positions (as after an accepted trajectory)
-        if ((rc = list_import(c)) || (rc = list_invalidate(c))) return rc;
+        // with start rebuilds the run starts from lists built at its start positions (as after an
+        // accepted trajectory)
+        if ((rc = list_import(c)) || ((c->v.start_rebuild || !c->list_valid) && (rc = list_invalidate(c)))) return rc;
         // speculative copies start from chain 0's state (positions, lists, rebuild requests)
         if (spec) HIPCHK(c, launch_spec_broadcast(c->v, 0, false, c->stream));
         if ((rc = list_energy(c, true))) return rc;
@@ -2227,6 +2230,20 @@ extern "C" int chmc_set_list_lanes(chmc_ctx* c, int lanes)
         if (ms < best) { best = ms; pick = lp; }
     }
     c->v.lanes = pick;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_set_list_start_rebuild(chmc_ctx* c, int on)
+{
+    CHECK_CTX(c);
+    c->v.start_rebuild = on ? 1 : 0;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_get_list_start_rebuild(const chmc_ctx* c, int* on)
+{
+    if (!c || !on) return CHMC_E_ARG;
+    *on = c->v.start_rebuild;
     return CHMC_OK;
 }
 

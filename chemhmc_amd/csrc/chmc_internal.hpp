@@ -151,6 +151,7 @@ struct DevView {
     // chain state after the batch, committed by the broadcast
     int* spec_ctl;
     ChainState* spec_st;
+    int start_rebuild;    // list MD: an accepted trajectory requests a rebuild at the next trajectory's start
 };
 
 // The updates of lp_leapfrog / lp_omf4 (code-carsten/integrator.hh:51-79, 156-197) round the updated

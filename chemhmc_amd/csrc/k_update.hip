@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void k_save_restore_slots(DevView v, int mode,
     const int ch = v.atom_chain[s];               // a chain's slots span the same range as its atoms
     const bool save = mode == 2 ? v.state[ch].accepted != 0 : mode == 0;
     slot_save_restore(v, s, ch, save, list_q);
-    if (mode == 2 && save && s == v.chains[ch].atom_off) request_start_rebuild(v, list_q, ch);
+    if (mode == 2 && save && v.start_rebuild && s == v.chains[ch].atom_off) request_start_rebuild(v, list_q, ch);
 }
 
 // ---- speculative trajectories: the context's chains are K identical copies of one chain ----------
@@ -346,7 +346,8 @@ __global__ __launch_bounds__(256) void k_spec_restore(DevView v, const double* _
     if (s >= v.Ntot) return;
     const int ch = v.atom_chain[s];
     slot_save_restore(v, s, ch, ch == sj, list_q);
-    if (ch == sj && s == v.chains[ch].atom_off) request_start_rebuild(v, list_q, ch);   // copied to all by the broadcast
+    if (ch == sj && v.start_rebuild && s == v.chains[ch].atom_off)
+        request_start_rebuild(v, list_q, ch);            // copied to every copy by the broadcast
 }
 
 // commit: advance the counter by the consumed trajectories and give every copy the chain state after

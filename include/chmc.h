@@ -283,6 +283,14 @@ int chmc_get_list_lanes(const chmc_ctx* ctx, int* lanes);
  * the flags fail with CHMC_E_STATE.  1: the check before every evaluation.  No reference counterpart. */
 int chmc_set_list_check(chmc_ctx* ctx, int every);
 int chmc_get_list_check(const chmc_ctx* ctx, int* every);
+/* start rebuilds of list MD (default off): a chain whose trajectory was accepted rebuilds its lists at
+ * the start of the next trajectory, before the save of the start state (and so do a chmc_hmc_run start
+ * and chmc_save_positions), so that every trajectory from those positions -- the accepted state is the
+ * start of all the rejected trajectories that follow it -- begins with lists built where it starts.
+ * Fewer rebuilds inside trajectories where acceptance is low, one more per accepted trajectory: a gain
+ * at low acceptance, a loss at high.  No reference counterpart. */
+int chmc_set_list_start_rebuild(chmc_ctx* ctx, int on);
+int chmc_get_list_start_rebuild(const chmc_ctx* ctx, int* on);
 /* speculative trajectories: a context created with `copies` identical chains (same descriptor, one
  * skin, FP32 neighbour-list MD) runs chmc_hmc_run as ONE chain, chain 0.  Each batch runs the chain's
  * next `copies` trajectories at once, all from its current state (copy k: trajectory t + 1 + k, heat

@@ -7,7 +7,9 @@ the sequential one bit for bit: every record field (dH, U, K, W, decision), the 
 the final positions and momenta.  Checked at an acceptance near one half (most batches end early and hand
 an accepted copy's positions and lists to the others), through the accept-all thermalisation (every
 batch keeps one trajectory) and at N = 32768, T* = 0.722 with the OMF2 10 x 0.01 trajectories of the
-benchmark (acceptance of a few percent: most batches keep every trajectory), graph-replayed and direct.
+benchmark (acceptance of a few percent: most batches keep every trajectory), graph-replayed and direct,
+with and without start rebuilds (chmc_set_list_start_rebuild: the lists rebuilt at the start of the
+trajectory after an accepted one, in both loops).
 """
 import numpy as np
 import pytest
@@ -17,7 +19,7 @@ pytestmark = pytest.mark.gpu
 RC = 2.5
 
 
-def _pair(chmc, N, rho, T, K, lanes=8, check=2, skin=0.6):
+def _pair(chmc, N, rho, T, K, lanes=8, check=2, skin=0.6, start_rebuild=False):
     L = (N / rho) ** (1 / 3)
     ch = chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T)
     seq = chmc.Engine(ch)
@@ -27,6 +29,8 @@ def _pair(chmc, N, rho, T, K, lanes=8, check=2, skin=0.6):
         e.set_skin(skin)
         e.set_list_lanes(lanes)
         e.set_list_check(check)
+        e.set_list_start_rebuild(start_rebuild)
+        assert e.list_start_rebuild() == start_rebuild
     return seq, spec
 
 
@@ -42,9 +46,9 @@ def _same_run(chmc, seq, spec, ntraj, therm, dt, steps, seed, use_graph):
     return r1[:, 0], n1[0]
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_speculative_equals_sequential_mixed_acceptance(chmc, use_graph):
-    seq, spec = _pair(chmc, 4000, 0.8442, 0.722, 4)
+@pytest.mark.parametrize("use_graph,start_rebuild", [(True, False), (False, False), (True, True), (False, True)])
+def test_speculative_equals_sequential_mixed_acceptance(chmc, use_graph, start_rebuild):
+    seq, spec = _pair(chmc, 4000, 0.8442, 0.722, 4, start_rebuild=start_rebuild)
     seq.init_lattice(3, 0.05)
     x0 = seq.positions()
     spec.set_positions(x0)                 # chain 0 only: the run copies it into the other copies
@@ -59,8 +63,9 @@ def test_speculative_equals_sequential_mixed_acceptance(chmc, use_graph):
     spec.close()
 
 
-def test_speculative_equals_sequential_c2(chmc):
-    seq, spec = _pair(chmc, 32768, 0.8442, 0.722, 8, lanes=816, skin=0.7)
+@pytest.mark.parametrize("start_rebuild", [False, True])
+def test_speculative_equals_sequential_c2(chmc, start_rebuild):
+    seq, spec = _pair(chmc, 32768, 0.8442, 0.722, 8, lanes=816, skin=0.7, start_rebuild=start_rebuild)
     seq.init_lattice(1, 0.05)
     spec.set_positions(seq.positions())
     # accept-all thermalisation (one trajectory per batch), then the benchmark's trajectories
