@@ -55,11 +55,15 @@ SWEEP_TRAFFIC_PROFILED = {(0.6, 8, 1): (FETCH_SCALE * 9976.0 + 2048.0) * 1024.0,
                           (0.7, 8, 1): (FETCH_SCALE * 9970.0 + 2048.0) * 1024.0,
                           (0.7, 816, 1): (FETCH_SCALE * 10086.8 + 2048.0) * 1024.0,
                           (0.4, 812, 8): (FETCH_SCALE * 60498.2 + 16384.2) * 1024.0}
-# the same at N=1,048,576 (workload c5), skin 0.5, 8 lanes (profiles/lj_list_sweep_N1048576.md)
-SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8): (FETCH_SCALE * 322982.0 + 65536.0) * 1024.0}
-# the same for the 256-chain batch (workload c3) at the per-chain skins base 0.4, exponent 1.5 and 4
-# lanes per atom (profiles/lj_list_sweep_c3_batch.md), keyed by the skin family the run uses
-SWEEP_TRAFFIC_PROFILED_C3 = {("0.4/1.5", 4): (FETCH_SCALE * 115006.0 + 38233.0) * 1024.0}
+# the same at N=1,048,576 (workload c5), skin 0.5 with 8 lanes and skin 0.7 with the 816 shape
+# (profiles/lj_list_sweep_N1048576.md)
+SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8): (FETCH_SCALE * 322982.0 + 65536.0) * 1024.0,
+                             (0.7, 816): (FETCH_SCALE * 323121.7 + 65536.3) * 1024.0}
+# the same for the 256-chain batch (workload c3) at the per-chain skin families base 0.4, exponent 1.5
+# and base 0.45, exponent 1.0, 4 lanes per atom (profiles/lj_list_sweep_c3_batch.md), keyed by the skin
+# family the run uses
+SWEEP_TRAFFIC_PROFILED_C3 = {("0.4/1.5", 4): (FETCH_SCALE * 115006.0 + 38233.0) * 1024.0,
+                             ("0.45/1.0", 4): (FETCH_SCALE * 116471.5 + 38304.6) * 1024.0}
 
 
 def _omp_threads(n):
