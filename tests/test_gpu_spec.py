@@ -75,6 +75,30 @@ def test_speculative_equals_sequential_c2(chmc, start_rebuild):
     spec.close()
 
 
+def test_speculative_equals_sequential_large_cell_grid(chmc):
+    """N = 131072: 19^3 list cells, more than one build workgroup scans, so rebuilds take the separate
+    scan kernel and the broadcast's cell offsets are the ones the build reads; lp_leapfrog trajectories
+    short enough to be accepted part of the time."""
+    seq, spec = _pair(chmc, 131072, 0.8442, 0.722, 2, start_rebuild=True)
+    for e in (seq, spec):
+        e.set_lp_exponent(12)
+    seq.init_lattice(2, 0.05)
+    spec.set_positions(seq.positions())
+    seq.hmc(4, 4, chmc.OMF2, 0.002, 10, seed=3)
+    spec.hmc(4, 4, chmc.OMF2, 0.002, 10, seed=3)
+    # dH at this size comes mostly from pairs crossing the unshifted cut-off, in proportion to the
+    # trajectory's length: 4 x 0.0005 keeps it near one
+    r1, n1 = seq.hmc(24, 0, chmc.LP_LEAPFROG, 0.0005, 4, seed=9)
+    r2, n2 = spec.hmc(24, 0, chmc.LP_LEAPFROG, 0.0005, 4, seed=9)
+    print("accepted", n1[0], "of 24", np.array2string(r1["dh"][:, 0], precision=3))
+    assert 1 <= n1[0] <= 23, n1
+    assert np.array_equal(r1[:, 0], r2[:, 0]) and n1[0] == n2[0]
+    assert np.array_equal(seq.positions(), spec.positions())
+    assert np.array_equal(seq.momenta(), spec.momenta())
+    seq.close()
+    spec.close()
+
+
 def test_speculation_refusals(chmc):
     L = (512 / 0.8) ** (1 / 3)
     a, b = chmc.Chain(N=512, L=L, cutoff=RC, beta=1 / 1.5), chmc.Chain(N=512, L=L, cutoff=RC, beta=1 / 1.4)
@@ -88,6 +112,10 @@ def test_speculation_refusals(chmc):
     e.init_lattice(1, 0.05)
     with pytest.raises(chmc.ChmcError):
         e.hmc(4, 0, chmc.LEAPFROG, 0.005, 10, seed=1, randomize=True)
+    e.set_chain_skins([0.5, 0.6])
+    with pytest.raises(chmc.ChmcError):
+        e.hmc(4, 0, chmc.LEAPFROG, 0.005, 10, seed=1)      # copies with different skins
+    e.set_chain_skins([0.5, 0.5])
     e.set_speculation(1)                       # off: two independent chains again
     assert e.speculation() == 1
     rec, nacc = e.hmc(3, 0, chmc.LEAPFROG, 0.005, 10, seed=1)
