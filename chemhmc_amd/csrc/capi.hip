@@ -38,6 +38,7 @@ struct chmc_ctx {
     int lp_exponent = 16;        // n_prec of the lp_* integrators (lp_leapfrog() : n_prec(16))
     int traj_done = 0;           // trajectories completed on the device counter
     int spec = 0;                // > 1: the chains are that many copies of one chain, run speculatively
+    int pad_span = 0;            // every row of both list buffers is padded with no-op entries at least this far
     bool skip_next_check = false; // the next checked list evaluation's rebuild kernels already ran (a
                                   // trajectory's start check, before its save)
     std::string err;
@@ -827,7 +828,21 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
     c->cpar = 0;
     c->list_valid = false;
     if (v.lanes == 0) v.lanes = 8;
+    v.lspan = std::min(sweep_span(v.lanes), nbcap);
+    c->pad_span = v.lspan;      // whole rows are padded now, and every build pads to lspan
     c->list_ready = true;
+    return CHMC_OK;
+}
+
+// the list-sweep shape; rows are padded to its unconditional span (longer rows built for another shape
+// are fine; shorter ones are padded further here, in both buffers, before any sweep reads them)
+static int set_list_shape(chmc_ctx* c, int lanes)
+{
+    const int span = std::min(sweep_span(lanes), c->v.nbcap);
+    if (span > c->pad_span) HIPCHK(c, launch_list_repad(c->v, span, c->stream));
+    c->pad_span = span;                                  // builds from here on pad to span
+    c->v.lanes = lanes;
+    c->v.lspan = span;
     return CHMC_OK;
 }
 
@@ -2217,20 +2232,16 @@ extern "C" int chmc_set_list_lanes(chmc_ctx* c, int lanes)
         return fail(c, CHMC_E_ARG, "set_list_lanes: 0 (autotune), 4, 8, 16, 812 or 816");
     int rc = list_ensure(c);
     if (rc) return rc;
-    if (lanes) {
-        c->v.lanes = lanes;
-        return CHMC_OK;
-    }
+    if (lanes) return set_list_shape(c, lanes);
     double best = 1e30;
     int pick = 8;
     for (int lp : { 4, 8, 16, 812, 816 }) {
-        c->v.lanes = lp;
+        if ((rc = set_list_shape(c, lp))) return rc;
         double ms = 0.0;
         if ((rc = chmc_time_list(c, 20, &ms, nullptr))) return rc;
         if (ms < best) { best = ms; pick = lp; }
     }
-    c->v.lanes = pick;
-    return CHMC_OK;
+    return set_list_shape(c, pick);
 }
 
 extern "C" int chmc_set_list_start_rebuild(chmc_ctx* c, int on)

@@ -104,6 +104,12 @@ constexpr int kTile = 32;          // slots per neighbour-list sweep workgroup (
 // kFarCode, which no neighbour has: the sweep's shift table places that image beyond any cut-off)
 constexpr int kListSpan = 128;
 constexpr int kFarCode = 63;
+// the unconditional first batch of each list-sweep shape (lanes per atom x entries per lane): 4 lanes
+// x 16, 8 x 14, 16 x 7, 8 x 12 ("812"), 8 x 16 ("816"); rows are padded at least this far
+__host__ __device__ constexpr int sweep_span(int shape)
+{
+    return shape == 4 ? 64 : shape == 8 ? 112 : shape == 16 ? 112 : shape == 812 ? 96 : 128;
+}
 constexpr float kFarShift = 1.0e18f;
 constexpr int kScanCells = 3072;   // list cells of a chain whose counts one build workgroup scans in LDS
 
@@ -152,6 +158,8 @@ struct DevView {
     int* spec_ctl;
     ChainState* spec_st;
     int start_rebuild;    // list MD: an accepted trajectory requests a rebuild at the next trajectory's start
+    int lspan;            // list rows are padded with no-op entries past their count up to this many entries
+                          // (the sweep shape's unconditional span, at most nbcap)
 };
 
 // The updates of lp_leapfrog / lp_omf4 (code-carsten/integrator.hh:51-79, 156-197) round the updated
@@ -322,6 +330,8 @@ hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s);
 // a standalone drift (q -> q ^ 1) and the FP64 energy / virial / kinetic sums into tile_part
 hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s);
 hipError_t launch_list_pad_init(const DevView& v, hipStream_t s);
+// every row of both list buffers padded with no-op entries from its count up to span
+hipError_t launch_list_repad(const DevView& v, int span, hipStream_t s);
 // checked: this evaluation's rebuild check ran (its parity's requests are consumed); otherwise a pending
 // request is carried to the next parity
 hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, bool checked,

@@ -470,8 +470,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         L.nbn[ibeg + r] = min(n, nbcap);
         if (n > nbcap) atomicOr(&v.lflag[2], 2);
     }
-    // rows padded to the sweep's unconditional span with no-op entries (the far image of the slot)
-    const int span = min(kListSpan, nbcap);
+    // rows padded to the sweep shape's unconditional span with no-op entries (the far image of the slot)
+    const int span = v.lspan;
     for (int r = wave; r < ni; r += W) {
         const int slot = __float_as_int(si[r].w);
         const int pad = (slot << kImageBits) | kFarCode;
@@ -488,6 +488,26 @@ __global__ __launch_bounds__(256) void k_lpad_init(DevView v)
     if (i >= 2 * per) return;
     const int slot = (int)((i % per) / v.nbcap);
     v.nb[i] = (slot << kImageBits) | kFarCode;
+}
+
+// rows of both buffers padded from their counts up to span (a sweep shape with a longer first batch
+// than the span the rows were built with)
+__global__ __launch_bounds__(256) void k_lrepad(DevView v, int span)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * (size_t)v.Ntot) return;
+    const int b = (int)(i / v.Ntot), slot = (int)(i % v.Ntot);
+    const ListBuf L = list_buf(v, b);
+    int* __restrict__ row = L.nb + (size_t)slot * v.nbcap;
+    const int pad = (slot << kImageBits) | kFarCode;
+    for (int k = max(L.nbn[slot], 0); k < span; k++) row[k] = pad;
+}
+
+hipError_t launch_list_repad(const DevView& v, int span, hipStream_t s)
+{
+    const size_t n = 2 * (size_t)v.Ntot;
+    hipLaunchKernelGGL(k_lrepad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v, min(span, v.nbcap));
+    return hipGetLastError();
 }
 
 hipError_t launch_list_pad_init(const DevView& v, hipStream_t s)
@@ -1003,6 +1023,9 @@ template <int MODE>
 static hipError_t sweep_lp(const DevView& v, int shape, int q, double kick_dt, double drift_dt, int ck, hipStream_t s)
 {
     const dim3 g(v.ntiles);
+    static_assert(sweep_span(4) == 4 * 16 && sweep_span(8) == 8 * 14 && sweep_span(16) == 16 * 7 &&
+                  sweep_span(812) == 8 * 12 && sweep_span(816) == 8 * 16, "shape spans");
+    if (sweep_span(shape) > v.lspan) return hipErrorInvalidValue;    // rows not padded that far
     switch (shape) {
     case 4: hipLaunchKernelGGL((k_lsweep<MODE, 4, 16>), g, dim3(kTile * 4), 0, s, v, kick_dt, drift_dt, q, ck); break;
     case 8: hipLaunchKernelGGL((k_lsweep<MODE, 8, 14>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;

@@ -396,7 +396,7 @@ __global__ __launch_bounds__(256) void k_spec_broadcast(DevView v, int src, int 
         }
     }
     // list rows, four entries per thread, up to the padded span or the count
-    const int q4 = v.nbcap / 4, span = min(kListSpan, v.nbcap);
+    const int q4 = v.nbcap / 4, span = v.lspan;
     for (size_t u = gid; u < (size_t)Nt * q4; u += stride) {
         const int s = (int)(u / q4), c4 = (int)(u % q4);
         const int k = v.atom_chain[s];

@@ -313,9 +313,9 @@ def test_list_md_vs_cell_md_at_1m_atoms(chmc):
 
 def test_list_sweep_lanes_agree(chmc):
     """The list sweep at every shape, 4, 8 and 16 lanes per atom and 8 lanes with first batches of 12 and
-    16 entries (64, 112, 112, 96 and 128 entries read without a count test from rows padded to a fixed
-    span), gives the cell sweep's forces, also when the shape changes after the lists were built (the
-    padding does not depend on it).  The shapes differ only in the FP32 summation order."""
+    16 entries (64, 112, 112, 96 and 128 entries read without a count test from rows padded that far),
+    gives the cell sweep's forces, also when the shape changes after the lists were built.  The shapes
+    differ only in the FP32 summation order."""
     N, T = 4000, 0.722
     e, L = make(chmc, N, 0.8442, T=T)
     ec, _ = make(chmc, N, 0.8442, T=T, cell_md=True)
@@ -339,6 +339,37 @@ def test_list_sweep_lanes_agree(chmc):
         if lanes in forces:
             assert np.array_equal(fl, forces[lanes])  # same lanes, same positions: bitwise
         forces[lanes] = fl
+
+
+def test_list_rows_repadded_for_a_longer_shape(chmc):
+    """Rows are padded only as far as the current shape reads them without a count test.  Lists rebuilt
+    under the 4-lane shape (64 entries) keep stale entries of earlier, longer rows past 64; switching to
+    a longer shape pads both list buffers again first, so every shape still gives the cell sweep's
+    forces after MD under the short one."""
+    N, T = 4000, 0.722
+    e, L = make(chmc, N, 0.8442, T=T)
+    ec, _ = make(chmc, N, 0.8442, T=T, cell_md=True)
+    e.set_skin(0.7)
+    e.set_list_lanes(816)
+    e.heatbath(9, 1)
+    e.integrate(6, 0.01, 10, 0.1932)
+    e.set_list_lanes(4)
+    e.heatbath(9, 2)
+    e.integrate(6, 0.01, 20, 0.1932)                 # rebuilds pad to 64 entries only
+    _, _, nreb = e.list_stats()
+    assert nreb > 2
+    ec.set_positions(e.positions())
+    ec.compute_force()
+    fc = ec.force()
+    frms = np.sqrt(np.mean(fc ** 2))
+    fcut = abs(48.0 * RC ** -6 * (RC ** -6 - 0.5) / RC)
+    for lanes in (816, 812, 8, 16):
+        e.set_list_lanes(lanes)
+        e.integrate(6, 0.0, 1, 0.1932)
+        dev = np.max(np.abs(e.force() - fc), axis=1)
+        odd = np.flatnonzero(dev >= 5e-5 * frms)
+        print(f"lanes {lanes}: max deviation {dev.max():.2e}")
+        assert len(odd) <= 2 and np.all(dev[odd] < 2 * fcut + 5e-5 * frms), (lanes, odd, dev[odd])
 
 
 @pytest.mark.parametrize("N,rho,nb", [(512, 0.8, 200), (4000, 0.8442, 500)])
