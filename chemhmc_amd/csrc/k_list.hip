@@ -697,14 +697,6 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         if constexpr ((MODE & LJ_STOREF) != 0) atom = L.lperm[s];     // the stored force is in atom order
         xi = xq[s];
         n = L.nbn[s];
-        // the tail's operands are fetched now, alongside the list, not after the butterfly
-        if (sub < 3) {
-            p_d = v.pl[d * N + s];
-            if constexpr ((MODE & (LJ_DRIFT | LJ_REL64)) != 0) {
-                x_d = v.xl[d * N + s];
-                ref = L.lref[s];
-            }
-        }
 #pragma unroll
         for (int u = 0; u < kBatch; u++) pj[u] = xq[(unsigned)ent[u] >> kImageBits];
     }
@@ -716,6 +708,14 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         for (int k = sub + kBatch * LP; k < n; k += LP) {
             const int ee = nbs[k];
             pair(xq[(unsigned)ee >> kImageBits], ee & 63);
+        }
+    }
+    // the tail's operands, fetched after the pairs (fewer registers live through the pair loop)
+    if (val && sub < 3) {
+        p_d = v.pl[d * N + s];
+        if constexpr ((MODE & (LJ_DRIFT | LJ_REL64)) != 0) {
+            x_d = v.xl[d * N + s];
+            ref = L.lref[s];
         }
     }
     ax = group_sum<LP>(ax);
