@@ -325,9 +325,11 @@ def surrogate_workload(args):
 
     params, t_collect, t_train, le, lf = train_c4_surrogate(args, chmc)
     L = (N / RHO) ** (1 / 3)
-    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T))
+    # both legs run the chain as args.spec speculative copies (chmc_set_speculation), as the c2 line does
+    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T), speculate=args.spec)
     e.init_lattice(1, 0.05)
     e.set_list_check(args.list_check)
+    e.set_list_start_rebuild(bool(args.start_rebuild))
     e.hmc(THERM, THERM, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10)
     if args.warmup > 0:
         e.hmc(args.warmup, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11)
@@ -362,6 +364,7 @@ def surrogate_workload(args):
         "dtype": "bf16 MFMA surrogate MD, f64 LJ accept", "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": "c4: surrogate HMC N=32768 rho=0.8442 T=0.722 OMF2(0.1932) 10x0.01; surrogate 32-64-64-1 "
                                "trained with energy + force matching on snapshots of the same system's plain-HMC chain",
+                   "speculative_copies": e.speculation(), "start_rebuild": e.list_start_rebuild(),
                    "train_configs": args.train_configs, "train_steps": args.train_steps, "w_force": C4_W_FORCE, "lr": 1e-3,
                    "collect_seconds": t_collect, "train_seconds": t_train,
                    "energy_mse_first": float(le[:20].mean()), "energy_mse_last": float(le[-100:].mean()),
@@ -483,10 +486,10 @@ def main():
                     help="list MD: rebuild checks before every LIST_CHECK-th force evaluation (chmc_set_list_check); "
                          "tuned with the skin (1 or 2) unless given; 2 where the skin is fixed")
     ap.add_argument("--spec", type=int, default=8, choices=[1, 2, 4, 8, 16],
-                    help="c2: speculative copies of the chain (chmc_set_speculation; 1: the sequential loop).  The "
+                    help="c2, c4: speculative copies of the chain (chmc_set_speculation; 1: the sequential loop).  The "
                          "chain, its records and its acceptance are the sequential run's, bit for bit")
     ap.add_argument("--start-rebuild", type=int, default=1, choices=[0, 1],
-                    help="c2: rebuild the lists at the start of the trajectory after an accepted one "
+                    help="c2, c4: rebuild the lists at the start of the trajectory after an accepted one "
                          "(chmc_set_list_start_rebuild; pays at low acceptance)")
     ap.add_argument("--therm", type=int, default=THERM,
                     help="c2, kernels: accept-all burn-in trajectories off the lattice (shorter for counter runs)")

@@ -1310,7 +1310,7 @@ static int enqueue_trajectory(chmc_ctx* c, const chmc_hmc_params& hp, int steps,
 // and the accepted copy's state goes to every copy.  The chain is the one the sequential loop of
 // enqueue_trajectory produces, trajectory for trajectory: a rejected trajectory leaves the state
 // unchanged, so every trajectory up to the first accepted one starts from the same state.
-static int enqueue_spec_batch(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst, int rec_base, int tend)
+static int enqueue_spec_batch(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst, int rec_base)
 {
     int rc;
     if (c->v.start_rebuild && (rc = list_start_check(c))) return rc;
@@ -1320,7 +1320,8 @@ static int enqueue_spec_batch(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst
     c->list_valid = true;
     if ((rc = list_energy(c, false))) return rc;
     HIPCHK(c, launch_reduce(c->v, SUM_TILES | SUM_CHUNKS_KI, c->stream));
-    HIPCHK(c, launch_spec_restore(c->v, c->d_uniforms, hp.therm, ufirst, c->d_rec, rec_base, tend, c->lpar, c->stream));
+    HIPCHK(c, launch_spec_restore(c->v, c->d_uniforms, hp.therm, ufirst, c->d_rec, rec_base, c->md_surr ? 1 : 0, c->lpar,
+                                  c->stream));
     HIPCHK(c, launch_spec_broadcast(c->v, -1, true, c->stream));
     c->list_fresh = c->rel64_fresh = false;
     c->force_kind = 0;
@@ -1408,10 +1409,14 @@ static int run_spec(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst, int rec_
                     std::vector<TrajGraph>& graphs)
 {
     const int K = c->nchains;
-    int done = rec_base, rc;
+    int done, rc;
+    // the window's last trajectory, read by the decisions of every batch (graph replays included)
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->v.spec_ctl + 3, &tend, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(&done, c->v.traj, sizeof(int), hipMemcpyDeviceToHost));
     while (done < tend) {
         const int nb = (tend - done + K - 1) / K;
-        auto enq = [&]() { return enqueue_spec_batch(c, hp, ufirst, rec_base, tend); };
+        auto enq = [&]() { return enqueue_spec_batch(c, hp, ufirst, rec_base); };
         if (hp.use_graph) {
             if ((rc = run_units(c, nb, true, graphs, enq))) return rc;
         } else {
@@ -1436,8 +1441,8 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     if (hp->surrogate && !c->surr_params) return fail(c, CHMC_E_STATE, "hmc_run: surrogate parameters not set");
     const bool spec = c->spec > 1;
     if (spec) {
-        if (hp->randomize || hp->surrogate)
-            return fail(c, CHMC_E_ARG, "hmc_run: speculative trajectories need fixed-length trajectories with LJ forces");
+        if (hp->randomize)
+            return fail(c, CHMC_E_ARG, "hmc_run: speculative trajectories need fixed-length trajectories");
         for (double sk : c->chain_skin)
             if (sk != c->chain_skin[0]) return fail(c, CHMC_E_ARG, "hmc_run: speculative copies need one skin");
     }
@@ -1522,18 +1527,16 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     c->md_surr = hp->surrogate != 0;
     const int window = (c->md_surr && hp->fallback_window > 0) ? hp->fallback_window : ntraj;
     std::vector<HmcRecord> wrec;
-    if (spec) {
-        // records of the chain in column 0, the other columns zero
-        HIPCHK(c, hipMemsetAsync(c->d_rec, 0, sizeof(HmcRecord) * nrec, c->stream));
-        rc = run_spec(c, *hp, ufirst, rec_base, rec_base + ntraj, graphs);
-    }
-    for (int t0 = 0; t0 < ntraj && !rc && !spec; t0 += window) {
+    // speculative runs: the chain's records in column 0, the other columns zero
+    if (spec) HIPCHK(c, hipMemsetAsync(c->d_rec, 0, sizeof(HmcRecord) * nrec, c->stream));
+    for (int t0 = 0; t0 < ntraj && !rc; t0 += window) {
         const int count = std::min(window, ntraj - t0);
         if (list_md(c) && !c->lstate) {          // LJ and surrogate MD both run on the slot-ordered state
             c->list_valid = false;
             if ((rc = list_import(c))) break;
         }
-        rc = run_segment(c, *hp, t0, count, ufirst, rec_base, rsteps, graphs);
+        rc = spec ? run_spec(c, *hp, ufirst, rec_base, rec_base + t0 + count, graphs)
+                  : run_segment(c, *hp, t0, count, ufirst, rec_base, rsteps, graphs);
         if (rc || !c->md_surr) continue;
         if ((rc = surr_check_flag(c))) break;
         if (t0 + count >= ntraj) break;
@@ -1543,7 +1546,8 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
         if (e != hipSuccess) { rc = fail(c, CHMC_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e)); break; }
         int acc = 0;
         for (const HmcRecord& r : wrec) acc += r.acc;
-        if ((double)acc < hp->fallback_acc * (double)wrec.size()) c->md_surr = false;   // LJ forces from here on
+        const double nwin = spec ? (double)count : (double)wrec.size();   // speculative: one chain, column 0
+        if ((double)acc < hp->fallback_acc * nwin) c->md_surr = false;     // LJ forces from here on
     }
     for (TrajGraph& g : graphs) hipGraphExecDestroy(g.exec);
     c->md_surr = false;

@@ -271,20 +271,22 @@ __global__ __launch_bounds__(256) void k_save_restore_slots(DevView v, int mode,
 // sequential chain would run trajectory t0 + 1 + k from that same state exactly when trajectories
 // t0 + 1 .. t0 + k were all rejected, so the copies are tested in order with the Metropolis step of
 // k_metropolis against the state after t0; the first accepted copy closes the batch (the copies after
-// it started from a state the chain no longer holds and are discarded), and so does the run's last
-// trajectory tend.  Every workgroup evaluates the same decision from the same inputs; workgroup 0
+// it started from a state the chain no longer holds and are discarded), and so does the last trajectory
+// of the host's window, spec_ctl[3] (a batch after it consumes nothing).  Every workgroup evaluates the same decision from the same inputs; workgroup 0
 // writes the records, the control words and the chain state after the batch, which the broadcast
 // commits (the kernel that reads the counter and the state never writes them).
 //   spec_ctl[0]: copy whose end state the chain takes (-1: none accepted; every copy restores the
 //                start state), [1]: trajectories the batch consumed, [2]: copy of the last consumed
-//                trajectory (its post-MD momenta are the chain's at the end of a run)
+//                trajectory (its post-MD momenta are the chain's at the end of a run), [3]: last
+//                trajectory of the window (set by the host before the window's batches, so that graph
+//                replays captured in an earlier window read it too)
 struct SpecDecision {
     int j, consumed;
     ChainState st;
 };
 
 __device__ SpecDecision spec_decide(const DevView& v, const double* __restrict__ uniforms, int therm, int ufirst, int tend,
-                                    HmcRecord* __restrict__ rec, int rec_base)
+                                    HmcRecord* __restrict__ rec, int rec_base, int surr)
 {
     SpecDecision d;
     d.j = -1;
@@ -311,7 +313,7 @@ __device__ SpecDecision spec_decide(const DevView& v, const double* __restrict__
         d.consumed = k + 1;
         if (rec) {
             HmcRecord h;
-            h.dh = dh; h.u = st.Vi; h.k = acc ? Kf : Ki; h.w = st.Wi; h.acc = acc; h.surr = 0;
+            h.dh = dh; h.u = st.Vi; h.k = acc ? Kf : Ki; h.w = st.Wi; h.acc = acc; h.surr = surr;
             rec[(size_t)(i - 1 - rec_base) * v.nchains] = h;
         }
         if (acc) {
@@ -327,12 +329,12 @@ __device__ SpecDecision spec_decide(const DevView& v, const double* __restrict__
 // the heat bath's chunk partials), then per slot: the accepted copy keeps its end state as its start
 // state, every other copy restores its start state
 __global__ __launch_bounds__(256) void k_spec_restore(DevView v, const double* __restrict__ uniforms, int therm, int ufirst,
-                                                      HmcRecord* __restrict__ rec, int rec_base, int tend, int list_q)
+                                                      HmcRecord* __restrict__ rec, int rec_base, int surr, int list_q)
 {
     __shared__ int sj;
     if (threadIdx.x == 0) {
         const bool lead = blockIdx.x == 0;
-        const SpecDecision d = spec_decide(v, uniforms, therm, ufirst, tend, lead ? rec : nullptr, rec_base);
+        const SpecDecision d = spec_decide(v, uniforms, therm, ufirst, v.spec_ctl[3], lead ? rec : nullptr, rec_base, surr);
         sj = d.j;
         if (lead) {
             v.spec_ctl[0] = d.j;
@@ -477,10 +479,10 @@ hipError_t launch_save_restore_slots(const DevView& v, int mode, int list_q, hip
 }
 
 hipError_t launch_spec_restore(const DevView& v, const double* uniforms, int therm, int ufirst, HmcRecord* rec,
-                               int rec_base, int tend, int list_q, hipStream_t s)
+                               int rec_base, int surr, int list_q, hipStream_t s)
 {
     hipLaunchKernelGGL(k_spec_restore, dim3((v.Ntot + 255) / 256), dim3(256), 0, s, v, uniforms, therm, ufirst, rec,
-                       rec_base, tend, list_q);
+                       rec_base, surr, list_q);
     return hipGetLastError();
 }
 

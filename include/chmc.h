@@ -300,7 +300,8 @@ int chmc_get_list_start_rebuild(const chmc_ctx* ctx, int* on);
  * ones before it were rejected, so records, positions, momenta and acceptance equal the single-chain
  * run's bit for bit; where acceptance is low, nearly every batch keeps all `copies` trajectories.
  * chmc_hmc_run then writes the chain's records in column 0 (the other columns zero) and its count in
- * nacc[0]; randomize and surrogate runs are refused.  copies <= 1 switches it off.  No reference
+ * nacc[0]; randomize runs are refused (surrogate runs, with their fallback windows, are speculative
+ * too).  copies <= 1 switches it off.  No reference
  * counterpart (HMC.cpp:77-117 is the sequential loop this reproduces). */
 int chmc_set_speculation(chmc_ctx* ctx, int copies);
 int chmc_get_speculation(const chmc_ctx* ctx, int* copies);

@@ -224,6 +224,34 @@ def test_fallback_switches_to_lj(chmc, trained):
     assert np.all(r["surrogate"] == 1)
 
 
+@pytest.mark.parametrize("fallback_acc", [0.0, 1.01])
+def test_speculative_surrogate_equals_sequential(chmc, trained, fallback_acc):
+    """Surrogate-driven trajectories run as 4 speculative copies of one chain give the sequential chain's
+    records, positions and momenta bit for bit, also across the fallback to LJ forces after the first
+    10-trajectory window (fallback_acc 1.01: every window's acceptance is below it)."""
+    t, _ = trained
+    ch = chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T)
+    seq, spec = chmc.Engine(ch), chmc.Engine(ch, speculate=4)
+    x0, p = t.positions(), t.surrogate_params()
+    for e in (seq, spec):
+        e.set_list_lanes(8)
+        e.surrogate_set_params(p)
+        e.set_positions(x0)
+    kw = dict(seed=83, surrogate=True, fallback_window=10, fallback_acc=fallback_acc)
+    r1, n1 = seq.hmc(30, 0, chmc.LEAPFROG, 2 * DT, STEPS, **kw)
+    r2, n2 = spec.hmc(30, 0, chmc.LEAPFROG, 2 * DT, STEPS, **kw)
+    print("accepted", n1[0], "of 30; surrogate", r1["surrogate"][:, 0])
+    assert 1 <= n1[0] <= 29, n1
+    assert np.all(r1["surrogate"][:10] == 1)
+    assert np.all(r1["surrogate"][10:] == (0 if fallback_acc > 1 else 1))
+    assert r2.shape == (30, 1)
+    assert np.array_equal(r1[:, 0], r2[:, 0]) and n1[0] == n2[0]
+    assert np.array_equal(seq.positions(), spec.positions())
+    assert np.array_equal(seq.momenta(), spec.momenta())
+    seq.close()
+    spec.close()
+
+
 @pytest.mark.slow
 def test_surrogate_chain_samples_lj_and_reports_ess(chmc, trained):
     """Same distribution as the LJ-driven chain; prints acceptance, time and ESS/s of U for both."""
