@@ -119,6 +119,8 @@ def _declare(L):
         "chmc_set_speculation": ([P, _I], _I),
         "chmc_get_speculation": ([P, C.POINTER(_I)], _I),
         "chmc_get_list_lanes": ([P, C.POINTER(_I)], _I),
+        "chmc_set_list_entry_bits": ([P, _I], _I),
+        "chmc_get_list_entry_bits": ([P, C.POINTER(_I)], _I),
         "chmc_set_trajectory": ([P, _I], _I),
         "chmc_get_trajectory": ([P, C.POINTER(_I)], _I),
         "chmc_surrogate_set_params": ([P, C.c_void_p, _I], _I),
@@ -153,7 +155,7 @@ EXPORTED = (
     "chmc_set_force_tiling",
     "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_set_chain_skins",
     "chmc_get_chain_skin", "chmc_list_stats",
-    "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_list_check", "chmc_get_list_check", "chmc_set_speculation", "chmc_get_speculation", "chmc_set_list_start_rebuild", "chmc_get_list_start_rebuild", "chmc_set_trajectory", "chmc_get_trajectory",
+    "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_list_entry_bits", "chmc_get_list_entry_bits", "chmc_set_list_check", "chmc_get_list_check", "chmc_set_speculation", "chmc_get_speculation", "chmc_set_list_start_rebuild", "chmc_get_list_start_rebuild", "chmc_set_trajectory", "chmc_get_trajectory",
     "chmc_surrogate_set_params", "chmc_surrogate_get_params",
     "chmc_surrogate_eval", "chmc_surrogate_features", "chmc_surrogate_reserve", "chmc_surrogate_collect",
     "chmc_surrogate_training_data", "chmc_surrogate_loss_grad", "chmc_surrogate_train",
@@ -513,6 +515,15 @@ class Engine:
     def list_lanes(self):
         v = _I()
         self._ck(lib().chmc_get_list_lanes(self._h, C.byref(v)), "get_list_lanes")
+        return v.value
+
+    def set_list_entry_bits(self, bits=0):
+        """List entry format: 16 (segment-relative), 32 (slot), or 0 (16 where the rows exceed 256 MiB)."""
+        self._ck(lib().chmc_set_list_entry_bits(self._h, int(bits)), "set_list_entry_bits")
+
+    def list_entry_bits(self):
+        v = _I()
+        self._ck(lib().chmc_get_list_entry_bits(self._h, C.byref(v)), "get_list_entry_bits")
         return v.value
 
     def set_list_check(self, every):

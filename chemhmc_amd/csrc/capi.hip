@@ -70,6 +70,7 @@ struct chmc_ctx {
     std::vector<double> chain_skin;  // per-chain skins when set individually (empty: skin for all)
     bool list_ready = false;
     int list_check = 1;          // rebuild checks before every list_check-th force evaluation of list MD
+    int entry_bits = 0;          // list entry format: 16, 32, or 0 (chosen by the lists' size)
     int lpar = 0;                // list parity: which FP32 coordinate buffer / rebuild flag is current
     int cpar = 0;                // list count buffer of the next rebuild launch (alternates per launch)
     bool list_valid = false;     // the lists and their rebuild flags account for every position change
@@ -742,11 +743,27 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
         cd.ntiles = (int)c->htiles.size() - cd.tile_first;
     }
     if (lcells > (1LL << 28) || dense > (1LL << 30)) return fail(c, CHMC_E_ARG, "list grids too large");
+    // entry format: 16-bit segment-relative entries halve the rows' bytes and pay where the rows no
+    // longer fit the 256 MiB last-level cache (the sweep reads every row once per force evaluation);
+    // below that the 32-bit slot entries decode in fewer instructions
+    double row_bytes = 0.0;                   // 4 B per expected entry: pairs within rc + skin at the chain's density
+    for (int k = 0; k < c->nchains; k++) {
+        const ChainDev& cd = c->hchains[k];
+        double rl = cd.rc + (skins.empty() ? skin : skins[k]);
+        for (int d = 0; d < 3; d++) rl = std::min(rl, cd.L[d]);
+        const double rho = cd.natoms / (cd.L[0] * cd.L[1] * cd.L[2]);
+        row_bytes += 4.0 * cd.natoms * (4.0 / 3.0 * M_PI * rl * rl * rl * rho);
+    }
+    // (32-bit entries hold slot << 6: fewer than 2^25 slots)
+    const bool slots32 = (size_t)c->Ntot < ((size_t)1 << (31 - kImageBits));
+    if (c->entry_bits == 32 && !slots32) return fail(c, CHMC_E_ARG, "32-bit list entries: fewer than 2^25 atoms");
+    const bool e16 = c->entry_bits == 16 || (c->entry_bits == 0 && (row_bytes > 256.0 * (1 << 20) || !slots32));
     const int nbcap = std::max(((int)std::ceil(nbmax) + 31) / 32 * 32, (kListSpan + 31) / 32 * 32);
     const size_t N = (size_t)c->Ntot;
     if ((double)N * nbcap >= 2147483647.0) return fail(c, CHMC_E_ARG, "neighbour lists too large (reduce the skin)");
-    if (N >= (size_t)1 << (31 - kImageBits)) return fail(c, CHMC_E_ARG, "too many atoms for the list entry format");
+    if (N >= (size_t)1 << 30) return fail(c, CHMC_E_ARG, "too many atoms for the list slots");
     DevView& v = c->v;
+    v.entry16 = e16 ? 1 : 0;
     v.Cl = (int)lcells;
     v.ntiles = (int)c->htiles.size();
     v.nbcap = nbcap;
@@ -772,8 +789,11 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
     v.ldense = (int)dense;
     HIPCHK(c, lalloc(c, &v.lperm, 2 * N));
     HIPCHK(c, lalloc(c, &v.linv, 2 * N));
-    HIPCHK(c, lalloc(c, &v.nb, 2 * N * nbcap));
+    v.nb_stride = N * nbcap * (e16 ? 2 : 4);
+    HIPCHK(c, lalloc(c, &v.nb, 2 * v.nb_stride));
     HIPCHK(c, lalloc(c, &v.nbn, 2 * N));
+    HIPCHK(c, lalloc(c, &v.lcs, 2 * N));
+    HIPCHK(c, lalloc(c, &v.ctab, 2 * (size_t)lcells * kSegStride));
     HIPCHK(c, lalloc(c, &v.lbuf, 3 * (size_t)c->nchains));
     HIPCHK(c, lalloc(c, &v.tcur, (size_t)v.ntiles));
     HIPCHK(c, lalloc(c, &v.lflag, 4));
@@ -789,6 +809,9 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
     HIPCHK(c, hipMemcpy(tiles, c->htiles.data(), sizeof(Chunk) * c->htiles.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_chains, c->hchains.data(), sizeof(ChainDev) * c->nchains, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(v.lcount, 0, sizeof(int) * 2 * lcells));
+    // before the first rebuild every slot's cell is its chain's first and every segment starts at slot 0
+    HIPCHK(c, hipMemset(v.lcs, 0, sizeof(int) * 2 * N));
+    HIPCHK(c, hipMemset(v.ctab, 0, sizeof(int) * 2 * (size_t)lcells * kSegStride));
     HIPCHK(c, hipMemset(v.lflag, 0, sizeof(int) * 4));
     HIPCHK(c, hipMemset(v.lreq, 0, sizeof(int) * 2 * (c->nchains + 1)));
     HIPCHK(c, hipMemset(v.lbuf, 0, sizeof(int) * 2 * c->nchains));
@@ -802,7 +825,7 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
     for (int b = 0; b < 2; b++)
         HIPCHK(c, hipMemcpy(v.loffs + (size_t)b * (lcells + 1), lo0.data(), sizeof(int) * lo0.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemset(v.nbn, 0, sizeof(int) * 2 * N));
-    HIPCHK(c, launch_list_pad_init(v, c->stream));      // every entry a no-op entry of its row's slot
+    HIPCHK(c, launch_list_pad_init(v, c->stream));      // every entry a no-op entry
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // valid atom ids in every slot before the first rebuild (a slot is rewritten by each rebuild)
     std::vector<int> iota(N);
@@ -2308,6 +2331,24 @@ extern "C" int chmc_get_list_check(const chmc_ctx* c, int* every)
 {
     if (!c || !every) return CHMC_E_ARG;
     *every = c->list_check;
+    return CHMC_OK;
+}
+
+extern "C" int chmc_set_list_entry_bits(chmc_ctx* c, int bits)
+{
+    CHECK_CTX(c);
+    if (bits != 0 && bits != 16 && bits != 32) return fail(c, CHMC_E_ARG, "set_list_entry_bits: 0, 16 or 32");
+    c->entry_bits = bits;
+    if (!c->list_ready) return CHMC_OK;
+    int rc = flush_drift(c);
+    if (rc) return rc;
+    return list_setup(c, c->skin, c->chain_skin);
+}
+
+extern "C" int chmc_get_list_entry_bits(const chmc_ctx* c, int* bits)
+{
+    if (!c || !bits) return CHMC_E_ARG;
+    *bits = c->list_ready ? (c->v.entry16 ? 16 : 32) : c->entry_bits;
     return CHMC_OK;
 }
 

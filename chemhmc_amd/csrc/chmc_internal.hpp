@@ -39,14 +39,23 @@
 //                                              bx | by << 10 | bz << 20 in w)
 //   xsl           : float4 [2][Ntot] by slot  (current position relative to the rebuild cell origin,
 //                                              FP32, atom id in w; double-buffered across drifts)
-//   nb            : int [Ntot][nbcap]         (j slot << 6 | image code: 2 bits per dimension, the
-//                                              neighbour cell offset + 1, bits 0-1 x, 2-3 y, 4-5 z)
+//   nb            : [Ntot][nbcap] list entries in one of two formats (entry16, chosen per context):
+//                   32-bit: j slot << 6 | image code (2 bits per dimension, the neighbour cell offset + 1,
+//                           bits 0-1 x, 2-3 y, 4-5 z; code kFarCode: no-op padding)
+//                   16-bit: segment k << 8 | rank r (the neighbour is member r, in slot order, of
+//                           segment k of the slot's list cell, k = 0..26 the 27 neighbour cells at
+//                           offsets (k % 3 - 1, k / 3 % 3 - 1, k / 9 - 1); k = kPadSeg: no-op padding)
 //   nbn           : int [Ntot]
+//   lcs           : int [Ntot] by slot        (16-bit entries: the slot's list cell, relative to its
+//                                              chain's first)
+//   ctab          : int [Cl][kSegStride]      (16-bit entries: first slot of each segment of each list
+//                                              cell, the padding segments holding the cell's own first
+//                                              slot, so that neighbour slot = ctab[cell][k] + r)
 //   lreq          : int [2][1 + nchains]      (parity q: [0] any chain, [1 + k] chain k needs a
 //                                              rebuild before its next sweep; chains rebuild
 //                                              independently)
 //   lflag         : int [4]                   ([2]: capacity overflow, [3]: chain rebuilds run)
-//   The per-slot list state (loffs, lperm, linv, lref, xsl, nb, nbn) exists twice: a rebuild writes
+//   The per-slot list state (loffs, lperm, linv, lref, xsl, nb, nbn, lcs, ctab) exists twice: a rebuild writes
 //   the buffer that is not the trajectory's start buffer, so a rejected trajectory goes back to the
 //   start buffer (valid for the restored positions) instead of rebuilding; the momenta pl are left in the
 //   slot order of the buffer the trajectory ended in (lbuf[2 nchains + k], -1 when that is the current
@@ -100,10 +109,15 @@ constexpr int kMaxJ = 768;         // j-list entries staged in LDS per pass
 constexpr int kRedChunk = 256;     // atoms per deterministic reduction chunk
 constexpr int kTile = 32;          // slots per neighbour-list sweep workgroup (8 lanes per slot)
 // leading entries of every list row that the sweep reads unconditionally: the build pads a row past its
-// count up to this span with entries that contribute nothing (the slot itself with the image code
-// kFarCode, which no neighbour has: the sweep's shift table places that image beyond any cut-off)
+// count up to this span with entries that contribute nothing (32-bit: the slot itself with the image
+// code kFarCode, which no neighbour has; 16-bit: segment kPadSeg; the shift tables place either beyond
+// any cut-off)
 constexpr int kListSpan = 128;
 constexpr int kFarCode = 63;
+constexpr int kSegStride = 32;     // ctab words per list cell: 27 neighbour segments + padding segments
+constexpr int kPadSeg = 27;
+constexpr int kRankBits = 8;       // a list cell holds at most 256 members
+constexpr uint16_t kPadEntry = (uint16_t)(kPadSeg << kRankBits);
 // the unconditional first batch of each list-sweep shape (lanes per atom x entries per lane): 4 lanes
 // x 16, 8 x 14, 16 x 7, 8 x 12 ("812"), 8 x 16 ("816"); rows are padded at least this far
 __host__ __device__ constexpr int sweep_span(int shape)
@@ -135,7 +149,10 @@ struct DevView {
                                     // first batches of 12 / 16 entries)
     const int* lcell_chain;
     const Chunk* tiles;
-    int *lcount, *loffs, *lperm, *linv, *nb, *nbn, *lflag, *lreq, *lscan_tmp;
+    int *lcount, *loffs, *lperm, *linv, *nbn, *lflag, *lreq, *lscan_tmp, *lcs, *ctab;
+    char* nb;             // list rows (entry bytes: 2 with entry16, else 4)
+    size_t nb_stride;     // bytes of one buffer's rows: Ntot * nbcap * entry bytes
+    int entry16;          // 16-bit segment-relative list entries (else 32-bit slot entries)
     float4* lmem;
     double *lmemx, *lmemp; // [3][dense]
     double *xl, *pl, *xl_saved;
@@ -219,7 +236,8 @@ enum LjMode { LJ_FORCE = 1, LJ_KICK = 2, LJ_STOREF = 4, LJ_ENERGY = 8, LJ_DRIFT 
 
 // one of the two neighbour-list buffers
 struct ListBuf {
-    int *nb, *nbn, *lperm, *linv, *loffs;
+    char* nb;
+    int *nbn, *lperm, *linv, *loffs, *lcs, *ctab;
     float4* lref;         // [Ntot] by slot
     float4* xsl;          // [2 parities][Ntot] by slot
 };
@@ -227,11 +245,13 @@ __device__ __forceinline__ ListBuf list_buf(const DevView& v, int b)
 {
     const size_t N = (size_t)v.Ntot;
     ListBuf L;
-    L.nb = v.nb + (size_t)b * N * v.nbcap;
+    L.nb = v.nb + (size_t)b * v.nb_stride;
     L.nbn = v.nbn + (size_t)b * N;
     L.lperm = v.lperm + (size_t)b * N;
     L.linv = v.linv + (size_t)b * N;
     L.loffs = v.loffs + (size_t)b * ((size_t)v.Cl + 1);
+    L.lcs = v.lcs + (size_t)b * N;
+    L.ctab = v.ctab + (size_t)b * v.Cl * kSegStride;
     L.lref = v.lref + (size_t)b * N;
     L.xsl = v.xsl + (size_t)b * 2 * N;
     return L;
@@ -256,7 +276,14 @@ __device__ __forceinline__ int xcd_block(int b, int n)
     return x * per + min(x, rem) + (b >> 3);
 }
 
-// image code of a list entry: neighbour cell offset + 1 in two bits per dimension
+// list row of slot s with entries of type E (uint16_t or int, as v.entry16 says)
+template <typename E>
+__device__ __forceinline__ E* list_row(const DevView& v, const ListBuf& L, int s)
+{
+    return reinterpret_cast<E*>(L.nb) + (size_t)s * v.nbcap;
+}
+
+// 32-bit entries: image code of the neighbour cell offset + 1, two bits per dimension
 __device__ __forceinline__ int image_code(int dx, int dy, int dz) { return (dx + 1) | ((dy + 1) << 2) | ((dz + 1) << 4); }
 // the offset of one dimension times the cell side, exactly (-side, 0 or +side: f * side - side is exact)
 __device__ __forceinline__ float image_shift(int code, int dim, float side)
@@ -264,6 +291,29 @@ __device__ __forceinline__ float image_shift(int code, int dim, float side)
     return __builtin_fmaf((float)((code >> (2 * dim)) & 3), side, -side);
 }
 constexpr int kImageBits = 6;
+
+// 16-bit entries: cell offset of list segment k in dimension dim (-1, 0, +1)
+__host__ __device__ __forceinline__ int seg_offset(int k, int dim) { return (dim == 0 ? k : dim == 1 ? k / 3 : k / 9) % 3 - 1; }
+// image shift of segment k in one dimension (the offset times the cell side, exact), or far beyond any
+// cut-off for the padding segments
+__device__ __forceinline__ float seg_shift(int k, int dim, float side)
+{
+    return k >= kPadSeg ? kFarShift : (float)seg_offset(k, dim) * side;
+}
+__device__ __forceinline__ double seg_shift64(int k, int dim, double side)
+{
+    return k >= kPadSeg ? (double)kFarShift : (double)seg_offset(k, dim) * side;
+}
+// the first slots of the kSegStride segments of slot s's list cell, copied into an LDS row by the LP
+// lanes of the slot (all in one wave); the caller orders these writes before the row's reads
+template <int LP>
+__device__ __forceinline__ void load_seg_row(const ListBuf& L, const ChainDev& C, int s, int sub, int4* row)
+{
+    const int4* src = reinterpret_cast<const int4*>(L.ctab + (size_t)(C.lcell_off + L.lcs[s]) * kSegStride);
+    for (int w = sub; w < kSegStride / 4; w += LP) row[w] = src[w];
+}
+// neighbour slot of list entry e, from the slot's segment row
+__device__ __forceinline__ int entry_slot(const int* segs, unsigned e) { return segs[e >> kRankBits] + (int)(e & 255u); }
 
 // FP64 position of coordinate x (dimension d) relative to the rebuild cell origin of reference ref,
 // taken through the periodic image nearest the rebuild position r0 (an atom moves far less than L / 2

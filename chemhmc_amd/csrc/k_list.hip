@@ -16,12 +16,14 @@
 //             ranked by atom id give the compact slots and their slot data (x0, r0, FP32 xsl); the 27
 //             neighbour cells' members, ranked the same way, are staged in LDS in slot order with the
 //             image shift dd * lsize folded in (the way k_lj stages its j-list), and each own atom's
-//             list of the entries within rc + skin is appended in slot order; rows are then padded
-//             past their count to kListSpan entries with no-op entries (the slot itself with the
-//             image code kFarCode)
-// Sweep: 32 slots x LP lanes per workgroup (LP = 4, 8 or 16, tuned); each lane reads its first
-//   kBatch entries without a count test (the padding), gathers the neighbours' FP32 coordinates, adds
-//   the image shift from an LDS table by 6-bit code (kFarCode: 1e18 away), walks the rest of the list,
+//             list of the entries within rc + skin is appended in slot order, as 32-bit entries (slot,
+//             image code) or, in contexts of large lists, 16-bit ones (neighbour segment, rank in that
+//             cell; the cell's segment starts go to ctab).  Rows are then padded past their count to
+//             the sweep span with no-op entries (image code kFarCode, or segment kPadSeg)
+// Sweep: 32 slots x LP lanes per workgroup (LP = 4, 8 or 16, tuned); each lane reads its first kBatch
+//   entries without a count test (the padding), gathers the neighbours' FP32 coordinates (16-bit
+//   entries: slot = segment start from the slot's LDS segment row + rank), adds the image shift from an
+//   LDS table by code or segment (padding: 1e18 away), walks the rest of the list,
 //   and the partial sums meet in a butterfly.  Three lanes of an atom then kick p in FP64, optionally
 //   drift x in FP64 (identical_particles.cpp:651-673), and lane 0 writes the FP32 coordinates of the
 //   other parity and raises the rebuild request when the atom is beyond skin / 2 from x0.
@@ -234,8 +236,8 @@ __device__ __forceinline__ void block_scan_256(int* a, int n, int* wsum)
 
 // U groups of 64 staged members from sj[j0] against the wave's own atom: the wave's hit mask straight
 // from the compares (4: ordered less, 2: ordered greater), the positions from mbcnt, the count in a
-// scalar.  Row offsets in 32 bits: Ntot * nbcap < 2^31 (checked at setup).
-template <int U>
+// scalar.  Row offsets in 32 bits: Ntot * nbcap < 2^31 (checked at setup).  Entries of 16 or 32 bits (W16).
+template <int U, bool W16>
 __device__ __forceinline__ void build_step(const float4* sj, int j0, int lane, const float4& xi, float rl2,
                                            int rowoff, int nbcap, char* __restrict__ nbb, int& count)
 {
@@ -251,8 +253,10 @@ __device__ __forceinline__ void build_step(const float4* sj, int j0, int lane, c
     for (int u = 0; u < U; u++) {
         const unsigned long long hits = __builtin_amdgcn_fcmpf(r2[u], rl2, 4) & __builtin_amdgcn_fcmpf(r2[u], 0.f, 2);
         const int pos = count + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(hits >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)hits, 0u));
-        if (r2[u] < rl2 && r2[u] > 0.f && pos < nbcap)
-            *reinterpret_cast<int*>(nbb + (unsigned)((rowoff + pos) << 2)) = __float_as_int(p[u].w);
+        if (r2[u] < rl2 && r2[u] > 0.f && pos < nbcap) {
+            if constexpr (W16) *reinterpret_cast<uint16_t*>(nbb + ((unsigned)(rowoff + pos) << 1)) = (uint16_t)__float_as_int(p[u].w);
+            else *reinterpret_cast<int*>(nbb + ((unsigned)(rowoff + pos) << 2)) = __float_as_int(p[u].w);
+        }
         count += __popcll(hits);
     }
 }
@@ -264,19 +268,19 @@ __device__ __forceinline__ void build_step(const float4* sj, int j0, int lane, c
 // group of 64 tested pairs; two own atoms x 32 members took ~18 and 6-9 % more build time).  The loop
 // reads two groups of 64 while the second starts inside the members, then at most one more, so no read
 // passes jn + 63 and the padding stays one group.
+template <bool W16>
 __device__ __forceinline__ void build_pass(const float4* sj, const float4* si, int* icount, int ni, int jn,
-                                           float rl2, int nbcap, int* __restrict__ nb)
+                                           float rl2, int nbcap, char* __restrict__ nbb)
 {
     static_assert(kJPad >= 64, "every step reads fewer than 64 entries past the members");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    char* __restrict__ nbb = reinterpret_cast<char*>(nb);
     for (int i = wave; i < ni; i += kBuildWaves) {
         const float4 xi = si[i];
         int count = icount[i];
         const int rowoff = __float_as_int(xi.w) * nbcap;
         int j0 = 0;
-        for (; j0 + 64 < jn; j0 += 128) build_step<2>(sj, j0, lane, xi, rl2, rowoff, nbcap, nbb, count);
-        if (j0 < jn) build_step<1>(sj, j0, lane, xi, rl2, rowoff, nbcap, nbb, count);
+        for (; j0 + 64 < jn; j0 += 128) build_step<2, W16>(sj, j0, lane, xi, rl2, rowoff, nbcap, nbb, count);
+        if (j0 < jn) build_step<1, W16>(sj, j0, lane, xi, rl2, rowoff, nbcap, nbb, count);
         if (lane == 0) icount[i] = count;
     }
 }
@@ -295,7 +299,7 @@ __device__ __forceinline__ void build_pass(const float4* sj, const float4* si, i
 //            is fixed by the slot order and not by the arrival order of the count.
 // A neighbourhood of more than kLMaxJ members is staged in passes of whole cells; the own cell is
 // loaded with the first pass.
-template <bool SCAN>
+template <bool SCAN, bool W16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_lbuild(DevView v, int q, int cb)
 {
     const int c = xcd_block(blockIdx.x, v.Cl);
@@ -315,6 +319,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     __shared__ int seg_cell[27], seg_beg[27], seg_cnt[27], seg_pref[28], seg_code[27], wsum[4];
     __shared__ float seg_sh[27][3];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr bool e16 = W16;              // 16-bit segment-relative entries
     const ChainDev& C = v.chains[ch];
     const int tb = *list_start(v, ch) ^ 1;
     const ListBuf L = list_buf(v, tb);
@@ -424,6 +429,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             if (tid + NT * u < jn) sid[tid + NT * u] = __float_as_int(me[u].w);
         __syncthreads();                       // seg_beg, sid, sido; the scan area is free again
         if (first) ibeg = seg_beg[13];
+        // 16-bit entries: the cell's segment starts (the padding segments: its own first slot)
+        if (e16 && first && tid < kSegStride) L.ctab[(size_t)c * kSegStride + tid] = seg_beg[tid < 27 ? tid : 13];
         if (first && tid < ni) {
             // own slot data: slots by atom id; the rebuild reference (FP32 cell-relative position, cell)
             const int a = __float_as_int(meo.w);
@@ -440,6 +447,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             L.xsl[(size_t)q * N + slot] = meo;
             L.lperm[slot] = a;
             L.linv[a] = slot;
+            if (e16) L.lcs[slot] = lc;
             tile_cur_mark(v, C, slot, tb);
             si[r] = make_float4(meo.x, meo.y, meo.z, __int_as_float(slot));
             icount[r] = 0;
@@ -454,14 +462,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 int r = 0;
 #pragma unroll 8
                 for (int w = 0; w < nk; w++) r += sid[s0 + w] < a ? 1 : 0;
-                // w holds the finished list entry: j slot << kImageBits | image code
+                // w holds the finished list entry: segment k << kRankBits | rank r, or j slot <<
+                // kImageBits | image code
+                const int e = e16 ? ((k << kRankBits) | r) : (((seg_beg[k] + r) << kImageBits) | seg_code[k]);
                 sj[s0 + r] = make_float4(me[u].x + seg_sh[k][0], me[u].y + seg_sh[k][1], me[u].z + seg_sh[k][2],
-                                         __int_as_float(((seg_beg[k] + r) << kImageBits) | seg_code[k]));
+                                         __int_as_float(e));
             }
         }
         if (tid < kJPad) sj[jn + tid] = make_float4(kFarShift, kFarShift, kFarShift, 0.f);
         __syncthreads();
-        build_pass(sj, si, icount, ni, jn, rl2, nbcap, L.nb);
+        build_pass<W16>(sj, si, icount, ni, jn, rl2, nbcap, L.nb);
         k0 = k1;
     }
     __syncthreads();
@@ -470,24 +480,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         L.nbn[ibeg + r] = min(n, nbcap);
         if (n > nbcap) atomicOr(&v.lflag[2], 2);
     }
-    // rows padded to the sweep shape's unconditional span with no-op entries (the far image of the slot)
+    // rows padded to the sweep shape's unconditional span with no-op entries
     const int span = v.lspan;
     for (int r = wave; r < ni; r += W) {
         const int slot = __float_as_int(si[r].w);
-        const int pad = (slot << kImageBits) | kFarCode;
-        int* __restrict__ row = L.nb + (size_t)slot * nbcap;
-        for (int k = icount[r] + lane; k < span; k += 64) row[k] = pad;
+        if (e16) {
+            uint16_t* __restrict__ row = list_row<uint16_t>(v, L, slot);
+            for (int k = icount[r] + lane; k < span; k += 64) row[k] = kPadEntry;
+        } else {
+            int* __restrict__ row = list_row<int>(v, L, slot);
+            const int pad = (slot << kImageBits) | kFarCode;
+            for (int k = icount[r] + lane; k < span; k += 64) row[k] = pad;
+        }
     }
 }
 
-// every list row of both buffers filled with its slot's no-op entry (before the first rebuild)
+// every list row of both buffers filled with no-op entries (before the first rebuild: 32-bit entries
+// name the slot itself; with 16-bit ones the segment starts and cells are zero then, so the padding
+// reads slot 0)
 __global__ __launch_bounds__(256) void k_lpad_init(DevView v)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t per = (size_t)v.Ntot * v.nbcap;
     if (i >= 2 * per) return;
-    const int slot = (int)((i % per) / v.nbcap);
-    v.nb[i] = (slot << kImageBits) | kFarCode;
+    if (v.entry16) {
+        reinterpret_cast<uint16_t*>(v.nb)[i] = kPadEntry;
+    } else {
+        const int slot = (int)((i % per) / v.nbcap);
+        reinterpret_cast<int*>(v.nb)[i] = (slot << kImageBits) | kFarCode;
+    }
 }
 
 // rows of both buffers padded from their counts up to span (a sweep shape with a longer first batch
@@ -498,9 +519,14 @@ __global__ __launch_bounds__(256) void k_lrepad(DevView v, int span)
     if (i >= 2 * (size_t)v.Ntot) return;
     const int b = (int)(i / v.Ntot), slot = (int)(i % v.Ntot);
     const ListBuf L = list_buf(v, b);
-    int* __restrict__ row = L.nb + (size_t)slot * v.nbcap;
-    const int pad = (slot << kImageBits) | kFarCode;
-    for (int k = max(L.nbn[slot], 0); k < span; k++) row[k] = pad;
+    if (v.entry16) {
+        uint16_t* __restrict__ row = list_row<uint16_t>(v, L, slot);
+        for (int k = max(L.nbn[slot], 0); k < span; k++) row[k] = kPadEntry;
+    } else {
+        int* __restrict__ row = list_row<int>(v, L, slot);
+        const int pad = (slot << kImageBits) | kFarCode;
+        for (int k = max(L.nbn[slot], 0); k < span; k++) row[k] = pad;
+    }
 }
 
 hipError_t launch_list_repad(const DevView& v, int span, hipStream_t s)
@@ -524,12 +550,14 @@ hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s)
     if (e != hipSuccess) return e;
     const size_t dyn = (size_t)v.lcap_max * (sizeof(float4) + 2 * sizeof(int));
     if (v.lscan_in_build) {
-        hipLaunchKernelGGL(k_lbuild<true>, dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+        if (v.entry16) hipLaunchKernelGGL((k_lbuild<true, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+        else hipLaunchKernelGGL((k_lbuild<true, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_lscan, dim3(1), dim3(1024), 0, s, v, q, cb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_lbuild<false>, dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+    if (v.entry16) hipLaunchKernelGGL((k_lbuild<false, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+    else hipLaunchKernelGGL((k_lbuild<false, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
     return hipGetLastError();
 }
 
@@ -636,18 +664,22 @@ __device__ __forceinline__ void list_tail(const DevView& v, const ListBuf& L, co
 // LP lanes per slot, kTile slots per workgroup.  Each lane first loads up to kBatch of its list entries
 // and gathers their coordinates, so that one memory latency covers most lists; the per-atom tail (kick,
 // drift, next-parity coordinates, displacement test) runs on three lanes, one per dimension.  The
-// workgroup's list buffer comes from tcur[blockIdx] and the image shifts are decoded from the entries in
-// registers, so nothing delays the first list load but the tile's own two scalar loads (no LDS table,
-// no barrier).
+// workgroup's list buffer comes from tcur[blockIdx], so nothing delays the first list load but the
+// tile's own two scalar loads.  32-bit entries (W16 false) carry the slot and the image code, so the
+// gathers are issued before the barrier that publishes the shift table.  16-bit entries (W16) are read
+// two per 32-bit load where kBatch is even, while the slot's lanes copy the segment starts of its list
+// cell to an LDS row; the gathers (slot = segment start + rank) follow the barrier.
 // KB entries per lane in the first batch: KB * LP should cover most rows (at 8 lanes, 14 covers the
 // typical 105 entries at skin 0.6 and N = 32768; 12 the 94 of skin 0.5, 16 the 116 of skin 0.7); the
 // sweep shape (LP, KB) is chosen by the autotuner
-template <int MODE, int LP, int KB>
+template <int MODE, int LP, int KB, bool W16>
 __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt, double drift_dt, int q, int checked)
 {
     constexpr int kBatch = KB;
     static_assert(kBatch * LP <= kListSpan, "the first batch reads padded entries only");
-    __shared__ float4 shift_tab[64];                     // image shift by 6-bit image code
+    constexpr int kShifts = W16 ? kSegStride : 64;
+    __shared__ float4 shift_tab[kShifts];                // image shift by segment (W16) or 6-bit image code
+    __shared__ int4 seg_tab[W16 ? kTile : 1][kSegStride / 4];   // W16: segment starts of each slot's cell
     const int tile = xcd_block(blockIdx.x, v.ntiles);
     const Chunk tl = v.tiles[tile];
     const ListBuf L = list_buf(v, v.tcur[tile]);
@@ -658,13 +690,23 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
     const bool val = s < tl.end;
     const int d = sub < 3 ? sub : 0;                      // the per-dimension tail runs on lanes 0, 1, 2
     const float4* __restrict__ xq = L.xsl + (size_t)q * N;
+    const int* segs = reinterpret_cast<const int*>(seg_tab[W16 ? t : 0]);
     float ax = 0.f, ay = 0.f, az = 0.f;
     float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
     int atom = 0;
     double p_d = 0.0, x_d = 0.0;
     float4 ref = make_float4(0.f, 0.f, 0.f, 0.f);
-    auto pair = [&](const float4 pj, int code) {
-        const float4 sh = shift_tab[code];
+    // neighbour slot and shift-table index of an entry
+    auto slot_of = [&](unsigned e) -> unsigned {
+        if constexpr (W16) return (unsigned)entry_slot(segs, e);
+        else return e >> kImageBits;
+    };
+    auto shift_of = [&](unsigned e) -> unsigned {
+        if constexpr (W16) return e >> kRankBits;
+        else return e & 63u;
+    };
+    auto pair = [&](const float4 pj, unsigned e) {
+        const float4 sh = shift_tab[shift_of(e)];
         const float dx = xi.x - (pj.x + sh.x);
         const float dy = xi.y - (pj.y + sh.y);
         const float dz = xi.z - (pj.z + sh.z);
@@ -678,36 +720,66 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
         ay = __builtin_fmaf(gq, dy, ay);
         az = __builtin_fmaf(gq, dz, az);
     };
-    if (threadIdx.x < 64) {
-        const int code = threadIdx.x;
-        shift_tab[code] = code == kFarCode
-                              ? make_float4(kFarShift, kFarShift, kFarShift, 0.f)
-                              : make_float4(image_shift(code, 0, C.lsize_f[0]), image_shift(code, 1, C.lsize_f[1]),
-                                            image_shift(code, 2, C.lsize_f[2]), 0.f);
+    if (threadIdx.x < kShifts) {
+        const int k = threadIdx.x;
+        if constexpr (W16) {
+            shift_tab[k] = make_float4(seg_shift(k, 0, C.lsize_f[0]), seg_shift(k, 1, C.lsize_f[1]),
+                                       seg_shift(k, 2, C.lsize_f[2]), 0.f);
+        } else {
+            shift_tab[k] = k == kFarCode ? make_float4(kFarShift, kFarShift, kFarShift, 0.f)
+                                         : make_float4(image_shift(k, 0, C.lsize_f[0]), image_shift(k, 1, C.lsize_f[1]),
+                                                       image_shift(k, 2, C.lsize_f[2]), 0.f);
+        }
     }
-    int ent[kBatch];
-    float4 pj[kBatch];
+    unsigned ent[kBatch];
+    float4 pj[W16 ? 1 : kBatch];                          // 32-bit entries: gathered before the barrier
     int n = 0;
     if (val) {
-        const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
         // the first kBatch * LP entries are read unconditionally, before the count is known: past the
         // count the build pads them with no-op entries (nbcap >= kListSpan)
+        if constexpr (W16) {
+            const uint16_t* __restrict__ nbs = list_row<uint16_t>(v, L, s);
+            if constexpr (kBatch % 2 == 0) {
+                const unsigned* __restrict__ nb2 = reinterpret_cast<const unsigned*>(nbs);   // nbcap is even
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) ent[u] = nbs[sub + u * LP];
+                for (int u = 0; u < kBatch / 2; u++) {
+                    const unsigned w = nb2[sub + u * LP];
+                    ent[2 * u] = w & 0xffffu;
+                    ent[2 * u + 1] = w >> 16;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < kBatch; u++) ent[u] = nbs[sub + u * LP];
+            }
+            load_seg_row<LP>(L, C, s, sub, seg_tab[t]);
+        } else {
+            const int* __restrict__ nbs = list_row<int>(v, L, s);
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) ent[u] = (unsigned)nbs[sub + u * LP];
+        }
         if constexpr ((MODE & LJ_STOREF) != 0) atom = L.lperm[s];     // the stored force is in atom order
         xi = xq[s];
         n = L.nbn[s];
+        if constexpr (!W16) {
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) pj[u] = xq[(unsigned)ent[u] >> kImageBits];
+            for (int u = 0; u < kBatch; u++) pj[u] = xq[slot_of(ent[u])];
+        }
     }
-    __syncthreads();                                      // shift_tab
+    __syncthreads();                                      // shift_tab, seg_tab
     if (val) {
+        if constexpr (W16) {
+            float4 pw[kBatch];
 #pragma unroll
-        for (int u = 0; u < kBatch; u++) pair(pj[u], ent[u] & 63);
-        const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
+            for (int u = 0; u < kBatch; u++) pw[u] = xq[slot_of(ent[u])];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) pair(pw[u], ent[u]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) pair(pj[u], ent[u]);
+        }
         for (int k = sub + kBatch * LP; k < n; k += LP) {
-            const int ee = nbs[k];
-            pair(xq[(unsigned)ee >> kImageBits], ee & 63);
+            const unsigned ee = W16 ? (unsigned)list_row<uint16_t>(v, L, s)[k] : (unsigned)list_row<int>(v, L, s)[k];
+            pair(xq[slot_of(ee)], ee);
         }
     }
     // the tail's operands, fetched after the pairs (fewer registers live through the pair loop)
@@ -736,11 +808,12 @@ __global__ __launch_bounds__(kTile * LP) void k_lsweep(DevView v, double kick_dt
 // exp(-(t - D)^2 c) = exp(-t^2 c) rho, rho = exp(2 c D t - c D^2), and rho advances by exp(-2 c D^2).
 constexpr int kSurrPairCap = 104;   // pairs within rc per atom staged in LDS (3 workgroups per CU)
 
-template <int MODE, int LP>
+template <int MODE, int LP, bool W16>
 __global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const float* __restrict__ g, double kick_dt,
                                                             double drift_dt, int q, int checked)
 {
     __shared__ float4 spd[kTile][kSurrPairCap];          // (dx, dy, dz, neighbour slot bits) of the pairs within rc
+    __shared__ int4 seg_tab[W16 ? kTile : 1][kSegStride / 4];   // W16: segment starts of each slot's cell
     const int tile = xcd_block(blockIdx.x, v.ntiles);
     const Chunk tl = v.tiles[tile];
     const ListBuf L = list_buf(v, v.tcur[tile]);
@@ -765,9 +838,16 @@ __global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const flo
                 ref = L.lref[s];
             }
         }
-        // pass 1: the pairs within rc, in entry order
-        np = stage_pairs<LP, 16, true>(C, xq, L.nb + (size_t)s * v.nbcap, L.nbn[s], xq[s], sub, &spd[t][0], nullptr,
-                                       nullptr, kSurrPairCap);
+        // pass 1: the pairs within rc, in entry order (16-bit entries: the slot's lanes share a wave, and
+        // a wave barrier orders the segment row's writes before its reads)
+        if constexpr (W16) {
+            load_seg_row<LP>(L, C, s, sub, seg_tab[t]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        np = stage_pairs<LP, 16, true, W16>(C, xq, L.nb, v.nbcap, s, reinterpret_cast<const int*>(seg_tab[W16 ? t : 0]),
+                                            L.nbn[s], xq[s], sub, &spd[t][0], nullptr, kSurrPairCap);
         if (np > kSurrPairCap) {
             if (sub == 0) atomicOr(&v.lflag[2], 8);
             np = kSurrPairCap;
@@ -880,59 +960,68 @@ __device__ __forceinline__ double tile_sum_256(double s, double* part)
     return ((part[0] + part[1]) + part[2]) + part[3];
 }
 
+template <bool W16>
 __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
 {
+    constexpr int kShifts = W16 ? kSegStride : 64;
     __shared__ double wpart[4];
-    __shared__ double shift64[64][3];                   // image shift by 6-bit image code, exact in FP64
+    __shared__ double shift64[kShifts][3];              // image shift by segment or image code, exact in FP64
+    __shared__ int4 seg_tab[W16 ? kTile : 1][kSegStride / 4];   // W16: segment starts of each slot's cell
     const int tile = xcd_block(blockIdx.x, v.ntiles);
     const Chunk tl = v.tiles[tile];
     const ChainDev& C = v.chains[tl.chain];
-    if (threadIdx.x < 64) {
-        const int code = threadIdx.x;
-        shift64[code][0] = (double)((code & 3) - 1) * C.lsize[0];
-        shift64[code][1] = (double)(((code >> 2) & 3) - 1) * C.lsize[1];
-        shift64[code][2] = (double)(((code >> 4) & 3) - 1) * C.lsize[2];
+    const ListBuf L = list_buf(v, *list_cur(v, tl.chain));
+    const int t = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    const int s = tl.begin + t;
+    const bool val = s < tl.end;
+    if (threadIdx.x < kShifts) {
+        const int k = threadIdx.x;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            shift64[k][d] = W16 ? seg_shift64(k, d, C.lsize[d]) : (double)(((k >> (2 * d)) & 3) - 1) * C.lsize[d];
+    }
+    if constexpr (W16) {
+        if (val) load_seg_row<8>(L, C, s, sub, seg_tab[t]);
     }
     __syncthreads();
-    const ListBuf L = list_buf(v, *list_cur(v, tl.chain));
     if (threadIdx.x == 0) {
         if (tl.begin == C.atom_off) *req(v, q, tl.chain) = 0;
         if (blockIdx.x == 0) *req_any(v, q) = 0;
     }
     const int N = v.Ntot;
-    const int t = threadIdx.x >> 3, sub = threadIdx.x & 7;
-    const int s = tl.begin + t;
-    const bool val = s < tl.end;
+    const int* segs = reinterpret_cast<const int*>(seg_tab[W16 ? t : 0]);
     double au = 0.0, aw = 0.0, an = 0.0, ak = 0.0;
     if (val) {
         const double2* __restrict__ r64 = reinterpret_cast<const double2*>(v.rel64);   // [Ntot][4]: x, y | z, -
         const double2 ia = r64[2 * (size_t)s], ib = r64[2 * (size_t)s + 1];
         const double xi = ia.x, yi = ia.y, zi = ib.x;
         const int n = L.nbn[s];
-        const int* __restrict__ nbs = L.nb + (size_t)s * v.nbcap;
+        using E = std::conditional_t<W16, uint16_t, int>;
+        const E* __restrict__ nbs = list_row<E>(v, L, s);
         // sqrt(r2) < rc (the reference's test) decided without the square root away from the cut-off:
         // the FP64 square root is within one ulp, far inside a 1e-12 relative band
         const double lo2 = C.rc2 * (1.0 - 1e-12), hi2 = C.rc2 * (1.0 + 1e-12);
         // batches of kEB entries per lane: the batch's entries, then its neighbours' coordinates, are
         // loaded together (two memory round trips per batch); each lane still adds its entries in
-        // increasing order
+        // increasing order.  Entries past the count read the row's last one (with 16-bit entries only
+        // those below the count, and the padding, decode to slots of this buffer).
         constexpr int kEB = 8;
         for (int k0 = sub; k0 < n; k0 += 8 * kEB) {
-            int ent[kEB];
+            unsigned ent[kEB];
 #pragma unroll
-            for (int u = 0; u < kEB; u++) ent[u] = nbs[min(k0 + 8 * u, v.nbcap - 1)];
+            for (int u = 0; u < kEB; u++) ent[u] = (unsigned)nbs[min(k0 + 8 * u, n - 1)];
             double2 ja[kEB];
             double jz[kEB];
 #pragma unroll
             for (int u = 0; u < kEB; u++) {
-                const size_t j = (unsigned)ent[u] >> kImageBits;
+                const size_t j = W16 ? (size_t)entry_slot(segs, ent[u]) : (size_t)(ent[u] >> kImageBits);
                 ja[u] = r64[2 * j];
                 jz[u] = v.rel64[4 * j + 2];
             }
 #pragma unroll
             for (int u = 0; u < kEB; u++) {
                 const bool live = k0 + 8 * u < n;
-                const double* sh = shift64[ent[u] & 63];
+                const double* sh = shift64[W16 ? ent[u] >> kRankBits : ent[u] & 63u];
                 const double dx = xi - (ja[u].x + sh[0]);
                 const double dy = yi - (ja[u].y + sh[1]);
                 const double dz = zi - (jz[u] + sh[2]);
@@ -1019,22 +1108,29 @@ hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s)
 // ---- launchers -----------------------------------------------------------------------------------
 // sweep shape: 4, 8, 16 lanes per slot with first batches of 16, 14, 7 entries, or 812 / 816 (8 lanes with
 // batches of 12 / 16)
-template <int MODE>
-static hipError_t sweep_lp(const DevView& v, int shape, int q, double kick_dt, double drift_dt, int ck, hipStream_t s)
+template <int MODE, bool W16>
+static hipError_t sweep_lp_fmt(const DevView& v, int shape, int q, double kick_dt, double drift_dt, int ck, hipStream_t s)
 {
     const dim3 g(v.ntiles);
-    static_assert(sweep_span(4) == 4 * 16 && sweep_span(8) == 8 * 14 && sweep_span(16) == 16 * 7 &&
-                  sweep_span(812) == 8 * 12 && sweep_span(816) == 8 * 16, "shape spans");
-    if (sweep_span(shape) > v.lspan) return hipErrorInvalidValue;    // rows not padded that far
     switch (shape) {
-    case 4: hipLaunchKernelGGL((k_lsweep<MODE, 4, 16>), g, dim3(kTile * 4), 0, s, v, kick_dt, drift_dt, q, ck); break;
-    case 8: hipLaunchKernelGGL((k_lsweep<MODE, 8, 14>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
-    case 16: hipLaunchKernelGGL((k_lsweep<MODE, 16, 7>), g, dim3(kTile * 16), 0, s, v, kick_dt, drift_dt, q, ck); break;
-    case 812: hipLaunchKernelGGL((k_lsweep<MODE, 8, 12>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
-    case 816: hipLaunchKernelGGL((k_lsweep<MODE, 8, 16>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 4: hipLaunchKernelGGL((k_lsweep<MODE, 4, 16, W16>), g, dim3(kTile * 4), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 8: hipLaunchKernelGGL((k_lsweep<MODE, 8, 14, W16>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 16: hipLaunchKernelGGL((k_lsweep<MODE, 16, 7, W16>), g, dim3(kTile * 16), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 812: hipLaunchKernelGGL((k_lsweep<MODE, 8, 12, W16>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 816: hipLaunchKernelGGL((k_lsweep<MODE, 8, 16, W16>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t sweep_lp(const DevView& v, int shape, int q, double kick_dt, double drift_dt, int ck, hipStream_t s)
+{
+    static_assert(sweep_span(4) == 4 * 16 && sweep_span(8) == 8 * 14 && sweep_span(16) == 16 * 7 &&
+                  sweep_span(812) == 8 * 12 && sweep_span(816) == 8 * 16, "shape spans");
+    if (sweep_span(shape) > v.lspan) return hipErrorInvalidValue;    // rows not padded that far
+    return v.entry16 ? sweep_lp_fmt<MODE, true>(v, shape, q, kick_dt, drift_dt, ck, s)
+                     : sweep_lp_fmt<MODE, false>(v, shape, q, kick_dt, drift_dt, ck, s);
 }
 
 hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, double drift_dt, bool checked,
@@ -1055,32 +1151,45 @@ hipError_t launch_list_sweep(const DevView& v, int mode, int q, double kick_dt, 
 }
 
 // the surrogate sweep runs 8 lanes per slot whatever the LJ sweep's tuning
-hipError_t launch_lsurr_sweep(const DevView& v, const float* g, int mode, int q, double kick_dt, double drift_dt,
-                              bool checked, hipStream_t s)
+template <bool W16>
+static hipError_t lsurr_sweep_fmt(const DevView& v, const float* g, int mode, int q, double kick_dt, double drift_dt,
+                                  int ck, hipStream_t s)
 {
     const dim3 grid(v.ntiles), block(kTile * 8);
-    const int ck = checked ? 1 : 0;
     switch (mode) {
     case LJ_FORCE | LJ_KICK:
-        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK, 8>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck); break;
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK, 8, W16>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck);
+        break;
     case LJ_FORCE | LJ_KICK | LJ_STOREF:
-        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_STOREF, 8>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck);
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_STOREF, 8, W16>), grid, block, 0, s, v, g, kick_dt, drift_dt,
+                           q, ck);
         break;
     case LJ_FORCE | LJ_KICK | LJ_DRIFT:
-        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_DRIFT, 8>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck);
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_DRIFT, 8, W16>), grid, block, 0, s, v, g, kick_dt, drift_dt,
+                           q, ck);
         break;
     case LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT:
-        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT, 8>), grid, block, 0, s, v, g, kick_dt,
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_STOREF | LJ_DRIFT, 8, W16>), grid, block, 0, s, v, g, kick_dt,
                            drift_dt, q, ck);
         break;
     case LJ_FORCE | LJ_STOREF:
-        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_STOREF, 8>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck); break;
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_STOREF, 8, W16>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck);
+        break;
     case LJ_FORCE | LJ_KICK | LJ_REL64:
-        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_REL64, 8>), grid, block, 0, s, v, g, kick_dt, drift_dt, q, ck);
+        hipLaunchKernelGGL((k_lsurr_sweep<LJ_FORCE | LJ_KICK | LJ_REL64, 8, W16>), grid, block, 0, s, v, g, kick_dt, drift_dt,
+                           q, ck);
         break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_lsurr_sweep(const DevView& v, const float* g, int mode, int q, double kick_dt, double drift_dt,
+                              bool checked, hipStream_t s)
+{
+    const int ck = checked ? 1 : 0;
+    return v.entry16 ? lsurr_sweep_fmt<true>(v, g, mode, q, kick_dt, drift_dt, ck, s)
+                     : lsurr_sweep_fmt<false>(v, g, mode, q, kick_dt, drift_dt, ck, s);
 }
 
 hipError_t launch_list_drift(const DevView& v, int q, double dt, hipStream_t s)
@@ -1096,7 +1205,8 @@ hipError_t launch_list_energy(const DevView& v, int q, bool rel64, hipStream_t s
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_lenergy, dim3(v.ntiles), dim3(256), 0, s, v, q);
+    if (v.entry16) hipLaunchKernelGGL(k_lenergy<true>, dim3(v.ntiles), dim3(256), 0, s, v, q);
+    else hipLaunchKernelGGL(k_lenergy<false>, dim3(v.ntiles), dim3(256), 0, s, v, q);
     return hipGetLastError();
 }
 

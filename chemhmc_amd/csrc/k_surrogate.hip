@@ -592,12 +592,14 @@ __device__ __forceinline__ void act_times(MlpSmem& sm, const __bf16* B, f32x4 ou
 // waves 0 and 1 then run the MLP forward pass and its input gradient on 16 slots each (as k_surr_mlp).
 // Slots follow the current list buffer of their chain; groups of 32 slots are mapped XCD-contiguously.
 constexpr int kDescPairCap = 128;   // pairs within rc per atom staged in LDS
+template <bool W16>
 __global__ __launch_bounds__(256) void k_lsurr_desc_mlp(DevView v, SurrView s, int q, float* __restrict__ g_out,
                                                         float* __restrict__ e_out, float* __restrict__ G_out)
 {
     __shared__ MlpSmem sm;
     __shared__ float sG[32][kSurrFeat];
     __shared__ float srd[32][kDescPairCap];
+    __shared__ int4 seg_tab[W16 ? 32 : 1][kSegStride / 4];   // W16: segment starts of each slot's cell
     load_weights(sm, s);
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ngroups = (v.Ntot + 31) / 32;
@@ -614,8 +616,14 @@ __global__ __launch_bounds__(256) void k_lsurr_desc_mlp(DevView v, SurrView s, i
         const ListBuf L = list_buf(v, *list_cur(v, ch));
         const float4* __restrict__ xq = L.xsl + (size_t)q * N;
         const int n = L.nbn[slot];
-        int np = stage_pairs<8, 16, false>(C, xq, L.nb + (size_t)slot * v.nbcap, n, xq[slot], sub, nullptr, nullptr,
-                                           &srd[a][0], kDescPairCap);
+        if constexpr (W16) {
+            load_seg_row<8>(L, C, slot, sub, seg_tab[a]);      // the slot's 8 lanes share a wave
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        int np = stage_pairs<8, 16, false, W16>(C, xq, L.nb, v.nbcap, slot, reinterpret_cast<const int*>(seg_tab[W16 ? a : 0]),
+                                                n, xq[slot], sub, nullptr, &srd[a][0], kDescPairCap);
         if (np > kDescPairCap) {
             if (sub == 0) atomicOr(s.flag, 1);
             np = kDescPairCap;
@@ -1090,7 +1098,8 @@ hipError_t launch_surr_fm(const DevView& v, const SurrView& s, const float* G, c
 
 hipError_t launch_lsurr_desc_mlp(const DevView& v, const SurrView& s, int q, float* g, float* e, float* G, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_lsurr_desc_mlp, dim3((v.Ntot + 31) / 32), dim3(256), 0, st, v, s, q, g, e, G);
+    if (v.entry16) hipLaunchKernelGGL(k_lsurr_desc_mlp<true>, dim3((v.Ntot + 31) / 32), dim3(256), 0, st, v, s, q, g, e, G);
+    else hipLaunchKernelGGL(k_lsurr_desc_mlp<false>, dim3((v.Ntot + 31) / 32), dim3(256), 0, st, v, s, q, g, e, G);
     return hipGetLastError();
 }
 

@@ -355,8 +355,9 @@ __global__ __launch_bounds__(256) void k_spec_restore(DevView v, const double* _
 // commit: advance the counter by the consumed trajectories and give every copy the chain state after
 // the batch; then, when a copy j was accepted (or src >= 0 names one), copy its state into every other
 // copy: positions and start positions by slot and its current list buffer (rows, counts, slot <-> atom
-// maps, rebuild references, FP32 coordinates of both parities, cell offsets, rebuild requests), with
-// slot and atom numbers moved to the target copy's range, so that every copy continues exactly as
+// maps, rebuild references, FP32 coordinates of both parities, slot cells, cell offsets and segment
+// starts, rebuild requests), with slot and atom numbers moved to the target copy's range, so that every
+// copy continues exactly as
 // copy j would.  The momenta are not copied: the next heat bath draws them.
 __global__ __launch_bounds__(256) void k_spec_broadcast(DevView v, int src, int commit)
 {
@@ -387,6 +388,7 @@ __global__ __launch_bounds__(256) void k_spec_broadcast(DevView v, int src, int 
         L.lperm[s] = L.lperm[sj] + delta;
         L.linv[s] = L.linv[sj] + delta;          // atom Ck.atom_off + r: a chain's atoms and slots share a range
         L.lref[s] = L.lref[sj];
+        if (v.entry16) L.lcs[s] = L.lcs[sj];     // cells relative to the chain's first
         L.xsl[s] = L.xsl[sj];
         L.xsl[(size_t)Nt + s] = L.xsl[(size_t)Nt + sj];
         tile_cur_mark(v, Ck, (int)s, b);
@@ -397,25 +399,42 @@ __global__ __launch_bounds__(256) void k_spec_broadcast(DevView v, int src, int 
             for (int q = 0; q < 2; q++) lreq[q * (v.nchains + 1) + 1 + k] = lreq[q * (v.nchains + 1) + 1 + j];
         }
     }
-    // list rows, four entries per thread, up to the padded span or the count
-    const int q4 = v.nbcap / 4, span = v.lspan;
-    for (size_t u = gid; u < (size_t)Nt * q4; u += stride) {
-        const int s = (int)(u / q4), c4 = (int)(u % q4);
+    // list rows, 16 bytes per thread (four 32-bit entries with their slots moved by the copies' offset,
+    // or eight 16-bit ones, relative to the segment starts, as they are), up to the padded span or the count
+    const bool e16 = v.entry16 != 0;
+    const int per = e16 ? 8 : 4, qn = v.nbcap / per, span = v.lspan;
+    const size_t rowb = (size_t)v.nbcap * (e16 ? 2 : 4);
+    for (size_t u = gid; u < (size_t)Nt * qn; u += stride) {
+        const int s = (int)(u / qn), cq = (int)(u % qn);
         const int k = v.atom_chain[s];
         if (k == j) continue;
         const ChainDev& Ck = v.chains[k];
         const int sj = Cj.atom_off + (s - Ck.atom_off);
-        if (4 * c4 >= max(L.nbn[sj], span)) continue;
-        const int shift = (Ck.atom_off - Cj.atom_off) * (1 << kImageBits);
-        int4 e = reinterpret_cast<const int4*>(L.nb + (size_t)sj * v.nbcap)[c4];
-        e.x += shift; e.y += shift; e.z += shift; e.w += shift;
-        reinterpret_cast<int4*>(L.nb + (size_t)s * v.nbcap)[c4] = e;
+        if (per * cq >= max(L.nbn[sj], span)) continue;
+        int4 e = reinterpret_cast<const int4*>(L.nb + (size_t)sj * rowb)[cq];
+        if (!e16) {
+            const int shift = (Ck.atom_off - Cj.atom_off) * (1 << kImageBits);
+            e.x += shift; e.y += shift; e.z += shift; e.w += shift;
+        }
+        reinterpret_cast<int4*>(L.nb + (size_t)s * rowb)[cq] = e;
     }
+    // cell offsets and (16-bit entries) segment starts, moved to copy k's slot range
     for (size_t c = gid; c < (size_t)v.Cl; c += stride) {
         const int k = v.lcell_chain[c];
         if (k == j) continue;
         const ChainDev& Ck = v.chains[k];
-        L.loffs[c] = L.loffs[c - Ck.lcell_off + Cj.lcell_off] + (Ck.atom_off - Cj.atom_off);
+        const int delta = Ck.atom_off - Cj.atom_off;
+        const size_t cj = c - Ck.lcell_off + Cj.lcell_off;
+        L.loffs[c] = L.loffs[cj] + delta;
+        if (!e16) continue;
+        const int4* src = reinterpret_cast<const int4*>(L.ctab + cj * kSegStride);
+        int4* dst = reinterpret_cast<int4*>(L.ctab + c * kSegStride);
+#pragma unroll
+        for (int w = 0; w < kSegStride / 4; w++) {
+            int4 e = src[w];
+            e.x += delta; e.y += delta; e.z += delta; e.w += delta;
+            dst[w] = e;
+        }
     }
 }
 

@@ -33,50 +33,69 @@ struct SurrView {
 };
 
 // The pairs within rc of one slot, from its neighbour-list row (surrogate kernels): the LP lanes of the
-// slot read KB entries each without a count test (rows are padded past their count to kListSpan
-// entries), gather the neighbours' coordinates together, then test them; a row longer than KB * LP
-// entries continues in a loop.  The pairs within rc are appended in entry order to the slot's LDS row
-// by ballot prefix counts: with GEOM, pd = (dx, dy, dz, neighbour slot as float bits), otherwise r only
-// (rd).  Every lane of the slot returns the same count (which may exceed cap: the caller flags it).
-template <int LP, int KB, bool GEOM>
-__device__ __forceinline__ int stage_pairs(const ChainDev& C, const float4* __restrict__ xq, const int* __restrict__ nbs,
-                                           int n, float4 xi, int sub, float4* pd, int* /*unused*/, float* rd, int cap)
+// slot read KB entries each without a count test (entries past the count read the row's first entry,
+// which is a real entry or padding), gather the neighbours' coordinates together (32-bit entries: the
+// slot and image code; 16-bit (W16): slot = segment start from the slot's LDS segment row segs + rank),
+// then test them; a row longer than KB * LP entries continues in a loop.  The pairs within rc are
+// appended in entry order to the slot's LDS row by ballot prefix counts: with GEOM, pd = (dx, dy, dz,
+// neighbour slot as float bits), otherwise r only (rd).  Every lane of the slot returns the same count
+// (which may exceed cap: the caller flags it).
+template <int LP, int KB, bool GEOM, bool W16>
+__device__ __forceinline__ int stage_pairs(const ChainDev& C, const float4* __restrict__ xq, const char* nb, int nbcap,
+                                           int slot, const int* segs, int n, float4 xi, int sub, float4* pd, float* rd,
+                                           int cap)
 {
-    static_assert(KB * LP <= kListSpan, "the first batch reads padded entries only");
+    using E = std::conditional_t<W16, uint16_t, int>;
+    const E* __restrict__ nbs = reinterpret_cast<const E*>(nb) + (size_t)slot * nbcap;
     const int gbase = (threadIdx.x & 63) & ~(LP - 1);
     const float rc2 = C.rc2_f;
     int np = 0;
-    auto test = [&](int e, float4 q, bool live) {
-        const int code = e & 63;
-        const float dx = xi.x - (q.x + image_shift(code, 0, C.lsize_f[0]));
-        const float dy = xi.y - (q.y + image_shift(code, 1, C.lsize_f[1]));
-        const float dz = xi.z - (q.z + image_shift(code, 2, C.lsize_f[2]));
+    auto slot_of = [&](unsigned e) -> int {
+        if constexpr (W16) return entry_slot(segs, e);
+        else return (int)(e >> kImageBits);
+    };
+    auto test = [&](unsigned e, float4 q, bool live) {
+        float sx, sy, sz;
+        if constexpr (W16) {
+            const int k = (int)(e >> kRankBits);
+            sx = seg_shift(k, 0, C.lsize_f[0]); sy = seg_shift(k, 1, C.lsize_f[1]); sz = seg_shift(k, 2, C.lsize_f[2]);
+        } else {
+            const int code = (int)(e & 63u);
+            sx = image_shift(code, 0, C.lsize_f[0]); sy = image_shift(code, 1, C.lsize_f[1]);
+            sz = image_shift(code, 2, C.lsize_f[2]);
+        }
+        const float dx = xi.x - (q.x + sx);
+        const float dy = xi.y - (q.y + sy);
+        const float dz = xi.z - (q.z + sz);
         const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         const bool hit = live && r2 < rc2;
         const unsigned grp = (unsigned)((__ballot(hit) >> gbase) & ((1ull << LP) - 1ull));
         const int pos = np + __popc(grp & ((1u << sub) - 1u));
         if (hit && pos < cap) {
             if constexpr (GEOM) {
-                pd[pos] = make_float4(dx, dy, dz, __int_as_float((int)((unsigned)e >> kImageBits)));
+                pd[pos] = make_float4(dx, dy, dz, __int_as_float(slot_of(e)));
             } else {
                 rd[pos] = sqrtf(r2);
             }
         }
         np += __popc(grp);
     };
-    int ent[KB];
+    unsigned ent[KB];
     float4 q[KB];
 #pragma unroll
-    for (int u = 0; u < KB; u++) ent[u] = nbs[sub + u * LP];
+    for (int u = 0; u < KB; u++) {
+        const int k = sub + u * LP;
+        ent[u] = (unsigned)nbs[k < n ? k : 0];
+    }
 #pragma unroll
-    for (int u = 0; u < KB; u++) q[u] = xq[(unsigned)ent[u] >> kImageBits];
+    for (int u = 0; u < KB; u++) q[u] = xq[slot_of(ent[u])];
 #pragma unroll
     for (int u = 0; u < KB; u++) test(ent[u], q[u], sub + u * LP < n);
     for (int k0 = KB * LP; k0 < n; k0 += LP) {
         const int k = k0 + sub;
         const bool live = k < n;
-        const int e = nbs[live ? k : k0];
-        test(e, xq[(unsigned)e >> kImageBits], live);
+        const unsigned e = (unsigned)nbs[live ? k : k0];
+        test(e, xq[slot_of(e)], live);
     }
     return np;
 }

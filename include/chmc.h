@@ -276,6 +276,15 @@ int chmc_time_list(chmc_ctx* ctx, int reps, double* ms_sweep, double* ms_rebuild
  * each at the current positions and keeps the fastest) */
 int chmc_set_list_lanes(chmc_ctx* ctx, int lanes);
 int chmc_get_list_lanes(const chmc_ctx* ctx, int* lanes);
+/* list entry format: 32 (neighbour slot and image code per entry), 16 (segment of the list cell's
+ * 27-cell neighbourhood and rank inside that cell: half the bytes, a few more instructions per entry),
+ * or 0, the default: 16 when the lists' expected entries at 4 bytes exceed 256 MiB (the last-level
+ * cache), else 32.  Both give the same pairs in the same order; the FP64 energy is bitwise the same and
+ * the FP32 forces differ only where the sweep shape reads 16-bit entries two at a time.  get returns
+ * the format in use once the lists are set up.  No reference counterpart (a storage choice of the list
+ * MD above). */
+int chmc_set_list_entry_bits(chmc_ctx* ctx, int bits);
+int chmc_get_list_entry_bits(const chmc_ctx* ctx, int* bits);
 /* rebuild checks of list MD before every `every`-th force evaluation of an integration (1 to 4; default
  * 1), and before its last one.  In between, the rebuild trigger sits lower than skin / 2 by the
  * distance every - 1 drifts can add for momenta |p| <= 10 sqrt(m / beta), so the lists still hold every

@@ -228,9 +228,12 @@ def test_integrator_fp32_config2(chmc, orc):
     assert np.max(dx) < 2e-4, np.max(dx)
 
 
-@pytest.mark.parametrize("N,rho,Lb", [(32768, 0.8442, None), (131072, 0.8442, None), (262144, 0.8442, None),
-                                      (125, 0.58, None), (51, 0.8, None), (100, None, (8.0, 6.0, 4.5))])
-def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
+@pytest.mark.parametrize("N,rho,Lb,bits", [(32768, 0.8442, None, 32), (131072, 0.8442, None, 32),
+                                           (262144, 0.8442, None, 32), (125, 0.58, None, 32), (51, 0.8, None, 32),
+                                           (100, None, (8.0, 6.0, 4.5), 32), (32768, 0.8442, None, 16),
+                                           (131072, 0.8442, None, 16), (125, 0.58, None, 16), (51, 0.8, None, 16),
+                                           (100, None, (8.0, 6.0, 4.5), 16)])
+def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb, bits):
     """Neighbour-list MD (skin lists, rebuilt on displacement) against the 27-cell sweep with a
     re-binning per evaluation and against the FP64 oracle: 10 OMF2 steps from the same start.  Both FP32
     paths differ from the oracle only by FP32 rounding (same bars as the FP32 integrator test); the
@@ -238,9 +241,12 @@ def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
     (the anisotropic box holds 100 atoms: its 300-atom lattice of the force tests, at rho = 1.39 with
     0.64 sigma spacing, blows up under OMF2 dt = 0.01 in any arithmetic).  N = 131072 has 19^3 list
     cells, more than one build workgroup scans itself, so its rebuilds take the separate scan kernel;
-    N = 262144 has 22^3 list cells, more than one 8192-count round of that kernel."""
+    N = 262144 has 22^3 list cells, more than one 8192-count round of that kernel.  Both list entry
+    formats (bits): 32-bit slot entries and 16-bit segment-relative ones, whose small boxes name one
+    cell through several segments with different image shifts."""
     T = 0.722 if N >= 32768 else 1.0
     e, L = _eng(chmc, N, rho, Lb, T=T)
+    e.set_list_entry_bits(bits)
     ec, _ = _eng(chmc, N, rho, Lb, T=T, cell_md=True)
     e.heatbath(9, 1)
     ec.set_momenta(e.momenta())
@@ -253,6 +259,7 @@ def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
     s.integrate(6, 0.01, 10, 0.1932)
     entries, nint, rebuilds = e.list_stats()
     print(f"list entries/atom {entries / N:.1f}, interacting/atom {nint / N:.1f}, rebuilds so far {rebuilds}")
+    assert e.list_entry_bits() == bits
     for name, eng in (("list", e), ("cell", ec)):
         dx = np.abs(min_image(eng.positions() - s.x, L))
         print(f"{name}: rms {np.sqrt(np.mean(dx ** 2)):.2e} max {dx.max():.2e}")
@@ -278,16 +285,20 @@ def test_list_md_vs_cell_md_and_oracle(chmc, orc, N, rho, Lb):
     assert np.all(dev[odd] < 2 * fcut + 5e-5 * frms), dev[odd]
 
 
-def test_list_md_vs_cell_md_at_1m_atoms(chmc):
+@pytest.mark.parametrize("bits", [0, 32])
+def test_list_md_vs_cell_md_at_1m_atoms(chmc, bits):
     """The 1,048,576-atom workload (102^3 lattice sites, rho = 0.8442, T = 0.722) at full size, through
     properties that do not need the oracle: 3 OMF2 steps of list MD (with its rebuilds and the scan
     kernel of 42,875-cell chains) and of the 27-cell sweep from the same state stay within FP32
     rounding of each other, the interacting-pair counts of the two paths are equal, and the stored list
-    force matches the cell sweep's force at the final positions."""
+    force matches the cell sweep's force at the final positions.  The default entry format at this size
+    is the 16-bit one (382 MB of 32-bit entries); the 32-bit one is forced too."""
     N, T = 1048576, 0.722
     e, L = make(chmc, N, 0.8442, T=T)
     ec, _ = make(chmc, N, 0.8442, T=T, cell_md=True)
+    e.set_list_entry_bits(bits)
     e.set_skin(0.5)
+    assert e.list_entry_bits() == (bits or 16)
     e.heatbath(9, 1)
     ec.set_momenta(e.momenta())
     e.integrate(6, 0.01, 3, 0.1932)
@@ -311,13 +322,15 @@ def test_list_md_vs_cell_md_at_1m_atoms(chmc):
     assert np.all(dev[odd] < 2 * fcut + 5e-5 * frms), dev[odd]
 
 
-def test_list_sweep_lanes_agree(chmc):
+@pytest.mark.parametrize("bits", [32, 16])
+def test_list_sweep_lanes_agree(chmc, bits):
     """The list sweep at every shape, 4, 8 and 16 lanes per atom and 8 lanes with first batches of 12 and
     16 entries (64, 112, 112, 96 and 128 entries read without a count test from rows padded that far),
     gives the cell sweep's forces, also when the shape changes after the lists were built.  The shapes
     differ only in the FP32 summation order."""
     N, T = 4000, 0.722
     e, L = make(chmc, N, 0.8442, T=T)
+    e.set_list_entry_bits(bits)
     ec, _ = make(chmc, N, 0.8442, T=T, cell_md=True)
     e.heatbath(9, 1)
     e.integrate(6, 0.01, 10, 0.1932)                 # lists rebuilt along the way
@@ -341,13 +354,16 @@ def test_list_sweep_lanes_agree(chmc):
         forces[lanes] = fl
 
 
-def test_list_rows_repadded_for_a_longer_shape(chmc):
+@pytest.mark.parametrize("bits", [32, 16])
+def test_list_rows_repadded_for_a_longer_shape(chmc, bits):
     """Rows are padded only as far as the current shape reads them without a count test.  Lists rebuilt
     under the 4-lane shape (64 entries) keep stale entries of earlier, longer rows past 64; switching to
     a longer shape pads both list buffers again first, so every shape still gives the cell sweep's
-    forces after MD under the short one."""
+    forces after MD under the short one.  (A stale 16-bit entry would name a rank of a cell that may
+    have shrunk since: the padding keeps the sweep from reading any.)"""
     N, T = 4000, 0.722
     e, L = make(chmc, N, 0.8442, T=T)
+    e.set_list_entry_bits(bits)
     ec, _ = make(chmc, N, 0.8442, T=T, cell_md=True)
     e.set_skin(0.7)
     e.set_list_lanes(816)
@@ -370,6 +386,52 @@ def test_list_rows_repadded_for_a_longer_shape(chmc):
         odd = np.flatnonzero(dev >= 5e-5 * frms)
         print(f"lanes {lanes}: max deviation {dev.max():.2e}")
         assert len(odd) <= 2 and np.all(dev[odd] < 2 * fcut + 5e-5 * frms), (lanes, odd, dev[odd])
+
+
+@pytest.mark.parametrize("N,rho,skin", [(4000, 0.8442, 0.6), (125, 0.58, 0.6), (51, 0.8, 0.3)])
+def test_list_entry_formats_agree(chmc, N, rho, skin):
+    """The 16-bit and 32-bit list entry formats hold the same pairs in the same order.  At the same
+    positions: the same list statistics; the FP64 energy, virial and pair count bitwise (the energy
+    kernel adds the entries of each lane in list order, the image shifts dd * lsize are exact in both);
+    the 16-lane sweep's FP32 forces bitwise (7 entries per lane, each read alone in both formats); every
+    other shape within FP32 summation-order rounding (8-lane shapes read 16-bit entries two at a time)."""
+    T = 0.722
+    es = {}
+    for bits in (32, 16):
+        e, L = make(chmc, N, rho, T=T)
+        e.set_list_entry_bits(bits)
+        e.set_skin(skin)
+        e.heatbath(9, 1)
+        es[bits] = e
+    es[32].integrate(6, 0.01, 10, 0.1932)            # lists rebuilt along the way
+    x = es[32].positions()
+    for e in es.values():
+        e.set_positions(x)
+        assert e.list_entry_bits() in (16, 32)
+    assert es[16].list_entry_bits() == 16 and es[32].list_entry_bits() == 32
+    out = {}
+    for bits, e in es.items():
+        e.set_list_lanes(16)
+        e.integrate(6, 0.0, 1, 0.1932)               # dt = 0: the stored force at x
+        f, st = e.force(), e.list_stats()
+        # one accept-all trajectory of zero length: its record holds the list path's FP64 energy and
+        # virial at x
+        rec, _ = e.hmc(1, 2, chmc.LEAPFROG, 0.0, 1, seed=3)
+        out[bits] = (f, st, rec[0, 0])
+    f32, st32, r32 = out[32]
+    f16, st16, r16 = out[16]
+    print(f"N={N}: entries/atom {st32[0] / N:.1f}; U {r32['u']:.10f} / {r16['u']:.10f}")
+    assert st16[:2] == st32[:2]
+    assert r16["u"] == r32["u"] and r16["w"] == r32["w"] and r16["dh"] == r32["dh"]
+    assert np.array_equal(f16, f32)
+    frms = np.sqrt(np.mean(f32 ** 2))
+    for lanes in (8, 812, 816, 4):
+        for e in es.values():
+            e.set_list_lanes(lanes)
+        fa, fb = [(e.integrate(6, 0.0, 1, 0.1932), e.force())[1] for e in (es[32], es[16])]
+        assert np.max(np.abs(fa - fb)) < 2e-5 * frms, (lanes, np.max(np.abs(fa - fb)))
+    for e in es.values():
+        e.close()
 
 
 @pytest.mark.parametrize("N,rho,nb", [(512, 0.8, 200), (4000, 0.8442, 500)])
@@ -628,6 +690,8 @@ def test_error_paths(chmc):
         e.integrate(7, 0.01, 5)                                     # unknown scheme
     with pytest.raises(chmc.ChmcError):
         e.integrate(0, 0.01, 0)                                     # steps < 1 (reference loops forever)
+    with pytest.raises(chmc.ChmcError):
+        e.set_list_entry_bits(8)                                    # 0, 16 or 32
 
 
 @pytest.mark.parametrize("every", [1, 2, 3])

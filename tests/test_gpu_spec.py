@@ -19,13 +19,14 @@ pytestmark = pytest.mark.gpu
 RC = 2.5
 
 
-def _pair(chmc, N, rho, T, K, lanes=8, check=2, skin=0.6, start_rebuild=False):
+def _pair(chmc, N, rho, T, K, lanes=8, check=2, skin=0.6, start_rebuild=False, bits=0):
     L = (N / rho) ** (1 / 3)
     ch = chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T)
     seq = chmc.Engine(ch)
     spec = chmc.Engine(ch, speculate=K)
     assert spec.speculation() == K and spec.nchains == K
     for e in (seq, spec):
+        e.set_list_entry_bits(bits)
         e.set_skin(skin)
         e.set_list_lanes(lanes)
         e.set_list_check(check)
@@ -46,9 +47,12 @@ def _same_run(chmc, seq, spec, ntraj, therm, dt, steps, seed, use_graph):
     return r1[:, 0], n1[0]
 
 
-@pytest.mark.parametrize("use_graph,start_rebuild", [(True, False), (False, False), (True, True), (False, True)])
-def test_speculative_equals_sequential_mixed_acceptance(chmc, use_graph, start_rebuild):
-    seq, spec = _pair(chmc, 4000, 0.8442, 0.722, 4, start_rebuild=start_rebuild)
+@pytest.mark.parametrize("use_graph,start_rebuild,bits", [(True, False, 0), (False, False, 0), (True, True, 0),
+                                                          (False, True, 0), (True, True, 16), (False, False, 16)])
+def test_speculative_equals_sequential_mixed_acceptance(chmc, use_graph, start_rebuild, bits):
+    """bits 16: the 16-bit list entries, whose broadcast copies the rows as they are and moves the
+    segment starts of every list cell instead."""
+    seq, spec = _pair(chmc, 4000, 0.8442, 0.722, 4, start_rebuild=start_rebuild, bits=bits)
     seq.init_lattice(3, 0.05)
     x0 = seq.positions()
     spec.set_positions(x0)                 # chain 0 only: the run copies it into the other copies

@@ -224,16 +224,18 @@ def test_fallback_switches_to_lj(chmc, trained):
     assert np.all(r["surrogate"] == 1)
 
 
-@pytest.mark.parametrize("fallback_acc", [0.0, 1.01])
-def test_speculative_surrogate_equals_sequential(chmc, trained, fallback_acc):
+@pytest.mark.parametrize("fallback_acc,bits", [(0.0, 0), (1.01, 0), (0.0, 16)])
+def test_speculative_surrogate_equals_sequential(chmc, trained, fallback_acc, bits):
     """Surrogate-driven trajectories run as 4 speculative copies of one chain give the sequential chain's
     records, positions and momenta bit for bit, also across the fallback to LJ forces after the first
-    10-trajectory window (fallback_acc 1.01: every window's acceptance is below it)."""
+    10-trajectory window (fallback_acc 1.01: every window's acceptance is below it), and with 16-bit
+    list entries (bits 16) under the surrogate's descriptor and force kernels."""
     t, _ = trained
     ch = chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T)
     seq, spec = chmc.Engine(ch), chmc.Engine(ch, speculate=4)
     x0, p = t.positions(), t.surrogate_params()
     for e in (seq, spec):
+        e.set_list_entry_bits(bits)
         e.set_list_lanes(8)
         e.surrogate_set_params(p)
         e.set_positions(x0)
