@@ -55,10 +55,13 @@ SWEEP_TRAFFIC_PROFILED = {(0.6, 8, 1): (FETCH_SCALE * 9976.0 + 2048.0) * 1024.0,
                           (0.7, 8, 1): (FETCH_SCALE * 9970.0 + 2048.0) * 1024.0,
                           (0.7, 816, 1): (FETCH_SCALE * 10086.8 + 2048.0) * 1024.0,
                           (0.4, 812, 8): (FETCH_SCALE * 60498.2 + 16384.2) * 1024.0}
-# the same at N=1,048,576 (workload c5), skin 0.5 with 8 lanes and skin 0.7 with the 816 shape
-# (profiles/lj_list_sweep_N1048576.md)
-SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8): (FETCH_SCALE * 322982.0 + 65536.0) * 1024.0,
-                             (0.7, 816): (FETCH_SCALE * 323121.7 + 65536.3) * 1024.0}
+# the same at N=1,048,576 (workload c5), keyed by (skin, lanes, list entry bits): 32-bit entries at skin
+# 0.5 with 8 lanes and skin 0.7 with the 816 shape, 16-bit entries (the default at this size) at skin 0.6
+# with 8 lanes and skin 0.7 with the 816 shape (profiles/lj_list_sweep_N1048576.md)
+SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8, 32): (FETCH_SCALE * 322982.0 + 65536.0) * 1024.0,
+                             (0.7, 816, 32): (FETCH_SCALE * 323121.7 + 65536.3) * 1024.0,
+                             (0.6, 8, 16): (FETCH_SCALE * 229220.5 + 65536.2) * 1024.0,
+                             (0.7, 816, 16): (FETCH_SCALE * 229060.2 + 65536.0) * 1024.0}
 # the same for the 256-chain batch (workload c3) at the per-chain skin families base 0.4, exponent 1.5
 # and base 0.45, exponent 1.0, 4 lanes per atom (profiles/lj_list_sweep_c3_batch.md), keyed by the skin
 # family the run uses
@@ -133,14 +136,15 @@ def other_workload(args):
 
     w = configs.WORKLOADS[args.workload]()
     e = chmc.Engine(w.chains)
+    e.set_list_entry_bits(args.entry_bits)
     e.init_lattice(1, 0.05)
     e.set_list_check(args.list_check)
     nat = w.natoms
     steps = args.steps if args.steps_given else (200 if args.workload == "c3" else 50)
     # both workloads relax from their lattice starts over the first ~100 trajectories (more rebuilds
-    # early): 150 accept-all trajectories whatever --warmup says, then --warmup ordinary ones; the tuners
-    # and the timed run start after that
-    therm = 150
+    # early): 150 accept-all trajectories (--therm for counter runs) whatever --warmup says, then --warmup
+    # ordinary ones; the tuners and the timed run start after that
+    therm = args.therm if args.therm_given else 150
     warm = args.warmup if args.warmup_given else 0
     t0 = time.perf_counter()
     e.hmc(therm, therm, w.scheme, w.dt, w.steps, lam=w.lam, seed=10, use_graph=not args.no_graph)
@@ -182,9 +186,15 @@ def other_workload(args):
         ent += a
         nint += b
     ms_sweep, ms_rebuild = e.time_list(reps=20)
+    bits = e.list_entry_bits()
     flops = 8.0 * ent + 13.0 * nint
-    nbytes = 4.0 * ent + SWEEP_BYTES_PER_ATOM * nat
+    nbytes = bits / 8.0 * ent + SWEEP_BYTES_PER_ATOM * nat
     achieved = nbytes / (ms_sweep * 1e-3) / 1e9
+    # the FP32 work of the 27-cell sweep the lists replace (candidates and interacting pairs of the cell
+    # path, counted on the device at the final positions)
+    ncell = sum(e.pair_counts(k)[0] for k in range(len(w.chains)))
+    flops_cell = 8.0 * ncell + 13.0 * nint
+    tflops_cell = flops_cell / (ms_sweep * 1e-3) / 1e12
     u = rec["u"] / np.array([c.N for c in w.chains])[None, :]
     # chains whose U/N never changed in the timed run carry no tau_int and contribute no ESS
     taus = [stats.tau_int(u[:, k])[0] for k in range(len(w.chains))]
@@ -197,18 +207,24 @@ def other_workload(args):
         "data": "synthetic (seeded lattice + jitter, Philox momenta)",
         "config": {"workload": f"{args.workload}: {len(w.chains)} chain(s), {nat} atoms, "
                                f"{'LEAP' if w.scheme == chmc.LEAPFROG else 'OMF2'} {w.steps}x{w.dt}",
-                   "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "list_check": e.list_check(), "skin": e.skin(),
+                   "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "list_check": e.list_check(),
+                   "list_entry_bits": bits, "skin": e.skin(),
                    "chain_skin_range": [float(chain_skins.min()), float(chain_skins.max())], "skin_family": family,
                    "ms_per_traj_by_skin": skins, "warmup_seconds": t_warm},
         "acceptance": float(nacc.sum()) / (steps * len(w.chains)),
         "ess_per_s_sum_over_chains": ess / el if moving else None, "chains_with_tau_int": len(moving),
         "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": (SWEEP_TRAFFIC_PROFILED_C5.get((round(float(e.skin()), 2), int(e.list_lanes())))
+                     "traffic": (SWEEP_TRAFFIC_PROFILED_C5.get((round(float(e.skin()), 2), int(e.list_lanes()), bits))
                                  if args.workload == "c5" else
                                  SWEEP_TRAFFIC_PROFILED_C3.get((family, int(e.list_lanes())))
                                  if args.workload == "c3" else None),
-                     "bytes_per_launch": nbytes, "flops_per_launch": flops, "tflops": flops / (ms_sweep * 1e-3) / 1e12,
+                     "bytes_per_launch": nbytes, "bytes": f"{bits // 8} B per list entry + {SWEEP_BYTES_PER_ATOM:.0f} B per atom",
+                     "flops_per_launch": flops, "tflops": flops / (ms_sweep * 1e-3) / 1e12,
+                     "fp32_27cell": {"work": "8 * N_cand + 13 * N_int per atom-evaluation, N_cand of the 27-cell stencil",
+                                     "flops_per_launch": flops_cell, "tflops": tflops_cell,
+                                     "frac": tflops_cell / PEAK_FP32_TFLOPS,
+                                     "cell_candidates_per_atom": ncell / nat},
                      "ms_per_launch": ms_sweep, "ms_rebuild": ms_rebuild,
                      "list_entries_per_atom": ent / nat, "n_int_per_atom": nint / nat},
     }
@@ -482,6 +498,8 @@ def main():
                     help="fixed per-chain skins BASE/EXP of the density-scaled family (c3, c5; no skin autotuning)")
     ap.add_argument("--lanes", type=int, default=0, choices=[0, 4, 8, 16, 812, 816],
                     help="fixed list-sweep lanes per atom (0: autotune)")
+    ap.add_argument("--entry-bits", type=int, default=0, choices=[0, 16, 32],
+                    help="list entry format (chmc_set_list_entry_bits): 0 chooses by the lists' size")
     ap.add_argument("--list-check", type=int, default=2, choices=[1, 2, 3],
                     help="list MD: rebuild checks before every LIST_CHECK-th force evaluation (chmc_set_list_check); "
                          "tuned with the skin (1 or 2) unless given; 2 where the skin is fixed")
@@ -492,7 +510,8 @@ def main():
                     help="c2, c4: rebuild the lists at the start of the trajectory after an accepted one "
                          "(chmc_set_list_start_rebuild; pays at low acceptance)")
     ap.add_argument("--therm", type=int, default=THERM,
-                    help="c2, kernels: accept-all burn-in trajectories off the lattice (shorter for counter runs)")
+                    help="accept-all burn-in trajectories off the lattice (c2, kernels; c3 and c5 default to 150; "
+                         "shorter for counter runs)")
     ap.add_argument("--no-ess", action="store_true",
                     help="c2: ESS from the timed run itself, no separate long run (counter and trace runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -513,6 +532,7 @@ def main():
     args.steps_given = "--steps" in sys.argv
     args.warmup_given = "--warmup" in sys.argv
     args.list_check_given = "--list-check" in sys.argv
+    args.therm_given = "--therm" in sys.argv
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
@@ -540,6 +560,8 @@ def main():
     L = (N / RHO) ** (1 / 3)
     copies = 1 if args.cell_md else args.spec
     e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T), cell_md=args.cell_md, speculate=copies)
+    if not args.cell_md:
+        e.set_list_entry_bits(args.entry_bits)
     e.init_lattice(1, 0.05)
     e.set_list_check(args.list_check)
     e.set_list_start_rebuild(bool(args.start_rebuild))
@@ -601,16 +623,18 @@ def main():
     # dominant kernel: the neighbour-list FP32 sweep (force + kick + drift), against the FP32 VALU peak with
     # the algorithmic work of the 27-cell sweep it replaces (8 flops per candidate pair of the 27-cell
     # stencil + 13 per interacting pair, per atom-evaluation; candidates and interacting pairs counted on
-    # the device).  Its HBM side: 4 B per list entry + SWEEP_BYTES_PER_ATOM per atom algorithmic, and the
-    # rocprofv3-measured traffic when the tuned skin and lanes are the profiled ones.
+    # the device).  Its HBM side: the list entry bytes (4 B, or 2 in the 16-bit format) + SWEEP_BYTES_PER_ATOM
+    # per atom algorithmic, and the rocprofv3-measured traffic when the tuned skin and lanes are the profiled
+    # ones (32-bit entries).
     # (a launch sweeps every speculative copy: copies x the one chain's work)
     ms_sweep, ms_rebuild = e.time_list(reps=50)
     ncand_cell, nint_cell = e.pair_counts(0)
     flops_alg = copies * (8.0 * ncand_cell + 13.0 * nint_cell)
     flops_exec = copies * (8.0 * entries + 13.0 * nint)
-    nbytes = copies * (4.0 * entries + SWEEP_BYTES_PER_ATOM * N)
+    ebytes = e.list_entry_bits() / 8.0
+    nbytes = copies * (ebytes * entries + SWEEP_BYTES_PER_ATOM * N)
     tflops = flops_alg / (ms_sweep * 1e-3) / 1e12
-    traffic = SWEEP_TRAFFIC_PROFILED.get((round(float(skin), 2), int(lanes), copies))
+    traffic = SWEEP_TRAFFIC_PROFILED.get((round(float(skin), 2), int(lanes), copies)) if ebytes == 4.0 else None
 
     out = {
         "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N",
@@ -621,7 +645,7 @@ def main():
                    "N": N, "trajectories": args.steps, "md_steps_per_traj": NSTEPS, "force_evals_per_traj": 2 * NSTEPS + 1,
                    "thermalisation": args.therm, "md_path": "neighbour lists", "speculative_copies": copies,
                    "skin": skin, "list_lanes": lanes, "list_check": e.list_check(),
-                   "start_rebuild": e.list_start_rebuild(),
+                   "list_entry_bits": e.list_entry_bits(), "start_rebuild": e.list_start_rebuild(),
                    "ms_per_traj_by_skin": skins},
         "acceptance": float(nacc[0]) / args.steps,
         "ess_per_s": ess_info["ess_per_s"], "ess": ess_info,
