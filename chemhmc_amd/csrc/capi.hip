@@ -769,10 +769,16 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
     v.nbcap = nbcap;
     v.lscan_in_build = 1;
     v.lcap_max = 1;
+    int max_cells = 0;
     for (const ChainDev& cd : c->hchains) {
         if (cd.lncells > kScanCells) v.lscan_in_build = 0;
         v.lcap_max = std::max(v.lcap_max, cd.lcap);
+        max_cells = std::max(max_cells, cd.lncells);
     }
+    // a batch of large chains (speculative copies): each build workgroup scanning its chain's counts
+    // costs more than a scan launch of its own (one chain keeps the in-build scan: its idle rebuild
+    // checks would pay the extra launch)
+    v.lscan_chains = v.lscan_in_build && c->nchains > 1 && max_cells >= kScanChainCells ? 1 : 0;
     int* lcell_chain = nullptr;
     Chunk* tiles = nullptr;
     HIPCHK(c, lalloc(c, &lcell_chain, (size_t)lcells));

@@ -126,6 +126,7 @@ __host__ __device__ constexpr int sweep_span(int shape)
 }
 constexpr float kFarShift = 1.0e18f;
 constexpr int kScanCells = 3072;   // list cells of a chain whose counts one build workgroup scans in LDS
+constexpr int kScanChainCells = 512;   // batches of chains this large scan per chain in a kernel of its own
 
 // ---- launch wrappers (each defined in the TU that owns the kernel) ----------------------
 struct DevView {
@@ -157,6 +158,7 @@ struct DevView {
     double *lmemx, *lmemp; // [3][dense]
     double *xl, *pl, *xl_saved;
     int ldense;           // member slots of all chains (sum of lncells * lcap)
+    int lscan_chains;     // a batch of chains of kScanChainCells list cells or more: a scan workgroup per chain
     int lscan_in_build;   // every chain has at most kScanCells list cells: the build scans the counts
     int lcap_max;         // largest list-cell member capacity over the chains (at most 256)
     int* lbuf;            // [3][nchains]: current list buffer, buffer of the trajectory's start, buffer whose

@@ -12,6 +12,7 @@
 //   k_lcount  atoms -> list cells (floor(x / lsize), clamped), arrival order into a dense member array
 //             that carries each member's FP32 cell-relative position
 //   k_lscan   only for chains of more than kScanCells cells: one workgroup scans the cell counts
+//             (k_lscan_chains: batches of chains of kScanChainCells cells or more, one workgroup per chain)
 //   k_lbuild  one workgroup per list cell: (small chains: the counts scanned in LDS) the cell's members
 //             ranked by atom id give the compact slots and their slot data (x0, r0, FP32 xsl); the 27
 //             neighbour cells' members, ranked the same way, are staged in LDS in slot order with the
@@ -201,6 +202,40 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q, int cb)
             loffs = list_buf(v, *list_start(v, k) ^ 1).loffs;
         }
         if (rq) loffs[c] = base + sc;
+    }
+}
+
+// batches of large chains (speculative copies): one 1024-thread workgroup per rebuilding chain scans its
+// counts (each thread a contiguous run) into the list offsets of the buffer being rebuilt, so that the
+// build workgroups of a chain read 27 offsets instead of each scanning the chain's counts in LDS
+__global__ __launch_bounds__(1024) void k_lscan_chains(DevView v, int q, int cb)
+{
+    const int k = blockIdx.x;
+    if (*req_any(v, q) == 0 || *req(v, q, k) == 0) return;
+    const ChainDev& C = v.chains[k];
+    constexpr int T = 1024;
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int n = C.lncells;
+    const int* __restrict__ cnt = v.lcount + (size_t)cb * v.Cl + C.lcell_off;
+    int* __restrict__ loffs = list_buf(v, *list_start(v, k) ^ 1).loffs + C.lcell_off;
+    const int per = (n + T - 1) / T;
+    const int beg = min(tid * per, n), end = min(beg + per, n);
+    int s = 0;
+    for (int i = beg; i < end; i++) s += cnt[i];
+    int incl = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int run = C.atom_off + incl - s;
+    for (int w = 0; w < wid; w++) run += wsum[w];
+    for (int i = beg; i < end; i++) {
+        loffs[i] = run;
+        run += cnt[i];
     }
 }
 
@@ -549,6 +584,13 @@ hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t dyn = (size_t)v.lcap_max * (sizeof(float4) + 2 * sizeof(int));
+    if (v.lscan_chains) {
+        hipLaunchKernelGGL(k_lscan_chains, dim3(v.nchains), dim3(1024), 0, s, v, q, cb);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (v.entry16) hipLaunchKernelGGL((k_lbuild<false, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+        else hipLaunchKernelGGL((k_lbuild<false, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+        return hipGetLastError();
+    }
     if (v.lscan_in_build) {
         if (v.entry16) hipLaunchKernelGGL((k_lbuild<true, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
         else hipLaunchKernelGGL((k_lbuild<true, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
