@@ -787,6 +787,7 @@ static int list_setup(chmc_ctx* c, double skin, const std::vector<double>& skins
     // the per-slot list state twice (current / trajectory-start buffers)
     HIPCHK(c, lalloc(c, &v.loffs, 2 * ((size_t)lcells + 1)));
     HIPCHK(c, lalloc(c, &v.lmem, (size_t)dense));
+    HIPCHK(c, lalloc(c, &v.lrank, (size_t)dense));
     HIPCHK(c, lalloc(c, &v.lmemx, 3 * (size_t)dense));
     HIPCHK(c, lalloc(c, &v.lmemp, 3 * (size_t)dense));
     HIPCHK(c, lalloc(c, &v.xl, 3 * N));

@@ -151,6 +151,7 @@ struct DevView {
     const int* lcell_chain;
     const Chunk* tiles;
     int *lcount, *loffs, *lperm, *linv, *nbn, *lflag, *lreq, *lscan_tmp, *lcs, *ctab;
+    int* lrank;           // [dense] rank of each counted member in its list cell (k_lscan_chains)
     char* nb;             // list rows (entry bytes: 2 with entry16, else 4)
     size_t nb_stride;     // bytes of one buffer's rows: Ntot * nbcap * entry bytes
     int entry16;          // 16-bit segment-relative list entries (else 32-bit slot entries)

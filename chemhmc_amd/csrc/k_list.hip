@@ -205,13 +205,45 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q, int cb)
     }
 }
 
-// batches of large chains (speculative copies): one 1024-thread workgroup per rebuilding chain scans its
-// counts (each thread a contiguous run) into the list offsets of the buffer being rebuilt, so that the
-// build workgroups of a chain read 27 offsets instead of each scanning the chain's counts in LDS
+// batches of large chains (speculative copies): workgroups 0 .. nchains - 1, one per rebuilding chain,
+// scan its counts (each thread a contiguous run) into the list offsets of the buffer being rebuilt, so
+// that the build workgroups of a chain read 27 offsets instead of each scanning the chain's counts in
+// LDS; the workgroups after them rank the members of every list cell of a rebuilding chain by atom id
+// (one wave per cell), so that the build workgroups do not rank each neighbour cell again
+__device__ __forceinline__ void rank_cell_members(const DevView& v, int q, int cb, int cell, int lane, int* ids)
+{
+    const int k = v.lcell_chain[cell];
+    if (*req(v, q, k) == 0) return;
+    const ChainDev& C = v.chains[k];
+    const int nk = min(v.lcount[(size_t)cb * v.Cl + cell], C.lcap);
+    const size_t base = C.ldense_off + (size_t)(cell - C.lcell_off) * C.lcap;
+    for (int i = lane; i < nk; i += 64) ids[i] = __float_as_int(v.lmem[base + i].w);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int i = lane; i < nk; i += 64) {
+        const int a = ids[i];
+        int r = 0;
+#pragma unroll 8
+        for (int j = 0; j < nk; j++) r += ids[j] < a ? 1 : 0;
+        v.lrank[base + i] = r;
+    }
+}
+
+constexpr int kRankWaves = 16;     // cells ranked per 1024-thread workgroup of k_lscan_chains
+
 __global__ __launch_bounds__(1024) void k_lscan_chains(DevView v, int q, int cb)
 {
+    __shared__ int rank_ids[kRankWaves][256];
+    if (*req_any(v, q) == 0) return;
+    if ((int)blockIdx.x >= v.nchains) {
+        const int wid = threadIdx.x >> 6;
+        const int cell = ((int)blockIdx.x - v.nchains) * kRankWaves + wid;
+        if (cell < v.Cl) rank_cell_members(v, q, cb, cell, threadIdx.x & 63, rank_ids[wid]);
+        return;
+    }
     const int k = blockIdx.x;
-    if (*req_any(v, q) == 0 || *req(v, q, k) == 0) return;
+    if (*req(v, q, k) == 0) return;
     const ChainDev& C = v.chains[k];
     constexpr int T = 1024;
     __shared__ int wsum[16];
@@ -334,7 +366,7 @@ __device__ __forceinline__ void build_pass(const float4* sj, const float4* si, i
 //            is fixed by the slot order and not by the arrival order of the count.
 // A neighbourhood of more than kLMaxJ members is staged in passes of whole cells; the own cell is
 // loaded with the first pass.
-template <bool SCAN, bool W16>
+template <bool SCAN, bool W16, bool RANKED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_lbuild(DevView v, int q, int cb)
 {
     const int c = xcd_block(blockIdx.x, v.Cl);
@@ -375,9 +407,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     // below lcap): issued first, their latency overlaps the offsets
     float4 meo = make_float4(0.f, 0.f, 0.f, 0.f);
     double xo[3] = { 0.0, 0.0, 0.0 }, po[3] = { 0.0, 0.0, 0.0 };
+    int orank = 0;                                   // RANKED: the member's rank by atom id (k_lscan_chains)
     if (tid < C.lcap) {
         const size_t mi = (size_t)lc * C.lcap + tid;
         meo = mem[mi];
+        if constexpr (RANKED) orank = v.lrank[C.ldense_off + mi];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             xo[d] = memx[d * xstride + mi];
@@ -437,11 +471,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         // members of this pass, loaded once (PT per thread); with the first pass the loads are in
         // flight during the scan of the counts, the ids go to LDS for the ranking after it
         float4 me[PT];
-        int kk[PT];
+        int kk[PT], rk[PT];
 #pragma unroll
         for (int u = 0; u < PT; u++) {
             const int t = tid + NT * u;
             kk[u] = k0;
+            rk[u] = 0;
             me[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (t < jn) {
                 int k = k0;
@@ -449,7 +484,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 for (int step = 16; step > 0; step >>= 1)
                     if (k + step < k1 && seg_pref[k + step] <= jlo + t) k += step;
                 kk[u] = k;
-                me[u] = mem[(size_t)seg_cell[k] * C.lcap + (jlo + t - seg_pref[k])];
+                const size_t mi = (size_t)seg_cell[k] * C.lcap + (jlo + t - seg_pref[k]);
+                me[u] = mem[mi];
+                if constexpr (RANKED) rk[u] = v.lrank[C.ldense_off + mi];
             }
         }
         if constexpr (SCAN) {
@@ -458,10 +495,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 if (tid < 27) seg_beg[tid] = C.atom_off + scan[seg_cell[tid]];
             }
         }
-        if (first && tid < ni) sido[tid] = __float_as_int(meo.w);
+        if constexpr (!RANKED) {
+            if (first && tid < ni) sido[tid] = __float_as_int(meo.w);
 #pragma unroll
-        for (int u = 0; u < PT; u++)
-            if (tid + NT * u < jn) sid[tid + NT * u] = __float_as_int(me[u].w);
+            for (int u = 0; u < PT; u++)
+                if (tid + NT * u < jn) sid[tid + NT * u] = __float_as_int(me[u].w);
+        }
         __syncthreads();                       // seg_beg, sid, sido; the scan area is free again
         if (first) ibeg = seg_beg[13];
         // 16-bit entries: the cell's segment starts (the padding segments: its own first slot)
@@ -469,9 +508,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         if (first && tid < ni) {
             // own slot data: slots by atom id; the rebuild reference (FP32 cell-relative position, cell)
             const int a = __float_as_int(meo.w);
-            int r = 0;
+            int r = orank;
+            if constexpr (!RANKED) {
 #pragma unroll 8
-            for (int k = 0; k < ni; k++) r += sido[k] < a ? 1 : 0;
+                for (int k = 0; k < ni; k++) r += sido[k] < a ? 1 : 0;
+            }
             const int slot = ibeg + r;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -494,9 +535,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 const int k = kk[u];
                 const int s0 = seg_pref[k] - jlo, nk = seg_cnt[k];
                 const int a = __float_as_int(me[u].w);
-                int r = 0;
+                int r = rk[u];
+                if constexpr (!RANKED) {
 #pragma unroll 8
-                for (int w = 0; w < nk; w++) r += sid[s0 + w] < a ? 1 : 0;
+                    for (int w = 0; w < nk; w++) r += sid[s0 + w] < a ? 1 : 0;
+                }
                 // w holds the finished list entry: segment k << kRankBits | rank r, or j slot <<
                 // kImageBits | image code
                 const int e = e16 ? ((k << kRankBits) | r) : (((seg_beg[k] + r) << kImageBits) | seg_code[k]);
@@ -585,21 +628,22 @@ hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s)
     if (e != hipSuccess) return e;
     const size_t dyn = (size_t)v.lcap_max * (sizeof(float4) + 2 * sizeof(int));
     if (v.lscan_chains) {
-        hipLaunchKernelGGL(k_lscan_chains, dim3(v.nchains), dim3(1024), 0, s, v, q, cb);
+        const int grid = v.nchains + (v.Cl + kRankWaves - 1) / kRankWaves;
+        hipLaunchKernelGGL(k_lscan_chains, dim3(grid), dim3(1024), 0, s, v, q, cb);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if (v.entry16) hipLaunchKernelGGL((k_lbuild<false, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
-        else hipLaunchKernelGGL((k_lbuild<false, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+        if (v.entry16) hipLaunchKernelGGL((k_lbuild<false, true, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+        else hipLaunchKernelGGL((k_lbuild<false, false, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
         return hipGetLastError();
     }
     if (v.lscan_in_build) {
-        if (v.entry16) hipLaunchKernelGGL((k_lbuild<true, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
-        else hipLaunchKernelGGL((k_lbuild<true, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+        if (v.entry16) hipLaunchKernelGGL((k_lbuild<true, true, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+        else hipLaunchKernelGGL((k_lbuild<true, false, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_lscan, dim3(1), dim3(1024), 0, s, v, q, cb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (v.entry16) hipLaunchKernelGGL((k_lbuild<false, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
-    else hipLaunchKernelGGL((k_lbuild<false, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+    if (v.entry16) hipLaunchKernelGGL((k_lbuild<false, true, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+    else hipLaunchKernelGGL((k_lbuild<false, false, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
     return hipGetLastError();
 }
 
