@@ -11,8 +11,10 @@
 // so a chain of a batch evolves exactly as it does alone):
 //   k_lcount  atoms -> list cells (floor(x / lsize), clamped), arrival order into a dense member array
 //             that carries each member's FP32 cell-relative position
-//   k_lscan   only for chains of more than kScanCells cells: one workgroup scans the cell counts
-//             (k_lscan_chains: batches of chains of kScanChainCells cells or more, one workgroup per chain)
+//   k_lscan   only for chains of more than kScanCells cells: one workgroup scans the cell counts and
+//             the others rank each list cell's members by atom id once, for every build workgroup
+//             that stages them (k_lscan_chains: the same for batches of chains of kScanChainCells cells
+//             or more, one scan workgroup per chain)
 //   k_lbuild  one workgroup per list cell: (small chains: the counts scanned in LDS) the cell's members
 //             ranked by atom id give the compact slots and their slot data (x0, r0, FP32 xsl); the 27
 //             neighbour cells' members, ranked the same way, are staged in LDS in slot order with the
@@ -122,14 +124,45 @@ __global__ __launch_bounds__(256) void k_lcount(DevView v, int q, int cb)
     }
 }
 
-// chains with more list cells than kScanCells: one workgroup scans all counts (zero in chains that did
+// the members of one list cell of a rebuilding chain ranked by atom id (one wave; ids staged in LDS)
+__device__ __forceinline__ void rank_cell_members(const DevView& v, int q, int cb, int cell, int lane, int* ids)
+{
+    const int k = v.lcell_chain[cell];
+    if (*req(v, q, k) == 0) return;
+    const ChainDev& C = v.chains[k];
+    const int nk = min(v.lcount[(size_t)cb * v.Cl + cell], C.lcap);
+    const size_t base = C.ldense_off + (size_t)(cell - C.lcell_off) * C.lcap;
+    for (int i = lane; i < nk; i += 64) ids[i] = __float_as_int(v.lmem[base + i].w);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int i = lane; i < nk; i += 64) {
+        const int a = ids[i];
+        int r = 0;
+#pragma unroll 8
+        for (int j = 0; j < nk; j++) r += ids[j] < a ? 1 : 0;
+        v.lrank[base + i] = r;
+    }
+}
+
+constexpr int kRankWaves = 16;     // cells ranked per 1024-thread workgroup of k_lscan and k_lscan_chains
+
+// chains with more list cells than kScanCells: workgroup 0 scans all counts (zero in chains that did
 // not count) and each rebuilding chain's cells get atom_off + the scan relative to the chain's first
-// cell, in the list offsets of the buffer being rebuilt.  The workgroup runs on one CU, so every global
-// access is coalesced: rounds of 8192 counts go through LDS, each thread scans 8 contiguous ones, a
-// block scan over the threads and a running carry join the rounds.
+// cell, in the list offsets of the buffer being rebuilt.  It runs on one CU, so every global access is
+// coalesced: rounds of 8192 counts go through LDS, each thread scans 8 contiguous ones, a block scan
+// over the threads and a running carry join the rounds.  The workgroups after it rank the members of
+// every list cell of a rebuilding chain for the build.
 __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q, int cb)
 {
     if (*req_any(v, q) == 0) return;
+    if (blockIdx.x > 0) {                  // workgroups after the first rank the members of kRankWaves cells
+        __shared__ int rank_ids[kRankWaves][256];
+        const int wid = threadIdx.x >> 6;
+        const int cell = ((int)blockIdx.x - 1) * kRankWaves + wid;
+        if (cell < v.Cl) rank_cell_members(v, q, cb, cell, threadIdx.x & 63, rank_ids[wid]);
+        return;
+    }
     constexpr int T = 1024, PT = 8;
     __shared__ int buf[T * PT];
     __shared__ int wsum[16];
@@ -210,28 +243,6 @@ __global__ __launch_bounds__(1024) void k_lscan(DevView v, int q, int cb)
 // that the build workgroups of a chain read 27 offsets instead of each scanning the chain's counts in
 // LDS; the workgroups after them rank the members of every list cell of a rebuilding chain by atom id
 // (one wave per cell), so that the build workgroups do not rank each neighbour cell again
-__device__ __forceinline__ void rank_cell_members(const DevView& v, int q, int cb, int cell, int lane, int* ids)
-{
-    const int k = v.lcell_chain[cell];
-    if (*req(v, q, k) == 0) return;
-    const ChainDev& C = v.chains[k];
-    const int nk = min(v.lcount[(size_t)cb * v.Cl + cell], C.lcap);
-    const size_t base = C.ldense_off + (size_t)(cell - C.lcell_off) * C.lcap;
-    for (int i = lane; i < nk; i += 64) ids[i] = __float_as_int(v.lmem[base + i].w);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int i = lane; i < nk; i += 64) {
-        const int a = ids[i];
-        int r = 0;
-#pragma unroll 8
-        for (int j = 0; j < nk; j++) r += ids[j] < a ? 1 : 0;
-        v.lrank[base + i] = r;
-    }
-}
-
-constexpr int kRankWaves = 16;     // cells ranked per 1024-thread workgroup of k_lscan_chains
-
 __global__ __launch_bounds__(1024) void k_lscan_chains(DevView v, int q, int cb)
 {
     __shared__ int rank_ids[kRankWaves][256];
@@ -640,10 +651,10 @@ hipError_t launch_list_rebuild(const DevView& v, int q, int cb, hipStream_t s)
         else hipLaunchKernelGGL((k_lbuild<true, false, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_lscan, dim3(1), dim3(1024), 0, s, v, q, cb);
+    hipLaunchKernelGGL(k_lscan, dim3(1 + (v.Cl + kRankWaves - 1) / kRankWaves), dim3(1024), 0, s, v, q, cb);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (v.entry16) hipLaunchKernelGGL((k_lbuild<false, true, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
-    else hipLaunchKernelGGL((k_lbuild<false, false, false>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+    if (v.entry16) hipLaunchKernelGGL((k_lbuild<false, true, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
+    else hipLaunchKernelGGL((k_lbuild<false, false, true>), dim3(v.Cl), dim3(256), dyn, s, v, q, cb);
     return hipGetLastError();
 }
 
