@@ -165,7 +165,7 @@ def other_workload(args):
     elif args.skin is None:
         def run(n, sd):
             e.hmc(n, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=sd, use_graph=not args.no_graph)
-        checks = None if args.list_check_given else (1, 2)
+        checks = None if args.list_check_given else (1, 2, 3)
         if len(w.chains) > 1:
             _, skins = tune.tune_skins(e, run, lanes=args.lanes, checks=checks)
         else:
@@ -492,7 +492,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000, help="timed HMC trajectories")
     ap.add_argument("--warmup", type=int, default=100, help="untimed trajectories (also thermalisation)")
-    ap.add_argument("--tune", type=int, default=40, help="trajectories per skin in the skin autotuner")
+    ap.add_argument("--tune", type=int, default=200,
+                    help="trajectories per skin in the skin autotuner (long enough that the graph capture of "
+                         "each timed call is a small part of it)")
     ap.add_argument("--skin", type=float, default=None, help="fixed list skin (no skin autotuning)")
     ap.add_argument("--skin-family", default=None,
                     help="fixed per-chain skins BASE/EXP of the density-scaled family (c3, c5; no skin autotuning)")
@@ -500,9 +502,9 @@ def main():
                     help="fixed list-sweep lanes per atom (0: autotune)")
     ap.add_argument("--entry-bits", type=int, default=0, choices=[0, 16, 32],
                     help="list entry format (chmc_set_list_entry_bits): 0 chooses by the lists' size")
-    ap.add_argument("--list-check", type=int, default=2, choices=[1, 2, 3],
+    ap.add_argument("--list-check", type=int, default=2, choices=[1, 2, 3, 4],
                     help="list MD: rebuild checks before every LIST_CHECK-th force evaluation (chmc_set_list_check); "
-                         "tuned with the skin (1 or 2) unless given; 2 where the skin is fixed")
+                         "tuned with the skin (1, 2 or 3) unless given; 2 where the skin is fixed")
     ap.add_argument("--spec", type=int, default=8, choices=[1, 2, 4, 8, 16],
                     help="c2, c4: speculative copies of the chain (chmc_set_speculation; 1: the sequential loop).  The "
                          "chain, its records and its acceptance are the sequential run's, bit for bit")
@@ -581,7 +583,7 @@ def main():
         def run(n, sd):
             e.hmc(n, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=sd, use_graph=not args.no_graph)
         _, table = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), reps=args.tune,
-                                   lanes=args.lanes, checks=None if args.list_check_given else (1, 2))
+                                   lanes=args.lanes, checks=None if args.list_check_given else (1, 2, 3))
         skins = {k.split("/")[0] + ("/" + k.split("/")[2] if k.count("/") == 2 else ""): v for k, v in table.items()}
     else:
         e.set_skin(args.skin)
