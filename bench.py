@@ -498,7 +498,7 @@ def main():
     ap.add_argument("--skin", type=float, default=None, help="fixed list skin (no skin autotuning)")
     ap.add_argument("--skin-family", default=None,
                     help="fixed per-chain skins BASE/EXP of the density-scaled family (c3, c5; no skin autotuning)")
-    ap.add_argument("--lanes", type=int, default=0, choices=[0, 4, 8, 16, 812, 816],
+    ap.add_argument("--lanes", type=int, default=0, choices=[0, 4, 8, 16, 810, 811, 812, 816],
                     help="fixed list-sweep lanes per atom (0: autotune)")
     ap.add_argument("--entry-bits", type=int, default=0, choices=[0, 16, 32],
                     help="list entry format (chmc_set_list_entry_bits): 0 chooses by the lists' size")

@@ -2262,14 +2262,15 @@ extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* m
 extern "C" int chmc_set_list_lanes(chmc_ctx* c, int lanes)
 {
     CHECK_CTX(c);
-    if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 812 && lanes != 816)
-        return fail(c, CHMC_E_ARG, "set_list_lanes: 0 (autotune), 4, 8, 16, 812 or 816");
+    if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 810 && lanes != 811 && lanes != 812 &&
+        lanes != 816)
+        return fail(c, CHMC_E_ARG, "set_list_lanes: 0 (autotune), 4, 8, 16, 810, 811, 812 or 816");
     int rc = list_ensure(c);
     if (rc) return rc;
     if (lanes) return set_list_shape(c, lanes);
     double best = 1e30;
     int pick = 8;
-    for (int lp : { 4, 8, 16, 812, 816 }) {
+    for (int lp : { 4, 8, 16, 810, 811, 812, 816 }) {
         if ((rc = set_list_shape(c, lp))) return rc;
         double ms = 0.0;
         if ((rc = chmc_time_list(c, 20, &ms, nullptr))) return rc;

@@ -119,10 +119,11 @@ constexpr int kPadSeg = 27;
 constexpr int kRankBits = 8;       // a list cell holds at most 256 members
 constexpr uint16_t kPadEntry = (uint16_t)(kPadSeg << kRankBits);
 // the unconditional first batch of each list-sweep shape (lanes per atom x entries per lane): 4 lanes
-// x 16, 8 x 14, 16 x 7, 8 x 12 ("812"), 8 x 16 ("816"); rows are padded at least this far
+// x 16, 8 x 14, 16 x 7, 8 x 10, 11, 12 or 16 ("810" ... "816"); rows are padded at least this far
 __host__ __device__ constexpr int sweep_span(int shape)
 {
-    return shape == 4 ? 64 : shape == 8 ? 112 : shape == 16 ? 112 : shape == 812 ? 96 : 128;
+    return shape == 4 ? 64 : shape == 8 ? 112 : shape == 16 ? 112 : shape == 810 ? 80 : shape == 811 ? 88
+         : shape == 812 ? 96 : 128;
 }
 constexpr float kFarShift = 1.0e18f;
 constexpr int kScanCells = 3072;   // list cells of a chain whose counts one build workgroup scans in LDS
@@ -146,8 +147,8 @@ struct DevView {
     ChainState* state;    // [nchains]
     int* traj;            // [1] device trajectory counter (1-based index of the last finished one)
     // neighbour-list MD
-    int Cl, ntiles, nbcap, lanes;   // lanes: list-sweep shape (4, 8, 16 lanes per slot; 812, 816: 8 lanes with
-                                    // first batches of 12 / 16 entries)
+    int Cl, ntiles, nbcap, lanes;   // lanes: list-sweep shape (4, 8, 16 lanes per slot; 810 ... 816: 8 lanes
+                                    // with first batches of 10 ... 16 entries)
     const int* lcell_chain;
     const Chunk* tiles;
     int *lcount, *loffs, *lperm, *linv, *nbn, *lflag, *lreq, *lscan_tmp, *lcs, *ctab;

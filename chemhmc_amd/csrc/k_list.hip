@@ -1203,8 +1203,8 @@ hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s)
 }
 
 // ---- launchers -----------------------------------------------------------------------------------
-// sweep shape: 4, 8, 16 lanes per slot with first batches of 16, 14, 7 entries, or 812 / 816 (8 lanes with
-// batches of 12 / 16)
+// sweep shape: 4, 8, 16 lanes per slot with first batches of 16, 14, 7 entries, or 810 / 811 / 812 / 816
+// (8 lanes with batches of 10 / 11 / 12 / 16)
 template <int MODE, bool W16>
 static hipError_t sweep_lp_fmt(const DevView& v, int shape, int q, double kick_dt, double drift_dt, int ck, hipStream_t s)
 {
@@ -1213,6 +1213,8 @@ static hipError_t sweep_lp_fmt(const DevView& v, int shape, int q, double kick_d
     case 4: hipLaunchKernelGGL((k_lsweep<MODE, 4, 16, W16>), g, dim3(kTile * 4), 0, s, v, kick_dt, drift_dt, q, ck); break;
     case 8: hipLaunchKernelGGL((k_lsweep<MODE, 8, 14, W16>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
     case 16: hipLaunchKernelGGL((k_lsweep<MODE, 16, 7, W16>), g, dim3(kTile * 16), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 810: hipLaunchKernelGGL((k_lsweep<MODE, 8, 10, W16>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
+    case 811: hipLaunchKernelGGL((k_lsweep<MODE, 8, 11, W16>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
     case 812: hipLaunchKernelGGL((k_lsweep<MODE, 8, 12, W16>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
     case 816: hipLaunchKernelGGL((k_lsweep<MODE, 8, 16, W16>), g, dim3(kTile * 8), 0, s, v, kick_dt, drift_dt, q, ck); break;
     default: return hipErrorInvalidValue;
@@ -1224,7 +1226,8 @@ template <int MODE>
 static hipError_t sweep_lp(const DevView& v, int shape, int q, double kick_dt, double drift_dt, int ck, hipStream_t s)
 {
     static_assert(sweep_span(4) == 4 * 16 && sweep_span(8) == 8 * 14 && sweep_span(16) == 16 * 7 &&
-                  sweep_span(812) == 8 * 12 && sweep_span(816) == 8 * 16, "shape spans");
+                  sweep_span(810) == 8 * 10 && sweep_span(811) == 8 * 11 && sweep_span(812) == 8 * 12 &&
+                  sweep_span(816) == 8 * 16, "shape spans");
     if (sweep_span(shape) > v.lspan) return hipErrorInvalidValue;    // rows not padded that far
     return v.entry16 ? sweep_lp_fmt<MODE, true>(v, shape, q, kick_dt, drift_dt, ck, s)
                      : sweep_lp_fmt<MODE, false>(v, shape, q, kick_dt, drift_dt, ck, s);

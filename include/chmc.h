@@ -272,8 +272,8 @@ int chmc_list_stats(chmc_ctx* ctx, int chain, long long* entries, long long* nin
  * one rebuild, over reps, at the current positions (the positions do not change: dt = 0) */
 int chmc_time_list(chmc_ctx* ctx, int reps, double* ms_sweep, double* ms_rebuild);
 /* shape of the list sweep: 4, 8 or 16 lanes per atom (first unconditional batches of 16, 14 and 7 list
- * entries per lane), or 812 / 816 (8 lanes, batches of 12 / 16 entries); 0 runs the autotuner (times
- * each at the current positions and keeps the fastest) */
+ * entries per lane), or 810 / 811 / 812 / 816 (8 lanes, batches of 10 / 11 / 12 / 16 entries); 0 runs
+ * the autotuner (times each at the current positions and keeps the fastest) */
 int chmc_set_list_lanes(chmc_ctx* ctx, int lanes);
 int chmc_get_list_lanes(const chmc_ctx* ctx, int* lanes);
 /* list entry format: 32 (neighbour slot and image code per entry), 16 (segment of the list cell's
