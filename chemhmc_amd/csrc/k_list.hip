@@ -31,7 +31,7 @@
 //   drift x in FP64 (identical_particles.cpp:651-673), and lane 0 writes the FP32 coordinates of the
 //   other parity and raises the rebuild request when the atom is beyond skin / 2 from x0.
 // Energy: FP64 coordinates relative to the rebuild cell origins (rel64 = r0 + wrap(x - x0)), the
-//   potential's r < rc test (identical_particles.cpp:432-435), entries in batches of 8 per lane,
+//   potential's r < rc test (identical_particles.cpp:432-435), entries in batches of 4 per lane,
 //   per-tile fixed-order partials.
 #include "chmc_internal.hpp"
 #include "surrogate.hpp"
@@ -1098,11 +1098,13 @@ __global__ __launch_bounds__(256) void k_lenergy(DevView v, int q)
         // sqrt(r2) < rc (the reference's test) decided without the square root away from the cut-off:
         // the FP64 square root is within one ulp, far inside a 1e-12 relative band
         const double lo2 = C.rc2 * (1.0 - 1e-12), hi2 = C.rc2 * (1.0 + 1e-12);
-        // batches of kEB entries per lane: the batch's entries, then its neighbours' coordinates, are
+        // batches of kEB entries per lane (4: 66 VGPRs, 7 waves per SIMD; 8 took 99 VGPRs and 4 waves and
+        // ran the trajectory 1.7 % slower with eight speculative copies): the batch's entries, then its
+        // neighbours' coordinates, are
         // loaded together (two memory round trips per batch); each lane still adds its entries in
         // increasing order.  Entries past the count read the row's last one (with 16-bit entries only
         // those below the count, and the padding, decode to slots of this buffer).
-        constexpr int kEB = 8;
+        constexpr int kEB = 4;
         for (int k0 = sub; k0 < n; k0 += 8 * kEB) {
             unsigned ent[kEB];
 #pragma unroll
