@@ -974,6 +974,8 @@ __global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const flo
         auto pair_force = [&](const float4 pd, const float gj[kSurrFeat]) {
             const float r = sqrtf(__builtin_fmaf(pd.z, pd.z, __builtin_fmaf(pd.y, pd.y, pd.x * pd.x)));
             const float fc = 0.5f * (__cosf(pirc * r) + 1.0f), dfc = -0.5f * pirc * __sinf(pirc * r);
+            // phi_k'(r) = g_k(r) (dfc - (r - mu_k) fc / w^2) = g_k(r) (P + Q mu_k): one FMA per centre
+            const float Q = iw2 * fc, P = __builtin_fmaf(-Q, r, dfc);
             float acc = 0.f;
 #pragma unroll
             for (int b = 0; b < kSurrFeat / 8; b++) {
@@ -983,8 +985,7 @@ __global__ __launch_bounds__(kTile * LP) void k_lsurr_sweep(DevView v, const flo
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
                     const int f = 8 * b + u;
-                    const float tt = r - mu_k(f);
-                    const float dphi = ex * __builtin_fmaf(-tt * iw2, fc, dfc);
+                    const float dphi = ex * __builtin_fmaf(Q, mu_k(f), P);
                     acc = __builtin_fmaf(gi[f] + gj[f], dphi, acc);
                     ex *= rho;
                     rho *= cq;
