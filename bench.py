@@ -3,8 +3,9 @@ rc = 2.5, 2nd-order Omelyan (lambda = 0.1932) with 10 steps of dt = 0.01, FP32 p
 positions / momenta / dH, on one MI355X.
 
 A "step" is one HMC trajectory (heat bath, 10 OMF2 steps = 21 force evaluations, FP64 V_f, K_f,
-Metropolis, restore-on-reject), replayed from one captured hipGraph.  The timed region starts with
-the state resident in HBM and ends after the device has finished the K-th trajectory.
+Metropolis, restore-on-reject), its kernels launched directly on the context stream (--graph replays
+captured hipGraphs instead: no faster per trajectory here, and each call pays its captures).  The timed
+region starts with the state resident in HBM and ends after the device has finished the K-th trajectory.
 
   value      atom-steps/s = N * 10 MD steps * K trajectories / seconds
   thermalisation  a fixed burn-in of accept-all trajectories (the reference's thermalization_steps)
@@ -148,9 +149,9 @@ def other_workload(args):
     therm = args.therm if args.therm_given else 150
     warm = args.warmup if args.warmup_given else 0
     t0 = time.perf_counter()
-    e.hmc(therm, therm, w.scheme, w.dt, w.steps, lam=w.lam, seed=10, use_graph=not args.no_graph)
+    e.hmc(therm, therm, w.scheme, w.dt, w.steps, lam=w.lam, seed=10, use_graph=args.graph)
     if warm > 0:
-        e.hmc(warm, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
+        e.hmc(warm, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=args.graph)
     t_warm = time.perf_counter() - t0
     # skin autotuner (wall time of a few trajectories per candidate, paired from the warmed-up state,
     # which it restores), unless --skin fixes one skin for every chain or --skin-family BASE/EXP fixes
@@ -165,7 +166,7 @@ def other_workload(args):
         e.set_list_lanes(args.lanes)
     elif args.skin is None:
         def run(n, sd):
-            e.hmc(n, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=sd, use_graph=not args.no_graph)
+            e.hmc(n, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=sd, use_graph=args.graph)
         checks = None if args.list_check_given else (1, 2, 3)
         if len(w.chains) > 1:
             _, skins = tune.tune_skins(e, run, lanes=args.lanes, checks=checks)
@@ -178,7 +179,7 @@ def other_workload(args):
         e.set_list_lanes(args.lanes)
     chain_skins = np.array([e.chain_skin(k) for k in range(len(w.chains))])
     t0 = time.perf_counter()
-    rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=not args.no_graph)
+    rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=args.graph)
     el = time.perf_counter() - t0
     value = nat * w.steps * steps / el
     ent = nint = 0
@@ -246,9 +247,9 @@ def surrogate_c5_workload(args):
     e = chmc.Engine(w.chains)
     e.init_lattice(1, 0.05)
     warm = 10                   # accept-all trajectories off the lattice start
-    e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=not args.no_graph)
+    e.hmc(warm, warm, w.scheme, w.dt, w.steps, lam=w.lam, seed=11, use_graph=args.graph)
     if args.warmup_given and args.warmup > 0:
-        e.hmc(args.warmup, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=9, use_graph=not args.no_graph)
+        e.hmc(args.warmup, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=9, use_graph=args.graph)
         warm += args.warmup
     e.set_skin(0.5)
     e.surrogate_set_params(params)
@@ -271,7 +272,7 @@ def surrogate_c5_workload(args):
         e.set_trajectory(warm + nconf + 2)
         e.kinetic()
         t0 = time.perf_counter()
-        rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=not args.no_graph,
+        rec, nacc = e.hmc(steps, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=12, use_graph=args.graph,
                           surrogate=surr, fallback_window=0)
         el = time.perf_counter() - t0
         out[name] = {"acceptance": float(nacc[0]) / steps, "ms_per_traj": el / steps * 1e3,
@@ -358,12 +359,12 @@ def surrogate_workload(args):
     for name, surr in (("plain", False), ("surrogate", True)):
         e.set_positions(x0)
         e.set_trajectory(t_start)
-        e.hmc(2, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph, surrogate=surr)   # graph warm-up
+        e.hmc(2, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=args.graph, surrogate=surr)   # graph warm-up
         e.set_positions(x0)
         e.set_trajectory(t_start)
         e.kinetic()
         t0 = time.perf_counter()
-        rec, nacc = e.hmc(steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph,
+        rec, nacc = e.hmc(steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=args.graph,
                           surrogate=surr, fallback_window=0)
         el = time.perf_counter() - t0
         uu = rec["u"][:, 0] / N
@@ -398,12 +399,12 @@ def surrogate_workload(args):
 def cell_md_run(args, e, chmc, stats):
     """The same c2 measurement through the 27-cell sweep with a re-binning per force evaluation (the
     reference's cost model, CHMC_CELL_MD) instead of the neighbour lists."""
-    e.hmc(THERM, THERM, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10, use_graph=not args.no_graph)
+    e.hmc(THERM, THERM, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10, use_graph=args.graph)
     if args.warmup > 0:
-        e.hmc(args.warmup, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
+        e.hmc(args.warmup, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=args.graph)
     e.set_force_tiling(0)
     t0 = time.perf_counter()
-    rec, nacc = e.hmc(args.steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph)
+    rec, nacc = e.hmc(args.steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=args.graph)
     el = time.perf_counter() - t0
     u = rec["u"][:, 0] / N
     tau, _ = stats.tau_int(u)
@@ -506,9 +507,11 @@ def main():
     ap.add_argument("--list-check", type=int, default=2, choices=[1, 2, 3, 4],
                     help="list MD: rebuild checks before every LIST_CHECK-th force evaluation (chmc_set_list_check); "
                          "tuned with the skin (1, 2 or 3) unless given; 2 where the skin is fixed")
-    ap.add_argument("--spec", type=int, default=8, choices=[1, 2, 4, 8, 16],
-                    help="c2, c4: speculative copies of the chain (chmc_set_speculation; 1: the sequential loop).  The "
-                         "chain, its records and its acceptance are the sequential run's, bit for bit")
+    ap.add_argument("--spec", type=int, default=0, choices=[0, 1, 2, 4, 8, 10, 16],
+                    help="c2, c4: speculative copies of the chain (chmc_set_speculation; 1: the sequential loop; 0: "
+                         "8, or for runs shorter than 160 trajectories the count that splits them into whole "
+                         "batches of at most 10).  The chain, its records and its acceptance are the sequential "
+                         "run's, bit for bit")
     ap.add_argument("--start-rebuild", type=int, default=1, choices=[0, 1],
                     help="c2, c4: rebuild the lists at the start of the trajectory after an accepted one "
                          "(chmc_set_list_start_rebuild; pays at low acceptance)")
@@ -519,8 +522,10 @@ def main():
                     help="c2: ESS from the timed run itself, no separate long run (counter and trace runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--no-graph", action="store_true",
-                    help="launch every trajectory directly instead of replaying the captured graph")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay captured hipGraphs of the trajectories instead of launching them directly (each "
+                         "hmc call captures its own: ~3 ms with eight speculative copies, no faster per trajectory)")
+    ap.add_argument("--no-graph", action="store_true", help="direct launches (the default; overrides --graph)")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "c5s", "kernels", "calib"],
                     help="c2 (default, the headline line), c3 batched chains, c4 surrogate vs plain, c5 N=1M, "
                          "c5s the surrogate fine-tuned and run at N=1M, kernels: every hot-path kernel a few "
@@ -536,6 +541,12 @@ def main():
     args.warmup_given = "--warmup" in sys.argv
     args.list_check_given = "--list-check" in sys.argv
     args.therm_given = "--therm" in sys.argv
+    args.graph = args.graph and not args.no_graph
+    if args.spec == 0:
+        # a batch runs all its copies whether the run needs them or not: a short run takes the copy count
+        # that divides it into whole batches (20 trajectories: 2 batches of 10 instead of 8 + 8 + 4)
+        nb = -(-args.steps // 10)
+        args.spec = 8 if args.steps >= 160 else -(-args.steps // nb)
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
@@ -572,9 +583,9 @@ def main():
         cell_md_run(args, e, chmc, stats)
         return
     # accept-all thermalisation away from the lattice (thermalization_steps), then the untimed warm-up
-    e.hmc(args.therm, args.therm, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10, use_graph=not args.no_graph)
+    e.hmc(args.therm, args.therm, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=10, use_graph=args.graph)
     if args.warmup > 0:
-        e.hmc(args.warmup, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=not args.no_graph)
+        e.hmc(args.warmup, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=11, use_graph=args.graph)
     # skin autotuner: wall time of a short run per skin from the equilibrated state (paired: every skin
     # starts from that state with the same random streams, and the state is restored afterwards, so the
     # timed run continues from it whatever the tuner ran), with the lanes per atom of the list sweep
@@ -582,7 +593,7 @@ def main():
     skins = {}
     if args.skin is None:
         def run(n, sd):
-            e.hmc(n, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=sd, use_graph=not args.no_graph)
+            e.hmc(n, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=sd, use_graph=args.graph)
         _, table = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), reps=args.tune,
                                    lanes=args.lanes, checks=None if args.list_check_given else (1, 2, 3))
         skins = {k.split("/")[0] + ("/" + k.split("/")[2] if k.count("/") == 2 else ""): v for k, v in table.items()}
@@ -594,7 +605,7 @@ def main():
     _, _, rebuilds0 = e.list_stats()
 
     t0 = time.perf_counter()
-    rec, nacc = e.hmc(args.steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=not args.no_graph)
+    rec, nacc = e.hmc(args.steps, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=12, use_graph=args.graph)
     el = time.perf_counter() - t0
     ms = el / args.steps * 1e3
     value = N * NSTEPS * args.steps / el
@@ -609,7 +620,7 @@ def main():
         ess_rec, ess_acc, ess_el = rec, nacc, el
     else:
         t0 = time.perf_counter()
-        ess_rec, ess_acc = e.hmc(ESS_TRAJ, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=13, use_graph=not args.no_graph)
+        ess_rec, ess_acc = e.hmc(ESS_TRAJ, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=13, use_graph=args.graph)
         ess_el = time.perf_counter() - t0
     u = ess_rec["u"][:, 0] / N
     nt = u.size
@@ -649,6 +660,7 @@ def main():
                    "thermalisation": args.therm, "md_path": "neighbour lists", "speculative_copies": copies,
                    "skin": skin, "list_lanes": lanes, "list_check": e.list_check(),
                    "list_entry_bits": e.list_entry_bits(), "start_rebuild": e.list_start_rebuild(),
+                   "launches": "graph replay" if args.graph else "direct",
                    "ms_per_traj_by_skin": skins},
         "acceptance": float(nacc[0]) / args.steps,
         "ess_per_s": ess_info["ess_per_s"], "ess": ess_info,

@@ -1352,7 +1352,7 @@ static int enqueue_spec_batch(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst
     HIPCHK(c, launch_reduce(c->v, SUM_TILES | SUM_CHUNKS_KI, c->stream));
     HIPCHK(c, launch_spec_restore(c->v, c->d_uniforms, hp.therm, ufirst, c->d_rec, rec_base, c->md_surr ? 1 : 0, c->lpar,
                                   c->stream));
-    HIPCHK(c, launch_spec_broadcast(c->v, -1, true, c->stream));
+    HIPCHK(c, launch_spec_broadcast(c->v, -1, true, true, c->stream));
     c->list_fresh = c->rel64_fresh = false;
     c->force_kind = 0;
     c->bin_state = 2;
@@ -1435,15 +1435,15 @@ static int run_segment(chmc_ctx* c, const chmc_hmc_params& hp, int t0, int count
 // a speculative run: batches of nchains trajectories until the device counter reaches tend.  A batch
 // consumes between one trajectory and nchains, so the host enqueues as many batches as the remaining
 // trajectories need when none is accepted, reads the counter and repeats.
-static int run_spec(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst, int rec_base, int tend,
+static int run_spec(chmc_ctx* c, const chmc_hmc_params& hp, int ufirst, int rec_base, int done0, int tend,
                     std::vector<TrajGraph>& graphs)
 {
     const int K = c->nchains;
-    int done, rc;
-    // the window's last trajectory, read by the decisions of every batch (graph replays included)
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(c->v.spec_ctl + 3, &tend, sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(&done, c->v.traj, sizeof(int), hipMemcpyDeviceToHost));
+    int rc;
+    // the window's last trajectory, read by the decisions of every batch (graph replays included), and
+    // its first: the device counter holds the trajectories committed so far
+    HIPCHK(c, launch_fill_int(c->v.spec_ctl + 3, tend, 1, c->stream));
+    int done = done0;
     while (done < tend) {
         const int nb = (tend - done + K - 1) / K;
         auto enq = [&]() { return enqueue_spec_batch(c, hp, ufirst, rec_base); };
@@ -1527,24 +1527,18 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     if (slot_run) {
         // with start rebuilds the run starts from lists built at its start positions (as after an
         // accepted trajectory)
-        if ((rc = list_import(c)) || ((c->v.start_rebuild || !c->list_valid) && (rc = list_invalidate(c)))) return rc;
-        // speculative copies start from chain 0's state (positions, lists, rebuild requests)
-        if (spec) HIPCHK(c, launch_spec_broadcast(c->v, 0, false, c->stream));
+        const bool rebuild_all = c->v.start_rebuild || !c->list_valid;
+        if ((rc = list_import(c)) || (rebuild_all && (rc = list_invalidate(c)))) return rc;
+        // speculative copies start from chain 0's state (positions, lists, rebuild requests); its list
+        // rows only when the copies do not all rebuild next
+        if (spec) HIPCHK(c, launch_spec_broadcast(c->v, 0, false, !rebuild_all, c->stream));
         if ((rc = list_energy(c, true))) return rc;
         c->list_valid = true;
     } else if ((rc = do_energy(c, true))) {
         return rc;
     }
-    std::vector<double> red((size_t)nch * RED_NSLOT);
-    HIPCHK(c, hipMemcpyAsync(red.data(), c->v.red, sizeof(double) * red.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     std::vector<ChainState> st(nch);
-    for (int k = 0; k < nch; k++) {
-        st[k] = ChainState{};
-        st[k].Vi = red[(size_t)k * RED_NSLOT + RED_V];
-        st[k].Wi = red[(size_t)k * RED_NSLOT + RED_W];
-    }
-    HIPCHK(c, hipMemcpyAsync(c->v.state, st.data(), sizeof(ChainState) * nch, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_state_init(c->v, c->stream));
     if (slot_run) {
         HIPCHK(c, launch_save_restore_slots(c->v, 0, c->lpar, c->stream));
     } else {
@@ -1565,7 +1559,7 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
             c->list_valid = false;
             if ((rc = list_import(c))) break;
         }
-        rc = spec ? run_spec(c, *hp, ufirst, rec_base, rec_base + t0 + count, graphs)
+        rc = spec ? run_spec(c, *hp, ufirst, rec_base, rec_base + t0, rec_base + t0 + count, graphs)
                   : run_segment(c, *hp, t0, count, ufirst, rec_base, rsteps, graphs);
         if (rc || !c->md_surr) continue;
         if ((rc = surr_check_flag(c))) break;

@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void k_spec_restore(DevView v, const double* _
 // starts, rebuild requests), with slot and atom numbers moved to the target copy's range, so that every
 // copy continues exactly as
 // copy j would.  The momenta are not copied: the next heat bath draws them.
-__global__ __launch_bounds__(256) void k_spec_broadcast(DevView v, int src, int commit)
+__global__ __launch_bounds__(256) void k_spec_broadcast(DevView v, int src, int commit, int rows)
 {
     if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
         *v.traj += v.spec_ctl[1];
@@ -400,7 +400,9 @@ __global__ __launch_bounds__(256) void k_spec_broadcast(DevView v, int src, int 
         }
     }
     // list rows, 16 bytes per thread (four 32-bit entries with their slots moved by the copies' offset,
-    // or eight 16-bit ones, relative to the segment starts, as they are), up to the padded span or the count
+    // or eight 16-bit ones, relative to the segment starts, as they are), up to the padded span or the
+    // count; without rows (every copy rebuilds its lists next) the rows and segment starts stay
+    if (!rows) return;
     const bool e16 = v.entry16 != 0;
     const int per = e16 ? 8 : 4, qn = v.nbcap / per, span = v.lspan;
     const size_t rowb = (size_t)v.nbcap * (e16 ? 2 : 4);
@@ -505,9 +507,26 @@ hipError_t launch_spec_restore(const DevView& v, const double* uniforms, int the
     return hipGetLastError();
 }
 
-hipError_t launch_spec_broadcast(const DevView& v, int src, bool commit, hipStream_t s)
+hipError_t launch_spec_broadcast(const DevView& v, int src, bool commit, bool rows, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_spec_broadcast, dim3(1024), dim3(256), 0, s, v, src, commit ? 1 : 0);
+    hipLaunchKernelGGL(k_spec_broadcast, dim3(1024), dim3(256), 0, s, v, src, commit ? 1 : 0, rows ? 1 : 0);
+    return hipGetLastError();
+}
+
+// chain states of a run start: V_i and W_i of the start configuration from the per-chain sums, the rest zero
+__global__ void k_state_init(DevView v)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= v.nchains) return;
+    ChainState st{};
+    st.Vi = v.red[(size_t)k * RED_NSLOT + RED_V];
+    st.Wi = v.red[(size_t)k * RED_NSLOT + RED_W];
+    v.state[k] = st;
+}
+
+hipError_t launch_state_init(const DevView& v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_state_init, dim3((v.nchains + 255) / 256), dim3(256), 0, s, v);
     return hipGetLastError();
 }
 
