@@ -1578,20 +1578,9 @@ extern "C" int chmc_hmc_run(chmc_ctx* c, const chmc_hmc_params* hp, chmc_hmc_rec
     if (rc) return rc;
     if ((rc = list_export(c, true))) return rc;
     c->saved_lists = false;
-    if (spec) {
-        // the chain's momenta after the run are the post-MD momenta of its last trajectory, which
-        // copy spec_ctl[2] ran (the positions are the same in every copy)
-        int ctl[4];
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        HIPCHK(c, hipMemcpy(ctl, c->v.spec_ctl, sizeof(ctl), hipMemcpyDeviceToHost));
-        const int last = ctl[2];
-        if (last > 0 && last < nch) {
-            const size_t Nt = (size_t)c->Ntot, n = (size_t)c->hchains[0].natoms;
-            for (int d = 0; d < 3; d++)
-                HIPCHK(c, hipMemcpyAsync(c->v.p + d * Nt + c->hchains[0].atom_off, c->v.p + d * Nt + c->hchains[last].atom_off,
-                                         sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-        }
-    }
+    // the chain's momenta after the run are the post-MD momenta of its last trajectory, which copy
+    // spec_ctl[2] ran (the positions are the same in every copy)
+    if (spec) HIPCHK(c, launch_spec_final_momenta(c->v, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (list_md(c) && (rc = list_check_overflow(c))) return rc;
     c->traj_done += ntraj;

@@ -375,6 +375,7 @@ hipError_t launch_spec_restore(const DevView& v, const double* uniforms, int the
                                int rec_base, int surr, int list_q, hipStream_t s);
 hipError_t launch_spec_broadcast(const DevView& v, int src, bool commit, bool rows, hipStream_t s);
 hipError_t launch_state_init(const DevView& v, hipStream_t s);
+hipError_t launch_spec_final_momenta(const DevView& v, hipStream_t s);
 // x, p (and x_saved from xl_saved) <-> xl, pl in the current list buffer's slot order
 hipError_t launch_list_import(const DevView& v, hipStream_t s);
 hipError_t launch_list_export(const DevView& v, bool saved, hipStream_t s);

@@ -513,6 +513,28 @@ hipError_t launch_spec_broadcast(const DevView& v, int src, bool commit, bool ro
     return hipGetLastError();
 }
 
+// end of a speculative run: copy 0 (the chain) takes the atom-order momenta of copy spec_ctl[2], which ran
+// the run's last trajectory
+__global__ __launch_bounds__(256) void k_spec_final_momenta(DevView v)
+{
+    const int last = v.spec_ctl[2];
+    if (last <= 0 || last >= v.nchains) return;
+    const ChainDev& C0 = v.chains[0];
+    const ChainDev& Cl = v.chains[last];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C0.natoms) return;
+    const size_t Nt = (size_t)v.Ntot;
+#pragma unroll
+    for (int d = 0; d < 3; d++) v.p[d * Nt + C0.atom_off + i] = v.p[d * Nt + Cl.atom_off + i];
+}
+
+hipError_t launch_spec_final_momenta(const DevView& v, hipStream_t s)
+{
+    const int n = v.Ntot / v.nchains;              // the copies' common size
+    hipLaunchKernelGGL(k_spec_final_momenta, dim3((n + 255) / 256), dim3(256), 0, s, v);
+    return hipGetLastError();
+}
+
 // chain states of a run start: V_i and W_i of the start configuration from the per-chain sums, the rest zero
 __global__ void k_state_init(DevView v)
 {
