@@ -58,10 +58,11 @@ SWEEP_TRAFFIC_PROFILED = {(0.6, 8, 1): (FETCH_SCALE * 9976.0 + 2048.0) * 1024.0,
                           (0.4, 812, 8): (FETCH_SCALE * 60488.0 + 16384.2) * 1024.0,
                           (0.4, 811, 8): (FETCH_SCALE * 60490.2 + 16388.0) * 1024.0}
 # the same at N=1,048,576 (workload c5), keyed by (skin, lanes, list entry bits): 32-bit entries at skin
-# 0.5 with 8 lanes and skin 0.7 with the 816 shape, 16-bit entries (the default at this size) at skin 0.6
-# with 8 lanes and skin 0.7 with the 816 shape (profiles/lj_list_sweep_N1048576.md)
+# 0.5 with 8 lanes and skin 0.7 with the 816 shape, 16-bit entries (the default at this size) at skins
+# 0.5 and 0.6 with 8 lanes and skin 0.7 with the 816 shape (profiles/lj_list_sweep_N1048576.md)
 SWEEP_TRAFFIC_PROFILED_C5 = {(0.5, 8, 32): (FETCH_SCALE * 322982.0 + 65536.0) * 1024.0,
                              (0.7, 816, 32): (FETCH_SCALE * 323121.7 + 65536.3) * 1024.0,
+                             (0.5, 8, 16): (FETCH_SCALE * 229185.4 + 65536.0) * 1024.0,
                              (0.6, 8, 16): (FETCH_SCALE * 229220.5 + 65536.2) * 1024.0,
                              (0.7, 816, 16): (FETCH_SCALE * 229060.2 + 65536.0) * 1024.0}
 # the same for the 256-chain batch (workload c3) at the per-chain skin families base 0.4, exponent 1.5
