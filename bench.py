@@ -4,8 +4,11 @@ positions / momenta / dH, on one MI355X.
 
 A "step" is one HMC trajectory (heat bath, 10 OMF2 steps = 21 force evaluations, FP64 V_f, K_f,
 Metropolis, restore-on-reject), its kernels launched directly on the context stream (--graph replays
-captured hipGraphs instead: no faster per trajectory here, and each call pays its captures).  The timed
-region starts with the state resident in HBM and ends after the device has finished the K-th trajectory.
+captured hipGraphs instead: no faster per trajectory here, and each call pays its captures).  The chain
+runs as speculative copies (--spec; by default 8, or for runs shorter than 160 trajectories the count
+that splits them into whole batches): a batch runs the chain's next trajectories at once, and the
+records are the sequential chain's, bit for bit.  The timed region starts with the state resident in
+HBM and ends after the device has finished the K-th trajectory.
 
   value      atom-steps/s = N * 10 MD steps * K trajectories / seconds
   thermalisation  a fixed burn-in of accept-all trajectories (the reference's thermalization_steps)
