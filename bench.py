@@ -59,7 +59,11 @@ SWEEP_TRAFFIC_PROFILED = {(0.6, 8, 1): (FETCH_SCALE * 9976.0 + 2048.0) * 1024.0,
                           (0.7, 8, 1): (FETCH_SCALE * 9970.0 + 2048.0) * 1024.0,
                           (0.7, 816, 1): (FETCH_SCALE * 10086.8 + 2048.0) * 1024.0,
                           (0.4, 812, 8): (FETCH_SCALE * 60488.0 + 16384.2) * 1024.0,
-                          (0.4, 811, 8): (FETCH_SCALE * 60490.2 + 16388.0) * 1024.0}
+                          (0.4, 811, 8): (FETCH_SCALE * 60490.2 + 16388.0) * 1024.0,
+                          (0.4, 811, 10): (FETCH_SCALE * 75927.1 + 20483.4) * 1024.0,
+                          (0.4, 812, 10): (FETCH_SCALE * 75931.4 + 20480.3) * 1024.0,
+                          (0.5, 811, 10): (FETCH_SCALE * 78359.7 + 20488.6) * 1024.0,
+                          (0.5, 812, 10): (FETCH_SCALE * 78363.0 + 20482.5) * 1024.0}
 # the same at N=1,048,576 (workload c5), keyed by (skin, lanes, list entry bits): 32-bit entries at skin
 # 0.5 with 8 lanes and skin 0.7 with the 816 shape, 16-bit entries (the default at this size) at skins
 # 0.5 and 0.6 with 8 lanes and skin 0.7 with the 816 shape (profiles/lj_list_sweep_N1048576.md)
