@@ -79,6 +79,17 @@ SWEEP_TRAFFIC_PROFILED_C3 = {("0.4/1.5", 4): (FETCH_SCALE * 115006.0 + 38233.0) 
                              ("0.45/1.0", 4): (FETCH_SCALE * 116471.5 + 38304.6) * 1024.0}
 
 
+def _tune(args, e, run, workload, **kw):
+    """The skin search (tune.tune_skins), or with --tune-cache its stored outcome for the same problem
+    (tune.cached_tune_skins).  Returns (skins, table, how the settings were found)."""
+    from chemhmc_amd import tune
+    if args.tune_cache:
+        skins, table, hit = tune.cached_tune_skins(e, run, args.tune_cache, workload, **kw)
+        return skins, table, "cache" if hit else "search (stored)"
+    skins, table = tune.tune_skins(e, run, **kw)
+    return skins, table, "search"
+
+
 def _omp_threads(n):
     """Set the OpenMP team size of the oracle library (libgomp) for the next parallel regions."""
     import ctypes
@@ -167,6 +178,7 @@ def other_workload(args):
     # chain's density, a single chain one skin; lanes per atom tuned unless --lanes fixes them
     skins = {}
     family = None
+    tuner = "fixed (--skin-family)" if args.skin_family is not None else "fixed (--skin)"
     if args.skin_family is not None:
         family = args.skin_family
         fb, fe = (float(t) for t in family.split("/"))
@@ -176,11 +188,13 @@ def other_workload(args):
         def run(n, sd):
             e.hmc(n, 0, w.scheme, w.dt, w.steps, lam=w.lam, seed=sd, use_graph=args.graph)
         checks = None if args.list_check_given else (1, 2, 3)
+        desc = {"workload": args.workload, "scheme": w.scheme, "dt": w.dt, "steps": w.steps, "lam": w.lam,
+                "graph": bool(args.graph), "list_check": args.list_check if args.list_check_given else None}
         if len(w.chains) > 1:
-            _, skins = tune.tune_skins(e, run, lanes=args.lanes, checks=checks)
+            _, skins, tuner = _tune(args, e, run, desc, lanes=args.lanes, checks=checks)
         else:
-            _, skins = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,),
-                                       lanes=args.lanes, checks=checks)
+            _, skins, tuner = _tune(args, e, run, desc, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,),
+                                    lanes=args.lanes, checks=checks)
         family = "/".join(min(skins, key=skins.get).split("/")[:2])
     else:
         e.set_skin(args.skin)
@@ -220,7 +234,7 @@ def other_workload(args):
                    "atoms": nat, "chains": len(w.chains), "list_lanes": e.list_lanes(), "list_check": e.list_check(),
                    "list_entry_bits": bits, "skin": e.skin(),
                    "chain_skin_range": [float(chain_skins.min()), float(chain_skins.max())], "skin_family": family,
-                   "ms_per_traj_by_skin": skins, "warmup_seconds": t_warm},
+                   "ms_per_traj_by_skin": skins, "tuner": tuner, "warmup_seconds": t_warm},
         "acceptance": float(nacc.sum()) / (steps * len(w.chains)),
         "ess_per_s_sum_over_chains": ess / el if moving else None, "chains_with_tau_int": len(moving),
         "roofline": {"bound": "hbm", "kernel": "k_lsweep<force|kick|drift> (FP32 list sweep)", "achieved": achieved,
@@ -505,6 +519,11 @@ def main():
     ap.add_argument("--tune", type=int, default=200,
                     help="trajectories per skin in the skin autotuner (long enough that the graph capture of "
                          "each timed call is a small part of it)")
+    ap.add_argument("--tune-cache", default=os.environ.get("CHMC_TUNE_CACHE"),
+                    help="JSON file of tuned settings (default $CHMC_TUNE_CACHE, unset: search every run): a run "
+                         "of a problem tuned before (same chains, workload, candidates and library build) takes "
+                         "the stored skins, check interval and lanes without searching; otherwise the search's "
+                         "outcome is added to it")
     ap.add_argument("--skin", type=float, default=None, help="fixed list skin (no skin autotuning)")
     ap.add_argument("--skin-family", default=None,
                     help="fixed per-chain skins BASE/EXP of the density-scaled family (c3, c5; no skin autotuning)")
@@ -599,11 +618,15 @@ def main():
     # timed run continues from it whatever the tuner ran), with the lanes per atom of the list sweep
     # (0: autotuned) chosen for every skin
     skins = {}
+    tuner = "fixed (--skin)"
     if args.skin is None:
         def run(n, sd):
             e.hmc(n, 0, chmc.OMF2, DT, NSTEPS, lam=LAM, seed=sd, use_graph=args.graph)
-        _, table = tune.tune_skins(e, run, bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), reps=args.tune,
-                                   lanes=args.lanes, checks=None if args.list_check_given else (1, 2, 3))
+        _, table, tuner = _tune(args, e, run, {"workload": "c2", "scheme": "OMF2", "dt": DT, "steps": NSTEPS, "lam": LAM,
+                                               "graph": bool(args.graph), "start_rebuild": bool(args.start_rebuild),
+                                               "list_check": args.list_check if args.list_check_given else None},
+                                bases=(0.3, 0.4, 0.5, 0.6, 0.7, 0.8), exponents=(0.0,), reps=args.tune,
+                                lanes=args.lanes, checks=None if args.list_check_given else (1, 2, 3))
         skins = {k.split("/")[0] + ("/" + k.split("/")[2] if k.count("/") == 2 else ""): v for k, v in table.items()}
     else:
         e.set_skin(args.skin)
@@ -669,7 +692,7 @@ def main():
                    "skin": skin, "list_lanes": lanes, "list_check": e.list_check(),
                    "list_entry_bits": e.list_entry_bits(), "start_rebuild": e.list_start_rebuild(),
                    "launches": "graph replay" if args.graph else "direct",
-                   "ms_per_traj_by_skin": skins},
+                   "ms_per_traj_by_skin": skins, "tuner": tuner},
         "acceptance": float(nacc[0]) / args.steps,
         "ess_per_s": ess_info["ess_per_s"], "ess": ess_info,
         "list_rebuilds_per_traj": rebuilds_per_traj,   # over all copies when speculative

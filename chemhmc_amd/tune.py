@@ -5,10 +5,16 @@ are rebuilt less often.  How the two balance depends on the chain: at low densit
 neighbours, so a wider shell costs little.  The tuner searches a two-parameter family of per-chain
 skins, skin_k = base * clamp((rho_ref / rho_k) ** exponent, lo, hi), by timing a few trajectories of
 the batch with each candidate, and keeps the fastest.  exponent = 0 is one skin for every chain.
-Candidates the list MD cannot hold (a list cell of more than 256 atoms) are skipped.
+Candidates the list MD cannot hold (a list cell of more than 256 atoms) are skipped.  cached_tune_skins
+keeps the outcome in a JSON file, keyed by the chains, the workload, the candidate set and the library
+build, so that a later run of the same problem sets the tuned skins, check interval and lanes without
+searching again.
 """
 from __future__ import annotations
 
+import hashlib
+import json
+import os
 import time
 
 import numpy as np
@@ -85,3 +91,56 @@ def tune_skins(engine, run, bases=(0.35, 0.4, 0.45, 0.5, 0.6), exponents=(0.0, 1
     restore(engine, snap)
     engine.set_list_lanes(best[2])
     return best[1], table
+
+
+def cache_key(engine, workload, candidates):
+    """Identity of a tuning problem: the chains (atoms, box, cut-off, LJ parameters, mass, beta), the
+    engine's speculative copies and list entry format, the caller's workload description, the candidate
+    settings of the search and the library build (size and modification time of libchmc.so), so that
+    another system, search or build is tuned afresh."""
+    from . import LIB_PATH
+    try:
+        st = os.stat(LIB_PATH)
+        build = [st.st_size, st.st_mtime_ns]
+    except OSError:
+        build = None
+    desc = {"chains": [[c.N, list(c.box()), c.cutoff, c.sigma, c.eps, c.mass, c.beta] for c in engine.chains],
+            "speculation": engine.speculation(), "entry_bits": engine.list_entry_bits(), "library": build,
+            "workload": workload, "candidates": candidates}
+    return hashlib.sha256(json.dumps(desc, sort_keys=True, default=list).encode()).hexdigest()
+
+
+def _read_cache(path):
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        return {}
+    return data if isinstance(data, dict) else {}
+
+
+def cached_tune_skins(engine, run, cache, workload, **kw):
+    """tune_skins(engine, run, **kw) with its outcome kept in the JSON file `cache` under
+    cache_key(engine, workload, kw).  A hit sets the stored skins, rebuild-check interval (when the search
+    covered it) and lanes, and runs nothing; a miss runs the search and adds its outcome to the file
+    (written whole to a temporary file and renamed over it).  Returns (skins, table, hit)."""
+    key = cache_key(engine, workload, kw)
+    entry = _read_cache(cache).get(key)
+    if isinstance(entry, dict):
+        skins = np.asarray(entry["skins"], dtype=np.float64)
+        engine.set_chain_skins(skins)
+        if entry.get("check") is not None:
+            engine.set_list_check(int(entry["check"]))
+        engine.set_list_lanes(int(entry["lanes"]))
+        return skins, dict(entry["table"]), True
+    skins, table = tune_skins(engine, run, **kw)
+    data = _read_cache(cache)
+    data[key] = {"skins": [float(v) for v in skins], "check": engine.list_check() if kw.get("checks") else None,
+                 "lanes": int(engine.list_lanes()), "table": table}
+    d = os.path.dirname(os.path.abspath(cache))
+    os.makedirs(d, exist_ok=True)
+    tmp = os.path.join(d, f".{os.path.basename(cache)}.{os.getpid()}.tmp")
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, cache)
+    return skins, table, False

@@ -1,6 +1,9 @@
 """Host logic of the skin autotuner (chemhmc_amd/tune.py) on a stand-in engine: the density-scaled skin
 family, and the paired protocol (every candidate from the same positions, trajectory counter and seeds,
-lanes set per candidate, candidates the list MD rejects skipped, the fastest kept, the state restored)."""
+lanes set per candidate, candidates the list MD rejects skipped, the fastest kept, the state restored),
+and the cache of tuned settings."""
+import json
+
 import numpy as np
 import pytest
 
@@ -34,7 +37,7 @@ class FakeEngine:
         self.chains, self.nchains = chains, len(chains)
         self.cost, self.reject_max = cost, reject_max
         self.x = [np.full((c.N, 3), float(k)) for k, c in enumerate(chains)]
-        self.traj, self.skins, self.lanes = 7, None, 8
+        self.traj, self.skins, self.lanes, self.check = 7, None, 8, 2
         self.log = []
 
     def positions(self, k):
@@ -60,6 +63,18 @@ class FakeEngine:
 
     def list_lanes(self):
         return self.lanes
+
+    def set_list_check(self, m):
+        self.check = m
+
+    def list_check(self):
+        return self.check
+
+    def speculation(self):
+        return 0
+
+    def list_entry_bits(self):
+        return 32
 
 
 def test_tune_skins_paired_protocol(monkeypatch):
@@ -101,3 +116,45 @@ def test_tune_skins_warm_and_no_candidate():
     calls = []
     tune.tune_skins(e, lambda n, s: calls.append((n, s)), bases=(0.3, 0.4), exponents=(0.0,), warm=3, reps=4, seed=50)
     assert calls == [(3, 50), (4, 51), (3, 50), (4, 51)]
+
+
+def test_cached_tune_skins(tmp_path, monkeypatch):
+    """A miss runs the search and stores its outcome; a later engine of the same problem gets the stored
+    skins, check interval and lanes without a single run; another workload or candidate set misses; a
+    damaged cache file reads as empty."""
+    clock = [0.0]
+    monkeypatch.setattr(tune.time, "perf_counter", lambda: clock[0])
+    runs = []
+
+    def make_run(e):
+        def run(n, seed):                        # ms per trajectory: smallest at skin 0.4, check 1
+            runs.append((n, seed, tuple(e.skins), e.check))
+            clock[0] += n * (1.0 + abs(e.skins.max() - 0.4) + 0.1 * e.check) * 1e-3
+        return run
+
+    cache = str(tmp_path / "sub" / "tune.json")
+    kw = dict(bases=(0.3, 0.4, 0.5), exponents=(0.0,), reps=2, lanes=0, checks=(1, 2))
+    e = FakeEngine(CHAINS, None)
+    skins, table, hit = tune.cached_tune_skins(e, make_run(e), cache, {"workload": "w"}, **kw)
+    assert not hit and len(runs) == 6 and len(table) == 6
+    assert np.allclose(skins, 0.4) and e.check == 1 and e.lanes == 4
+    e2 = FakeEngine(CHAINS, None)
+    e2.check, e2.lanes = 3, 16
+    runs.clear()
+    skins2, table2, hit2 = tune.cached_tune_skins(e2, make_run(e2), cache, {"workload": "w"}, **kw)
+    assert hit2 and not runs and table2 == table
+    assert np.allclose(skins2, skins) and np.allclose(e2.skins, skins) and e2.check == 1 and e2.lanes == 4
+    tune.cached_tune_skins(e2, make_run(e2), cache, {"workload": "v"}, **kw)
+    assert len(runs) == 6
+    runs.clear()
+    tune.cached_tune_skins(e2, make_run(e2), cache, {"workload": "w"}, **dict(kw, reps=3))
+    assert len(runs) == 6
+    with open(cache) as f:
+        assert len(json.load(f)) == 3
+    e3 = FakeEngine(CHAINS[:2], None)                # other chains: a miss
+    runs.clear()
+    assert not tune.cached_tune_skins(e3, make_run(e3), cache, {"workload": "w"}, **kw)[2] and len(runs) == 6
+    with open(cache, "w") as f:
+        f.write("{not json")
+    runs.clear()
+    assert not tune.cached_tune_skins(e2, make_run(e2), cache, {"workload": "w"}, **kw)[2] and len(runs) == 6
