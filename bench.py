@@ -310,7 +310,7 @@ def surrogate_c5_workload(args):
                    "w_force": C4_W_FORCE, "finetune_lr": 3e-4, "collect_seconds": t_collect, "finetune_seconds": t_train,
                    "energy_mse_first": float(le[:20].mean()), "energy_mse_last": float(le[-100:].mean()),
                    "force_mse_first": float(lf[:20].mean()), "force_mse_last": float(lf[-100:].mean())},
-        "plain": out["plain"], "surrogate": out["surrogate"],
+        "plain": out["plain"], "surrogate": out["surrogate"], "force_evaluation_kernels": kernels,
         "ms_per_traj_ratio_surrogate_over_plain": out["surrogate"]["ms_per_traj"] / out["plain"]["ms_per_traj"],
     }
     print(json.dumps(res))
@@ -397,6 +397,12 @@ def surrogate_workload(args):
                      "u_per_n": float(uu.mean()), "tau_int_u": tau if ok else None,
                      "ess_per_s": steps / (2 * tau) / el if ok else None, "reliable": stats.reliable(uu),
                      "value": N * NSTEPS * steps / el}
+    # per-launch device times of one force evaluation over every speculative copy (HIP events, 50 launches
+    # each at the current positions): the surrogate's descriptor + MLP and force sweep, the LJ list sweep
+    ms_desc, ms_ssweep = e.time_surrogate(reps=50)
+    ms_lj, _ = e.time_list(reps=50)
+    kernels = {"atoms_per_launch": N * e.speculation(), "ms_desc_mlp": ms_desc, "ms_surrogate_sweep": ms_ssweep,
+               "ms_lj_list_sweep": ms_lj, "ratio_per_force_evaluation": (ms_desc + ms_ssweep) / ms_lj}
     res = {
         "metric": "atom-steps/s (LJ force+integrator) and ESS/s of U/N", "value": out["surrogate"]["value"],
         "unit": "atom-steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
@@ -409,7 +415,7 @@ def surrogate_workload(args):
                    "collect_seconds": t_collect, "train_seconds": t_train,
                    "energy_mse_first": float(le[:20].mean()), "energy_mse_last": float(le[-100:].mean()),
                    "force_mse_first": float(lf[:20].mean()), "force_mse_last": float(lf[-100:].mean())},
-        "plain": out["plain"], "surrogate": out["surrogate"],
+        "plain": out["plain"], "surrogate": out["surrogate"], "force_evaluation_kernels": kernels,
         "ms_per_traj_ratio_surrogate_over_plain": out["surrogate"]["ms_per_traj"] / out["plain"]["ms_per_traj"],
         "ess_per_s_ratio_surrogate_over_plain": (out["surrogate"]["ess_per_s"] / out["plain"]["ess_per_s"]
                                                   if out["surrogate"]["ess_per_s"] and out["plain"]["ess_per_s"]

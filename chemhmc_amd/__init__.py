@@ -111,6 +111,7 @@ def _declare(L):
         "chmc_get_chain_skin": ([P, _I, C.POINTER(_D)], _I),
         "chmc_list_stats": ([P, _I, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], _I),
         "chmc_time_list": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
+        "chmc_time_surrogate": ([P, _I, C.POINTER(_D), C.POINTER(_D)], _I),
         "chmc_set_list_lanes": ([P, _I], _I),
         "chmc_set_list_check": ([P, _I], _I),
         "chmc_get_list_check": ([P, C.POINTER(_I)], _I),
@@ -155,7 +156,7 @@ EXPORTED = (
     "chmc_set_force_tiling",
     "chmc_get_force_tiling", "chmc_time_force", "chmc_set_skin", "chmc_get_skin", "chmc_set_chain_skins",
     "chmc_get_chain_skin", "chmc_list_stats",
-    "chmc_time_list", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_list_entry_bits", "chmc_get_list_entry_bits", "chmc_set_list_check", "chmc_get_list_check", "chmc_set_speculation", "chmc_get_speculation", "chmc_set_list_start_rebuild", "chmc_get_list_start_rebuild", "chmc_set_trajectory", "chmc_get_trajectory",
+    "chmc_time_list", "chmc_time_surrogate", "chmc_set_list_lanes", "chmc_get_list_lanes", "chmc_set_list_entry_bits", "chmc_get_list_entry_bits", "chmc_set_list_check", "chmc_get_list_check", "chmc_set_speculation", "chmc_get_speculation", "chmc_set_list_start_rebuild", "chmc_get_list_start_rebuild", "chmc_set_trajectory", "chmc_get_trajectory",
     "chmc_surrogate_set_params", "chmc_surrogate_get_params",
     "chmc_surrogate_eval", "chmc_surrogate_features", "chmc_surrogate_reserve", "chmc_surrogate_collect",
     "chmc_surrogate_training_data", "chmc_surrogate_loss_grad", "chmc_surrogate_train",
@@ -558,6 +559,12 @@ class Engine:
     def time_list(self, reps=20):
         a, b = _D(), _D()
         self._ck(lib().chmc_time_list(self._h, int(reps), C.byref(a), C.byref(b)), "time_list")
+        return a.value, b.value
+
+    def time_surrogate(self, reps=20):
+        """(ms per descriptor + MLP launch, ms per surrogate force sweep) at the current positions."""
+        a, b = _D(), _D()
+        self._ck(lib().chmc_time_surrogate(self._h, int(reps), C.byref(a), C.byref(b)), "time_surrogate")
         return a.value, b.value
 
     def time_force(self, reps=20):

@@ -2350,6 +2350,38 @@ extern "C" int chmc_get_list_lanes(const chmc_ctx* c, int* lanes)
     return CHMC_OK;
 }
 
+// the surrogate's force evaluation on the lists: `reps` descriptor + MLP launches, then `reps` force
+// sweeps with kick and drift at dt = 0 (positions and momenta unchanged, so the lists stay valid)
+extern "C" int chmc_time_surrogate(chmc_ctx* c, int reps, double* ms_desc_mlp, double* ms_sweep)
+{
+    CHECK_CTX(c);
+    if (reps < 1) return CHMC_E_ARG;
+    if (fp64_md(c)) return fail(c, CHMC_E_STATE, "time_surrogate: surrogate MD runs on the FP32 list path");
+    if (!c->surr_params) return fail(c, CHMC_E_STATE, "time_surrogate: surrogate parameters not set");
+    int rc = surr_geometry(c);
+    if (rc || (rc = list_ensure(c)) || (rc = flush_drift(c)) || (rc = list_import(c)) || (rc = list_invalidate(c))) return rc;
+    HIPCHK(c, list_rebuild(c));
+    HIPCHK(c, launch_lsurr_desc_mlp(c->v, c->sv, c->lpar, c->sg, nullptr, nullptr, c->stream));     // warm-up
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; r++) HIPCHK(c, launch_lsurr_desc_mlp(c->v, c->sv, c->lpar, c->sg, nullptr, nullptr, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    for (int r = 0; r < reps; r++) {
+        HIPCHK(c, launch_lsurr_sweep(c->v, c->sg, LJ_FORCE | LJ_KICK | LJ_DRIFT, c->lpar, 0.0, 0.0, true, c->stream));
+        c->lpar ^= 1;
+    }
+    HIPCHK(c, hipEventRecord(c->ev2, c->stream));
+    if ((rc = list_export(c, false))) return rc;
+    HIPCHK(c, hipEventSynchronize(c->ev2));
+    float a = 0.f, b = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&a, c->ev0, c->ev1));
+    HIPCHK(c, hipEventElapsedTime(&b, c->ev1, c->ev2));
+    c->force_kind = 0;
+    if (ms_desc_mlp) *ms_desc_mlp = a / reps;
+    if (ms_sweep) *ms_sweep = b / reps;
+    if ((rc = surr_check_flag(c))) return rc;
+    return list_check_overflow(c);
+}
+
 extern "C" int chmc_time_list(chmc_ctx* c, int reps, double* ms_sweep, double* ms_rebuild)
 {
     CHECK_CTX(c);

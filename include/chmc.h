@@ -271,6 +271,11 @@ int chmc_list_stats(chmc_ctx* ctx, int chain, long long* entries, long long* nin
 /* average device time (ms) of the list sweep with kick + drift (the MD step of the trajectory) and of
  * one rebuild, over reps, at the current positions (the positions do not change: dt = 0) */
 int chmc_time_list(chmc_ctx* ctx, int reps, double* ms_sweep, double* ms_rebuild);
+/* average device time (ms) of the surrogate's two force-evaluation launches on the lists: descriptors +
+ * MLP (forward and input gradient) and the force sweep with kick + drift, over reps each, at the current
+ * positions (dt = 0: nothing moves).  Needs surrogate parameters (chmc_surrogate_set_params or training).
+ * No reference counterpart (the surrogate has none). */
+int chmc_time_surrogate(chmc_ctx* ctx, int reps, double* ms_desc_mlp, double* ms_sweep);
 /* shape of the list sweep: 4, 8 or 16 lanes per atom (first unconditional batches of 16, 14 and 7 list
  * entries per lane), or 810 / 811 / 812 / 816 (8 lanes, batches of 10 / 11 / 12 / 16 entries); 0 runs
  * the autotuner (times each at the current positions and keeps the fastest) */

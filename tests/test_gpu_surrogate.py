@@ -216,6 +216,31 @@ def test_surrogate_hmc_graph_matches_direct(chmc, trained):
     assert np.array_equal(r1, r2)
 
 
+def test_time_surrogate_leaves_the_state(chmc, trained):
+    """chmc_time_surrogate times the descriptor + MLP launch and the surrogate force sweep at dt = 0:
+    positive times, positions and momenta unchanged bitwise, and a surrogate run after it equal to one
+    from the same positions and counters set through the API (both on lists built at those positions);
+    without surrogate parameters it is an error."""
+    t, _ = trained
+    e = chmc.Engine(chmc.Chain(N=N, L=L, cutoff=RC, beta=1 / T), atom_id_order=True)
+    e.init_lattice(13, 0.05)
+    with pytest.raises(chmc.ChmcError):
+        e.time_surrogate(2)
+    e.hmc(30, 31, chmc.LEAPFROG, DT, STEPS, seed=24)
+    e.surrogate_set_params(t.surrogate_params())
+    x0, p0, t0 = e.positions(), e.momenta(), e.trajectory()
+    ms_desc, ms_sweep = e.time_surrogate(5)
+    assert ms_desc > 0 and ms_sweep > 0
+    assert np.array_equal(e.positions(), x0) and np.array_equal(e.momenta(), p0) and e.trajectory() == t0
+    r1, _ = e.hmc(4, 0, chmc.LEAPFROG, DT, STEPS, seed=79, surrogate=True)
+    e.set_positions(x0)
+    e.set_trajectory(t0)
+    r2, _ = e.hmc(4, 0, chmc.LEAPFROG, DT, STEPS, seed=79, surrogate=True)
+    e.close()
+    assert np.all(r1["surrogate"] == 1)
+    assert np.array_equal(r1, r2)
+
+
 def test_fallback_switches_to_lj(chmc, trained):
     e, _ = trained
     r, _ = e.hmc(25, 0, chmc.LEAPFROG, DT, STEPS, seed=78, surrogate=True, fallback_window=10, fallback_acc=1.01)
